@@ -1,0 +1,273 @@
+/*
+ * mcgb.h — C ABI of the MI355X-native Gay-Berne checkerboard Monte Carlo library (libmcgb.so).
+ *
+ * Plain C: no C++ or torch types cross this boundary.  All device memory is owned by the
+ * opaque context; callers pass host arrays that are copied in and out.
+ *
+ * Each entry point names the reference interface it replaces (MCnptColloid.c file:line).
+ * Return convention: 0 on success, a negative MCGB_ERR_* code on failure; the text of the last
+ * error is available from mcgb_last_error().  The reference prints and calls exit(1) instead
+ * (e.g. MCnptColloid.c:3862-3866, :4100-4127); the C driver maps any error to that behaviour.
+ *
+ * Threading: one context is driven by one host thread; calls on the same context are not
+ * thread-safe.  Each context owns one HIP stream at the default priority.
+ */
+#ifndef MCGB_H
+#define MCGB_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MCGB_OK            0
+#define MCGB_ERR_ARG      -1   /* invalid argument */
+#define MCGB_ERR_IO       -2   /* file could not be opened / read */
+#define MCGB_ERR_PARAM    -3   /* parameter file: missing key or malformed value */
+#define MCGB_ERR_HIP      -4   /* HIP runtime error (including "no GPU") */
+#define MCGB_ERR_NOMEM    -5   /* allocation failed */
+#define MCGB_ERR_GEOMETRY -6   /* box too small for the 8-sublattice cell grid, bad move size */
+#define MCGB_ERR_STATE    -7   /* state not set, overlap with the colloid core, NaN energy */
+#define MCGB_ERR_CAPACITY -8   /* a fixed-size buffer limit was exceeded */
+
+/* Number of keys the reference's loader requires (MCnptColloid.c:3937, :4100). */
+#define MCGB_NUM_REQUIRED_KEYS 23
+
+/*
+ * Run parameters.  The first 23 fields are the reference's parametersFile.txt keys with the
+ * reference's names and C types (globals MCnptColloid.c:96-145, parsed at :3952-4089).
+ * The remaining fields are optional keys that the reference does not have; they take the
+ * defaults set by mcgb_params_default() when absent from the file.
+ */
+typedef struct mcgb_params {
+    int      mc_ensemble_type;              /* 1 = NVT, 2 = NPT (ln V moves)              */
+    int      num_particles;
+    double   system_density;
+    double   system_temperature;
+    double   system_pressure;
+    double   gb_sigma_sidetoside;           /* sigma_0                                    */
+    double   gb_epsilon0;
+    double   gb_kappa;
+    double   gb_kappa_prime;
+    double   gb_mu_exponent;
+    double   gb_nu_exponent;
+    float    max_displacement;              /* full width: step = d*(U-1/2) per axis       */
+    float    max_rotation;                  /* |d_rot * v| with v on the unit sphere      */
+    int      initial_configuration_type;
+    double   cutoff_radius;
+    double   verlet_radius;                 /* read and kept; the cell list replaces lists */
+    int      total_mc_cycles;               /* driver: number of sweeps                    */
+    int      configuration_save_interval;
+    int      statistics_print_interval;
+    int      colloid_wall_option;           /* 1 = colloid at the box centre               */
+    double   colloid_interaction_epsilon;   /* read, unused (as in the reference)          */
+    double   colloid_interaction_amplitude; /* read, unused (as in the reference)          */
+    double   colloid_radius;
+
+    /* ---- optional keys (no counterpart in the reference) ---- */
+    uint64_t seed;                          /* Philox key; "seed"                          */
+    int      precision;                     /* 32: FP32 pair math, 64: FP64; "precision"   */
+    double   npt_max_dlnv;                  /* max |d ln V| per volume move; "npt_max_dlnv" */
+    int      energy_check_interval;         /* sweeps between FP64 re-summations, 0 = off  */
+    double   lattice_jitter;                /* init: uniform jitter amplitude of sites     */
+    double   colloid_exclusion_radius;      /* init: sites closer than this are removed    */
+
+    uint32_t keys_read;                     /* bit i set: required key i was read          */
+} mcgb_params;
+
+/* Outcome of one trial move (trace records). */
+#define MCGB_MOVE_EMPTY   0   /* active cell had no particle: no move made                 */
+#define MCGB_MOVE_LEFT    1   /* trial position left the active cell: rejected             */
+#define MCGB_MOVE_HARD    2   /* trial position inside the colloid core: rejected          */
+#define MCGB_MOVE_REJECT  3   /* Metropolis rejection                                      */
+#define MCGB_MOVE_ACCEPT  4   /* accepted                                                  */
+
+/* One trial move, as emitted by mcgb_sweep_traced() (32 bytes). */
+typedef struct mcgb_move_trace {
+    int32_t pid;      /* particle id moved (-1 for an empty cell)                              */
+    int32_t outcome;  /* MCGB_MOVE_*                                                           */
+    int32_t dcap;     /* change of the number of capped GB pairs (U = 1e9, MCnptColloid.c:2670) */
+    int32_t dccap;    /* change of the number of capped colloid terms (U = 1e8, :1778, :1833)   */
+    double  dfin;     /* finite part of dE                                                     */
+    double  scale;    /* sum of |finite pair terms| of both states                              */
+} mcgb_move_trace;
+
+/* Per-replica running statistics. */
+typedef struct mcgb_stats {
+    uint64_t sweeps;
+    uint64_t attempted;            /* every trial move, including the two early rejections  */
+    uint64_t accepted;
+    uint64_t rejected_left_cell;
+    uint64_t rejected_hard_core;
+    uint64_t candidates;           /* stencil candidates tested (both states)              */
+    uint64_t pairs;                /* GB pair energies inside the cutoff (both states)      */
+    uint64_t colloid_evals;        /* colloid-rod energies evaluated (both states)          */
+    uint64_t volume_attempted;
+    uint64_t volume_accepted;
+    int64_t  cap_pairs;            /* running number of capped GB pairs                     */
+    int64_t  cap_colloid;          /* running number of capped colloid terms                */
+    double   energy_finite;        /* running finite part of the energy (FP64)              */
+    double   energy;               /* cap_pairs*1e9 + cap_colloid*1e8 + energy_finite       */
+    double   box[3];
+    double   volume;
+    double   density;              /* N / (V - V_colloid), MCnptColloid.c:2799-2804          */
+    double   max_drift;            /* largest |running - recomputed| seen at a re-summation  */
+} mcgb_stats;
+
+typedef struct mcgb_ctx mcgb_ctx;
+
+/* ---------------------------------------------------------------- parameters (host only) */
+
+/* Defaults for every field (required keys zeroed, keys_read = 0). */
+void mcgb_params_default(mcgb_params *p);
+
+/* Replaces load_initial_conditions() (MCnptColloid.c:3933-4133) and read_next_parameter()
+ * (:3892-3906): same 23 keys, same "key value # comment" format, lines starting with '#' or
+ * empty are skipped, unknown keys print "Warning: Unknown parameter '<k>' in configuration
+ * file", a missing key or a malformed value is an error (MCGB_ERR_PARAM; the message lists the
+ * missing keys).  Optional keys are recognised in addition. */
+int mcgb_params_load(const char *path, mcgb_params *out);
+
+/* Derived constants of get_system_const() (MCnptColloid.c:2752-2783), computed in dependency
+ * order: out = {beta, 4 eps0, rc^2, kappa^2, chi, chi^2, kappa'^(1/mu), chi', sigma_ee,
+ * colloid cutoff 4 sigma_ee, R^2}. */
+int mcgb_params_derived(const mcgb_params *p, double out[11]);
+
+/* ---------------------------------------------------------------- context */
+
+/* Creates a context holding n_replicas independent systems; p points to n_replicas parameter
+ * blocks (each replica has its own N, T*, P*, density, colloid).  All replicas must share
+ * p[0].precision.  Replaces the allocation/broadcast set-up of main() (MCnptColloid.c:379-530)
+ * and block_init()/cell_init() (:3283-3388).  Fails with MCGB_ERR_HIP when no GPU is present. */
+int mcgb_create(const mcgb_params *p, int n_replicas, mcgb_ctx **out);
+void mcgb_destroy(mcgb_ctx *ctx);
+
+/* Text of the last error on this thread ("" if none). */
+const char *mcgb_last_error(void);
+
+/* Number of GPUs visible to HIP (0 when none); never fails. */
+int mcgb_device_count(void);
+
+/* Sets replica r's configuration: n particles (must equal its num_particles), positions in
+ * [0, box) (wrapped otherwise), orientations (normalised here), box lengths.  Positions are
+ * stored as 32-bit fixed-point fractions of the box (x = q * L / 2^32).  Recomputes the FP64
+ * total energy.  Replaces the scatter of main() (MCnptColloid.c:528-609). */
+int mcgb_set_state(mcgb_ctx *ctx, int replica, int n,
+                   const double *x, const double *y, const double *z,
+                   const double *ux, const double *uy, const double *uz,
+                   const double box[3]);
+
+/* Same with the fixed-point positions given directly (bit-exact parity tests). */
+int mcgb_set_state_fixed(mcgb_ctx *ctx, int replica, int n,
+                         const uint32_t *qx, const uint32_t *qy, const uint32_t *qz,
+                         const double *ux, const double *uy, const double *uz,
+                         const double box[3]);
+
+/* Copies replica r's configuration back, in particle-id order. */
+int mcgb_get_state(mcgb_ctx *ctx, int replica, int n,
+                   double *x, double *y, double *z,
+                   double *ux, double *uy, double *uz, double box[3]);
+int mcgb_get_state_fixed(mcgb_ctx *ctx, int replica, int n,
+                         uint32_t *qx, uint32_t *qy, uint32_t *qz,
+                         double *ux, double *uy, double *uz, double box[3]);
+
+/* ---------------------------------------------------------------- hot path */
+
+/* Runs n_sweeps checkerboard sweeps on every replica.  One sweep = random grid shift, cell-list
+ * rebuild, 8 sublattice passes of ceil(N/n_cells) trial moves per active cell, then (NPT) one
+ * ln V move.  Replaces the sub-cell loop of main() (MCnptColloid.c:643-706) with its TODO body
+ * filled by the move of attempt_monte_carlo_move() (:2445-2604), and build_neighbor_cells()
+ * (:5048-5264) / clean_neighbor_cells() (:5035-5046) by the device cell list. */
+int mcgb_sweep(mcgb_ctx *ctx, int n_sweeps);
+
+/* Runs ONE sweep on replica 0 only (other replicas untouched) and records every trial move.
+ * Record index = pass*(n_cells/8)*k + active_cell*k + move; capacity must be >= n_cells*k.
+ * info receives {shift_x, shift_y, shift_z, order[0..7], k, n_cells}. */
+int mcgb_sweep_traced(mcgb_ctx *ctx, mcgb_move_trace *trace, int capacity, uint32_t info[13]);
+
+/* Running statistics of replica r. */
+int mcgb_get_stats(mcgb_ctx *ctx, int replica, mcgb_stats *out);
+
+/* Resets the running counters of replica r (state and energy are kept). */
+int mcgb_reset_counters(mcgb_ctx *ctx, int replica);
+
+/* FP64 full recompute of the energy of replica r over the cell list (half stencil,
+ * deterministic reduction).  out = {total, cap_pairs, cap_colloid, finite}.  Replaces
+ * tot_ene() (MCnptColloid.c:1598-1622) / tot_ene_lstnb() (:1857-1882). */
+int mcgb_total_energy(mcgb_ctx *ctx, int replica, double out[4]);
+
+/* Re-sums the FP64 energy and replaces the running total with it (drift bound). */
+int mcgb_resum_energy(mcgb_ctx *ctx, int replica, double *drift);
+
+/* Nematic order parameter of replica r: P2 = 1.5 * largest eigenvalue of Q = <uu> - I/3
+ * (FP64 sums on the GPU, closed-form 3x3 eigen solve on the host) and the director.
+ * Replaces compute_liquid_crystal_order_parameters() (MCnptColloid.c:2117-2218). */
+int mcgb_order_parameter(mcgb_ctx *ctx, int replica, double *p2, double director[3]);
+
+/* Per-sweep series recorded since the last mcgb_series_clear(): E/N, P2, density.
+ * Returns the number of entries (<= max) or a negative error code. */
+int mcgb_series(mcgb_ctx *ctx, int replica, int max,
+                double *energy_per_particle, double *p2, double *density);
+int mcgb_series_clear(mcgb_ctx *ctx);
+
+/* Average device time per launch of the sweep-pass kernel, the cell-list rebuild and the whole
+ * sweep, measured with HIP events on the context's stream over the last mcgb_sweep() call. */
+int mcgb_kernel_times(mcgb_ctx *ctx, double *ms_pass, double *ms_rebuild, double *ms_sweep);
+
+/* Cell list for sweep index t of replica r without moving anything: grid shift, sublattice
+ * order, cell_start[n_cells+1] and the particle ids in cell order.  Replaces
+ * get_owner_block_index()/get_owner_cell_index() (MCnptColloid.c:4371-4399) and
+ * dist_batch_to_cells() (:4942-4964). */
+int mcgb_cell_list(mcgb_ctx *ctx, int replica, uint32_t t,
+                   int *cell_start, int cell_start_len, int *pid_sorted, int n,
+                   uint32_t shift[3], uint32_t order[8], int dims[3]);
+
+/* ---------------------------------------------------------------- kernels on batches */
+
+/* GB pair energy of n pairs on the GPU: U(e_i, e_j, dr) exactly as pij_GB()
+ * (MCnptColloid.c:2618-2681) with inverse norms and e_i.e_j formed as in GBij() (:1524-1535).
+ * precision 32 or 64.  capped[k] = 1 where the reference returns its 1e9 cap; u[k] then
+ * holds 1e9.  ei, ej, dr are n*3 interleaved. */
+int mcgb_pair_energy_batch(const mcgb_params *p, int n, const double *ei, const double *ej,
+                           const double *dr, int precision, double *u, int *capped);
+
+/* Colloid-rod energy of n rods on the GPU, as GBCollRod() (MCnptColloid.c:1758-1854);
+ * rel = rod centre minus colloid centre (n*3), e = rod orientation (n*3). */
+int mcgb_colloid_energy_batch(const mcgb_params *p, int n, const double *rel, const double *e,
+                              int precision, double *u, int *capped);
+
+/* Philox4x32-10 on the GPU: ctr n*4, key n*2, out n*4 (known-answer tests). */
+int mcgb_philox_batch(int n, const uint32_t *ctr, const uint32_t *key, uint32_t *out);
+
+/* ---------------------------------------------------------------- host utilities */
+
+/* Seeded lattice start: n_s^3 sites, n_s the smallest side that leaves >= N sites outside the
+ * exclusion sphere, at (i+1/2)*L/n_s plus uniform jitter in [-jitter, jitter]; N of the
+ * remaining sites chosen by a seeded shuffle; orientations uniform on the sphere.  Box volume
+ * V = N/rho + 4/3 pi R^3 (MCnptColloid.c:2843-2846), cubic.  Host only. */
+int mcgb_init_lattice(int n, double density, double colloid_radius, double exclusion_radius,
+                      double jitter, uint64_t seed,
+                      double *x, double *y, double *z, double *ux, double *uy, double *uz,
+                      double box[3]);
+
+/* Integrated autocorrelation time with Sokal's automatic window (smallest M >= c*tau(M)),
+ * tau = 1 + 2 sum_{t=1..M} rho(t).  ESS = n / tau.  Host only. */
+int mcgb_tau_int(const double *x, int n, double c, double *tau, int *window);
+
+/* Eigen-decomposition of a symmetric 3x3 matrix q = {xx, yy, zz, xy, xz, yz}: eigenvalues in
+ * ascending order, eigenvectors as columns of evec (row-major 3x3).  Host only. */
+int mcgb_eigen_sym3(const double q[6], double evals[3], double evec[9]);
+
+/* fortSphere.1 restart file (MCnptColloid.c:3193-3210 writer, :2786-2831 reader):
+ * header "N Lx Ly Lz R", then per particle 4 lines: position, 0 0 0, orientation, 0 0 0. */
+int mcgb_write_fort1(const char *path, int n, const double *x, const double *y, const double *z,
+                     const double *ux, const double *uy, const double *uz,
+                     const double box[3], double colloid_radius);
+int mcgb_read_fort1(const char *path, int n, double *x, double *y, double *z,
+                    double *ux, double *uy, double *uz, double box[3], double *colloid_radius);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MCGB_H */

@@ -1,0 +1,87 @@
+"""The oracle's checkerboard sweep: every trial move replayed against brute force, running
+energy against full recomputes, thread-count independence, NPT ideal-gas limit."""
+import numpy as np
+import pytest
+
+from oracle import oracle_ctypes as oc
+from tests.replay import replay_sweep
+from tests.systems import lattice_system
+
+
+def make(colloid=0.0, N=1300, rho=0.3, T=1.0, seed=5, nthreads=1):
+    c = oc.consts(temperature=T, colloid_radius=colloid, colloid_on=1 if colloid > 0 else 0)
+    q, u, L = lattice_system(N, rho, colloid_radius=colloid, seed=seed)
+    return c, oc.System(c, q, u, L, seed=1234 + seed, nthreads=nthreads)
+
+
+@pytest.mark.parametrize("colloid", [0.0, 3.0])
+def test_sweep_replay_brute_force(colloid):
+    c, s = make(colloid)
+    for _ in range(2):                        # relax the lattice start a little
+        s.sweep()
+    q0 = [a.copy() for a in s.q]
+    u0 = [a.copy() for a in s.u]
+    t = s.t
+    tr, info, cnt = s.sweep(trace=True)
+    qf, uf, st = replay_sweep(c, q0, u0, s.L, s.seed, 0, t, tr, info)
+    assert st["checked"] > 500 and st["accepted"] > 50
+    for d in range(3):
+        assert np.array_equal(qf[d], s.q[d])
+        np.testing.assert_allclose(uf[d], s.u[d], rtol=0, atol=1e-13)
+    outcomes = np.bincount(tr["outcome"], minlength=5)
+    assert outcomes[1] > 0 and outcomes[3] > 0 and outcomes[4] > 0
+    if colloid:
+        assert cnt[6] > 0
+
+
+@pytest.mark.parametrize("colloid", [0.0, 3.0])
+def test_running_energy_matches_recompute(colloid):
+    c, s = make(colloid)
+    for _ in range(4):
+        s.sweep()
+    e = s.total_energy()
+    assert s.energy[0] == e[0] and s.energy[1] == e[1]
+    assert s.energy[2] == pytest.approx(e[2], rel=1e-10, abs=1e-9)
+    x = s.positions()
+    eb = np.empty(4)
+    import ctypes as C
+    oc.lib().orc_total_energy_brute(C.byref(c), s.N, *(oc._d(np.ascontiguousarray(a)) for a in x),
+                                    *(oc._d(a) for a in s.u), oc._d(s.L), oc._d(eb))
+    assert eb[0] == e[0] and eb[1] == e[1] and eb[3] == e[3]
+    assert eb[2] == pytest.approx(e[2], rel=1e-10, abs=1e-9)
+
+
+def test_thread_count_does_not_change_results():
+    _, s1 = make(3.0, nthreads=1)
+    _, s4 = make(3.0, nthreads=4)
+    for _ in range(3):
+        s1.sweep()
+        s4.sweep()
+    for d in range(3):
+        assert np.array_equal(s1.q[d], s4.q[d])
+        assert np.array_equal(s1.u[d], s4.u[d])
+    assert np.array_equal(s1.energy, s4.energy)
+
+
+def test_npt_ideal_gas_volume():
+    # eps0 = 0 and tiny sigma0: U = 0, so ln V sampling gives <V> = (N+1)/(beta P)
+    N, T, target = 200, 1.0, 8000.0
+    P = (N + 1) / target
+    c = oc.consts(epsilon0=0.0, sigma0=0.01, cutoff=3.0, temperature=T, pressure=P,
+                  max_displacement=0.5)
+    rng = np.random.default_rng(11)
+    L = np.array([20.0] * 3)
+    q = [rng.integers(0, 2 ** 32, size=N, dtype=np.uint64).astype(np.uint32) for _ in range(3)]
+    u = rng.normal(size=(3, N))
+    u /= np.linalg.norm(u, axis=0)
+    s = oc.System(c, q, [np.ascontiguousarray(a) for a in u], L, seed=99, npt=True,
+                  npt_max_dlnv=0.08)
+    V = []
+    for _ in range(6000):
+        s.sweep()
+        V.append(np.prod(s.L))
+    V = np.array(V[500:])
+    tau = 1 + 2 * sum(np.corrcoef(V[:-t], V[t:])[0, 1] for t in range(1, 60))
+    err = V.std() * np.sqrt(tau / len(V))
+    assert abs(V.mean() - target) < 4 * err + 0.005 * target
+    assert s.counters[8] > 0.2 * s.counters[7]

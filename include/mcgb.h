@@ -72,6 +72,7 @@ typedef struct mcgb_params {
     int      energy_check_interval;         /* sweeps between FP64 re-summations, 0 = off  */
     double   lattice_jitter;                /* init: uniform jitter amplitude of sites     */
     double   colloid_exclusion_radius;      /* init: sites closer than this are removed    */
+    int      stage_capacity;                /* particles staged in LDS per stencil, 0: auto */
 
     uint32_t keys_read;                     /* bit i set: required key i was read          */
 } mcgb_params;

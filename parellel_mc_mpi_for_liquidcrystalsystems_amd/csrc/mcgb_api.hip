@@ -1,0 +1,1004 @@
+// mcgb_api.hip — the C ABI of libmcgb (include/mcgb.h): context, device memory, the launch
+// sequence of a sweep, parity hooks.  Kernels live in mcgb_kernels.hpp.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "mcgb_internal.h"
+#include "mcgb_kernels.hpp"
+
+using namespace mcgb;
+
+#define HIPCHK(x)                                                                              \
+    do {                                                                                       \
+        hipError_t e_ = (x);                                                                   \
+        if (e_ != hipSuccess) {                                                                \
+            mcgb_set_error("%s failed: %s (%s:%d)", #x, hipGetErrorString(e_), __FILE__, __LINE__); \
+            return MCGB_ERR_HIP;                                                               \
+        }                                                                                      \
+    } while (0)
+
+namespace {
+
+constexpr double kPi = 3.14159265358979323846;
+constexpr int kSeriesCapDefault = 1 << 16;
+
+struct Events {
+    std::vector<hipEvent_t> ev;
+    int get(int i, hipEvent_t* out) {
+        while ((int)ev.size() <= i) {
+            hipEvent_t e;
+            if (hipEventCreate(&e) != hipSuccess) return -1;
+            ev.push_back(e);
+        }
+        *out = ev[i];
+        return 0;
+    }
+    ~Events() {
+        for (auto e : ev) (void)hipEventDestroy(e);
+    }
+};
+
+}  // namespace
+
+struct mcgb_ctx {
+    int nrep = 0, prec = 32;
+    size_t rsize = 4;
+    std::vector<mcgb_params> P;
+    std::vector<RepConst> hC;
+    std::vector<int> blk_rep;
+    std::vector<int> state_set;
+    int nblk = 0, Ntot = 0, Nmax = 0, ncell_tot = 0, ncs_tot = 0, ncell_max = 0;
+    int cap = 0;
+    size_t lds = 0;
+    int resum_interval = 100;
+    int series_cap = kSeriesCapDefault;
+    bool any_npt = false;
+    hipStream_t st = nullptr;
+    int device = 0;
+    RepConst* dC = nullptr;
+    RepDyn* dD = nullptr;
+    int* dBlk = nullptr;
+    uint32_t* q[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};
+    void* u[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};
+    int* pid[2] = {nullptr, nullptr};
+    int cur = 0;
+    int *cell_id = nullptr, *tmp = nullptr, *count = nullptr, *cell_start = nullptr;
+    int *cell_start2 = nullptr, *list2 = nullptr;
+    CellAcc* acc = nullptr;
+    double* efin = nullptr;
+    int *ecap = nullptr, *eccap = nullptr, *ehard = nullptr;
+    double* obs = nullptr;
+    double* series = nullptr;
+    mcgb_move_trace* dtrace = nullptr;
+    size_t dtrace_cap = 0;
+    std::vector<void*> allocs;
+    Events events;
+    bool timing = true;
+    double ms_pass = 0, ms_rebuild = 0, ms_sweep = 0;
+};
+
+namespace {
+
+int dalloc(mcgb_ctx* c, void** p, size_t bytes) {
+    if (bytes == 0) bytes = 16;
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) {
+        mcgb_set_error("hipMalloc(%zu bytes) failed: %s", bytes, hipGetErrorString(e));
+        return MCGB_ERR_NOMEM;
+    }
+    c->allocs.push_back(*p);
+    return MCGB_OK;
+}
+
+template <typename T>
+int dalloc_t(mcgb_ctx* c, T** p, size_t n) {
+    return dalloc(c, (void**)p, n * sizeof(T));
+}
+
+double box_length(const mcgb_params& p) {
+    double R = p.colloid_wall_option == 1 ? p.colloid_radius : 0.0;
+    double V = p.num_particles / p.system_density + (4.0 / 3.0) * kPi * R * R * R;
+    return cbrt(V);
+}
+
+// Constants of one replica (get_system_const, MCnptColloid.c:2752-2783, dependency order).
+void fill_consts(const mcgb_params& p, RepConst* C) {
+    double dv[11];
+    mcgb_params_derived(&p, dv);
+    C->sigma0 = p.gb_sigma_sidetoside;
+    C->four_eps0 = dv[1];
+    C->chi = dv[4];
+    C->chi2 = dv[5];
+    C->chip = dv[7];
+    C->chip2 = dv[7] * dv[7];
+    C->mu = p.gb_mu_exponent;
+    C->nu = p.gb_nu_exponent;
+    C->rc = p.cutoff_radius;
+    C->rc2 = dv[2];
+    C->temperature = p.system_temperature;
+    C->beta = dv[0];
+    C->pressure = p.system_pressure;
+    C->dmax = (double)p.max_displacement;
+    C->drot = (double)p.max_rotation;
+    C->colloid_on = p.colloid_wall_option == 1 ? 1 : 0;
+    C->R = p.colloid_radius;
+    C->R2 = dv[10];
+    C->colloid_cut = dv[9];
+    C->kappa_prime = p.gb_kappa_prime;
+    C->one_minus_chi = 1.0 - dv[4];
+    C->one_minus_chip = 1.0 - dv[7];
+    C->npt_max_dlnv = p.npt_max_dlnv;
+    C->npt = p.mc_ensemble_type == 2 ? 1 : 0;
+    C->vcoll = C->colloid_on ? (4.0 / 3.0) * kPi * p.colloid_radius * p.colloid_radius * p.colloid_radius : 0.0;
+    C->f_sigma0 = (float)C->sigma0;
+    C->f_four_eps0 = (float)C->four_eps0;
+    C->f_chi = (float)C->chi;
+    C->f_chi2 = (float)C->chi2;
+    C->f_chip = (float)C->chip;
+    C->f_chip2 = (float)C->chip2;
+    C->f_mu = (float)C->mu;
+    C->f_half_nu = (float)(0.5 * C->nu);
+    C->f_rc2 = (float)C->rc2;
+    C->f_drot = (float)C->drot;
+}
+
+int check_params(const mcgb_params& p, int r) {
+    if (p.num_particles <= 0 || p.system_density <= 0.0 || p.system_temperature <= 0.0 ||
+        p.cutoff_radius <= 0.0 || p.gb_sigma_sidetoside <= 0.0 || p.gb_mu_exponent == 0.0 ||
+        p.gb_kappa <= 0.0 || p.gb_kappa_prime <= 0.0) {
+        mcgb_set_error("replica %d: num_particles, density, temperature, cutoff, sigma0, kappa, "
+                       "kappa' must be > 0 and mu != 0", r);
+        return MCGB_ERR_ARG;
+    }
+    if (p.max_displacement < 0.0f || p.max_rotation < 0.0f) {
+        mcgb_set_error("replica %d: negative move size", r);
+        return MCGB_ERR_ARG;
+    }
+    return MCGB_OK;
+}
+
+template <typename Real>
+int enqueue_rebuild(mcgb_ctx* c) {
+    const int src = c->cur, dst = 1 - c->cur;
+    const dim3 gp((c->Nmax + 255) / 256, c->nrep);
+    k_zero<<<256, 256, 0, c->st>>>(c->count, c->ncell_tot);
+    k_count<<<gp, 256, 0, c->st>>>(c->dC, c->dD, c->q[src][0], c->q[src][1], c->q[src][2],
+                                   c->cell_id, c->count, 0);
+    k_scan<<<c->nrep, 1024, 0, c->st>>>(c->dC, c->dD, c->count, c->cell_start, 0);
+    k_scatter<<<gp, 256, 0, c->st>>>(c->dC, c->dD, c->cell_id, c->count, c->cell_start, c->tmp, 0);
+    const dim3 gs((c->ncell_max + 3) / 4, c->nrep);
+    k_sort_gather<Real><<<gs, 256, 0, c->st>>>(
+        c->dC, c->dD, c->cell_start, c->tmp, c->pid[src], 0, 0, nullptr, c->q[src][0], c->q[src][1],
+        c->q[src][2], (const Real*)c->u[src][0], (const Real*)c->u[src][1], (const Real*)c->u[src][2],
+        c->q[dst][0], c->q[dst][1], c->q[dst][2], (Real*)c->u[dst][0], (Real*)c->u[dst][1],
+        (Real*)c->u[dst][2], c->pid[dst]);
+    c->cur = dst;
+    HIPCHK(hipGetLastError());
+    return MCGB_OK;
+}
+
+// FP64 energy of the current cell list; mode as in k_energy_reduce.
+template <typename Real>
+int enqueue_energy(mcgb_ctx* c, int mode) {
+    const int b = c->cur;
+    const dim3 ge((c->Nmax + 127) / 128, c->nrep);
+    const int* cs = mode == 2 ? c->cell_start2 : c->cell_start;
+    k_energy<Real><<<ge, 128, 0, c->st>>>(c->dC, c->dD, mode, c->q[b][0], c->q[b][1], c->q[b][2],
+                                          (const Real*)c->u[b][0], (const Real*)c->u[b][1],
+                                          (const Real*)c->u[b][2], cs, c->list2, c->efin, c->ecap,
+                                          c->eccap, c->ehard);
+    k_energy_reduce<<<c->nrep, 256, 0, c->st>>>(c->dC, c->dD, mode, c->efin, c->ecap, c->eccap, c->ehard);
+    HIPCHK(hipGetLastError());
+    return MCGB_OK;
+}
+
+template <typename Real>
+int enqueue_npt(mcgb_ctx* c) {
+    const int b = c->cur;
+    k_npt_trial<<<(c->nrep + 63) / 64, 64, 0, c->st>>>(c->dC, c->dD, c->nrep);
+    const dim3 gp((c->Nmax + 255) / 256, c->nrep);
+    k_zero<<<256, 256, 0, c->st>>>(c->count, c->ncell_tot);
+    k_count<<<gp, 256, 0, c->st>>>(c->dC, c->dD, c->q[b][0], c->q[b][1], c->q[b][2], c->cell_id, c->count, 1);
+    k_scan<<<c->nrep, 1024, 0, c->st>>>(c->dC, c->dD, c->count, c->cell_start2, 1);
+    k_scatter<<<gp, 256, 0, c->st>>>(c->dC, c->dD, c->cell_id, c->count, c->cell_start2, c->tmp, 1);
+    const dim3 gs((c->ncell_max + 3) / 4, c->nrep);
+    k_sort_gather<Real><<<gs, 256, 0, c->st>>>(
+        c->dC, c->dD, c->cell_start2, c->tmp, c->pid[b], 1, 1, c->list2, nullptr, nullptr, nullptr,
+        nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
+    HIPCHK(hipGetLastError());
+    return enqueue_energy<Real>(c, 2);
+}
+
+template <typename Real>
+int enqueue_observe(mcgb_ctx* c, int advance, int record) {
+    const int b = c->cur;
+    k_observe<Real><<<dim3(64, c->nrep), 256, 0, c->st>>>(c->dC, (const Real*)c->u[b][0],
+                                                          (const Real*)c->u[b][1],
+                                                          (const Real*)c->u[b][2], c->obs);
+    k_observe_reduce<<<(c->nrep + 63) / 64, 64, 0, c->st>>>(c->dC, c->dD, c->nrep, c->obs, c->series,
+                                                            record ? c->series_cap : 0, advance);
+    HIPCHK(hipGetLastError());
+    return MCGB_OK;
+}
+
+template <typename Real>
+int enqueue_sweeps(mcgb_ctx* c, int n_sweeps, mcgb_move_trace* trace) {
+    int rc;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    const bool tm = c->timing;
+    double tp = 0, tr = 0, tsw = 0;
+    int evi = 0;
+    std::vector<int> marks;   // event indices: per sweep 11 marks
+    for (int s = 0; s < n_sweeps; ++s) {
+        if (tm) {
+            if (c->events.get(evi, &e0)) return MCGB_ERR_HIP;
+            HIPCHK(hipEventRecord(e0, c->st));
+        }
+        k_begin<<<(c->nrep + 63) / 64, 64, 0, c->st>>>(c->dC, c->dD, c->nrep, c->resum_interval);
+        if ((rc = enqueue_rebuild<Real>(c))) return rc;
+        if (tm) {
+            if (c->events.get(evi + 1, &e1)) return MCGB_ERR_HIP;
+            HIPCHK(hipEventRecord(e1, c->st));
+        }
+        PassArgs P;
+        P.RC = c->dC; P.RD = c->dD; P.blk_rep = c->dBlk;
+        const int b = c->cur;
+        P.qx = c->q[b][0]; P.qy = c->q[b][1]; P.qz = c->q[b][2];
+        P.ux = c->u[b][0]; P.uy = c->u[b][1]; P.uz = c->u[b][2];
+        P.pid = c->pid[b];
+        P.cell_start = c->cell_start;
+        P.acc = c->acc;
+        P.trace = trace;
+        P.cap = c->cap;
+        for (int p = 0; p < 8; ++p) {
+            P.pass = p;
+            k_pass<Real><<<c->nblk, 64, c->lds, c->st>>>(P);
+            if (tm) {
+                hipEvent_t ep;
+                if (c->events.get(evi + 2 + p, &ep)) return MCGB_ERR_HIP;
+                HIPCHK(hipEventRecord(ep, c->st));
+            }
+        }
+        HIPCHK(hipGetLastError());
+        k_end_moves<<<c->nrep, 256, 0, c->st>>>(c->dC, c->dD, c->acc);
+        if ((rc = enqueue_energy<Real>(c, 1))) return rc;
+        if (c->any_npt && (rc = enqueue_npt<Real>(c))) return rc;
+        if ((rc = enqueue_observe<Real>(c, 1, 1))) return rc;
+        if (tm) {
+            hipEvent_t ee;
+            if (c->events.get(evi + 10, &ee)) return MCGB_ERR_HIP;
+            HIPCHK(hipEventRecord(ee, c->st));
+            marks.push_back(evi);
+            evi += 11;
+        }
+    }
+    HIPCHK(hipStreamSynchronize(c->st));
+    if (tm && !marks.empty()) {
+        for (int m : marks) {
+            float ms;
+            HIPCHK(hipEventElapsedTime(&ms, c->events.ev[m], c->events.ev[m + 1]));
+            tr += ms;
+            for (int p = 0; p < 8; ++p) {
+                HIPCHK(hipEventElapsedTime(&ms, c->events.ev[m + 1 + p], c->events.ev[m + 2 + p]));
+                tp += ms;
+            }
+            HIPCHK(hipEventElapsedTime(&ms, c->events.ev[m], c->events.ev[m + 10]));
+            tsw += ms;
+        }
+        const double ns = (double)marks.size();
+        c->ms_rebuild = tr / ns;
+        c->ms_pass = tp / (8.0 * ns);
+        c->ms_sweep = tsw / ns;
+    }
+    return MCGB_OK;
+}
+
+int read_dyn(mcgb_ctx* c, std::vector<RepDyn>& D) {
+    D.resize(c->nrep);
+    HIPCHK(hipMemcpyAsync(D.data(), c->dD, sizeof(RepDyn) * c->nrep, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    return MCGB_OK;
+}
+
+int write_dyn(mcgb_ctx* c, const std::vector<RepDyn>& D) {
+    HIPCHK(hipMemcpyAsync(c->dD, D.data(), sizeof(RepDyn) * c->nrep, hipMemcpyHostToDevice, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    return MCGB_OK;
+}
+
+int check_ctx(mcgb_ctx* c, int replica) {
+    if (!c) { mcgb_set_error("null context"); return MCGB_ERR_ARG; }
+    if (replica < 0 || replica >= c->nrep) {
+        mcgb_set_error("replica %d out of range [0, %d)", replica, c->nrep);
+        return MCGB_ERR_ARG;
+    }
+    HIPCHK(hipSetDevice(c->device));
+    return MCGB_OK;
+}
+
+// Cell list with zero shift for the current boxes, then the FP64 energy (mode 0 or 3).
+int recompute(mcgb_ctx* c, int mode) {
+    k_grid_g0<<<(c->nrep + 63) / 64, 64, 0, c->st>>>(c->dC, c->dD, c->nrep);
+    int rc = c->prec == 64 ? enqueue_rebuild<double>(c) : enqueue_rebuild<float>(c);
+    if (rc) return rc;
+    rc = c->prec == 64 ? enqueue_energy<double>(c, mode) : enqueue_energy<float>(c, mode);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(c->st));
+    return MCGB_OK;
+}
+
+}  // namespace
+
+// ====================================================================== C ABI
+
+extern "C" {
+
+int mcgb_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+void mcgb_destroy(mcgb_ctx* c) {
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    if (c->st) (void)hipStreamSynchronize(c->st);
+    for (void* p : c->allocs) (void)hipFree(p);
+    if (c->st) (void)hipStreamDestroy(c->st);
+    delete c;
+}
+
+int mcgb_create(const mcgb_params* p, int n_replicas, mcgb_ctx** out) {
+    if (!p || !out || n_replicas <= 0 || n_replicas > kMaxReplicas) {
+        mcgb_set_error("mcgb_create: need 1..%d parameter blocks and an output pointer", kMaxReplicas);
+        return MCGB_ERR_ARG;
+    }
+    *out = nullptr;
+    int ndev = 0;
+    hipError_t he = hipGetDeviceCount(&ndev);
+    if (he != hipSuccess || ndev == 0) {
+        mcgb_set_error("no HIP device available (%s)", hipGetErrorString(he));
+        return MCGB_ERR_HIP;
+    }
+    const int prec = p[0].precision;
+    if (prec != 32 && prec != 64) {
+        mcgb_set_error("precision must be 32 or 64 (got %d)", prec);
+        return MCGB_ERR_ARG;
+    }
+    for (int r = 0; r < n_replicas; ++r) {
+        int rc = check_params(p[r], r);
+        if (rc) return rc;
+        if (p[r].precision != prec) {
+            mcgb_set_error("replica %d: all replicas must share one precision", r);
+            return MCGB_ERR_ARG;
+        }
+    }
+    mcgb_ctx* c = new mcgb_ctx();
+    c->nrep = n_replicas;
+    c->prec = prec;
+    c->rsize = prec == 64 ? 8 : 4;
+    c->P.assign(p, p + n_replicas);
+    c->resum_interval = p[0].energy_check_interval;
+    HIPCHK(hipGetDevice(&c->device));
+    HIPCHK(hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking));
+
+    // ---- layout of every replica
+    c->hC.resize(n_replicas);
+    c->state_set.assign(n_replicas, 0);
+    std::vector<RepDyn> D(n_replicas);
+    double occ_max = 0.0;
+    for (int r = 0; r < n_replicas; ++r) {
+        RepConst& C = c->hC[r];
+        memset(&C, 0, sizeof(C));
+        fill_consts(p[r], &C);
+        const mcgb_params& pr = p[r];
+        double L = box_length(pr);
+        double Lcap = C.npt ? L * 1.3 : L;
+        int ncell = 1;
+        for (int d = 0; d < 3; ++d) {
+            int n = mcgb_cell_dim(Lcap, pr.cutoff_radius);
+            if (n == 0) {
+                mcgb_set_error("replica %d: box %.4f is below 4 cells of the cutoff %.4f (8-sublattice "
+                               "grid impossible)", r, L, pr.cutoff_radius);
+                mcgb_destroy(c);
+                return MCGB_ERR_GEOMETRY;
+            }
+            C.ncap[d] = n;
+            ncell *= n;
+        }
+        if ((double)pr.max_displacement >= 0.5 * L) {
+            mcgb_set_error("replica %d: max_displacement must be below half the box", r);
+            mcgb_destroy(c);
+            return MCGB_ERR_GEOMETRY;
+        }
+        C.N = pr.num_particles;
+        C.part_off = c->Ntot;
+        C.cell_off = c->ncell_tot;
+        C.cs_off = c->ncs_tot;
+        C.act_off = c->nblk;
+        C.nact_cap = ncell / 8;
+        C.key0 = (uint32_t)pr.seed;
+        C.key1 = (uint32_t)(pr.seed >> 32) + (uint32_t)r;
+        c->Ntot += C.N;
+        c->Nmax = std::max(c->Nmax, C.N);
+        c->ncell_tot += ncell;
+        c->ncs_tot += ncell + 1;
+        c->ncell_max = std::max(c->ncell_max, ncell);
+        for (int b = 0; b < C.nact_cap; ++b) c->blk_rep.push_back(r);
+        c->nblk += C.nact_cap;
+        c->any_npt = c->any_npt || C.npt;
+        int n0 = mcgb_cell_dim(L, pr.cutoff_radius);
+        double occ = (double)C.N / ((double)n0 * n0 * n0);
+        occ_max = std::max(occ_max, occ);
+        RepDyn& Dr = D[r];
+        memset(&Dr, 0, sizeof(Dr));
+        for (int d = 0; d < 3; ++d) Dr.L[d] = L;
+    }
+    // LDS staging capacity: 1.25x the mean 3x3x3 stencil plus slack, within 64 KiB
+    const size_t per = 12 + 3 * c->rsize + 2;
+    const int capmax = (int)((65536 - 256 - 64 * (4 + 12 + 3 * c->rsize + 8) - 64) / per) & ~31;
+    int cap = (int)(27.0 * occ_max * 1.25 + 96.0);
+    cap = (cap + 31) & ~31;
+    if (p[0].stage_capacity > 0) cap = (p[0].stage_capacity + 31) & ~31;
+    cap = std::max(32, std::min(cap, capmax));
+    c->cap = cap;
+    c->lds = c->rsize == 8 ? PassLds<double>::bytes(cap) : PassLds<float>::bytes(cap);
+
+    // ---- device memory
+    int rc = 0;
+    const size_t N = c->Ntot;
+    for (int b = 0; b < 2 && !rc; ++b) {
+        for (int d = 0; d < 3 && !rc; ++d) rc = dalloc_t(c, &c->q[b][d], N);
+        for (int d = 0; d < 3 && !rc; ++d) rc = dalloc(c, &c->u[b][d], N * c->rsize);
+        if (!rc) rc = dalloc_t(c, &c->pid[b], N);
+    }
+    if (!rc) rc = dalloc_t(c, &c->cell_id, N);
+    if (!rc) rc = dalloc_t(c, &c->tmp, N);
+    if (!rc) rc = dalloc_t(c, &c->list2, N);
+    if (!rc) rc = dalloc_t(c, &c->count, c->ncell_tot);
+    if (!rc) rc = dalloc_t(c, &c->cell_start, c->ncs_tot);
+    if (!rc) rc = dalloc_t(c, &c->cell_start2, c->ncs_tot);
+    if (!rc) rc = dalloc_t(c, &c->acc, c->ncell_tot);
+    if (!rc) rc = dalloc_t(c, &c->efin, N);
+    if (!rc) rc = dalloc_t(c, &c->ecap, N);
+    if (!rc) rc = dalloc_t(c, &c->eccap, N);
+    if (!rc) rc = dalloc_t(c, &c->ehard, N);
+    if (!rc) rc = dalloc_t(c, &c->obs, (size_t)n_replicas * 64 * 6);
+    if (!rc) rc = dalloc_t(c, &c->series, (size_t)n_replicas * c->series_cap * kSeriesWidth);
+    if (!rc) rc = dalloc_t(c, &c->dC, n_replicas);
+    if (!rc) rc = dalloc_t(c, &c->dD, n_replicas);
+    if (!rc) rc = dalloc_t(c, &c->dBlk, c->nblk);
+    if (rc) { mcgb_destroy(c); return rc; }
+    if (hipMemcpy(c->dC, c->hC.data(), sizeof(RepConst) * n_replicas, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(c->dD, D.data(), sizeof(RepDyn) * n_replicas, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(c->dBlk, c->blk_rep.data(), sizeof(int) * c->nblk, hipMemcpyHostToDevice) != hipSuccess) {
+        mcgb_set_error("hipMemcpy of the replica tables failed");
+        mcgb_destroy(c);
+        return MCGB_ERR_HIP;
+    }
+    // placeholder configuration (spread points) until mcgb_set_state
+    {
+        std::vector<uint32_t> qq(N);
+        std::vector<int> ids(N);
+        const uint32_t mul[3] = {0x9E3779B9u, 0x85EBCA6Bu, 0xC2B2AE35u};
+        for (int r = 0; r < n_replicas; ++r)
+            for (int i = 0; i < c->hC[r].N; ++i) ids[c->hC[r].part_off + i] = i;
+        for (int d = 0; d < 3; ++d) {
+            for (size_t i = 0; i < N; ++i) qq[i] = (uint32_t)(ids[i] + 1) * mul[d];
+            if (hipMemcpy(c->q[0][d], qq.data(), N * 4, hipMemcpyHostToDevice) != hipSuccess) rc = MCGB_ERR_HIP;
+        }
+        std::vector<char> ub(N * c->rsize, 0);
+        for (size_t i = 0; i < N; ++i) {
+            if (c->rsize == 8) ((double*)ub.data())[i] = 1.0;
+            else ((float*)ub.data())[i] = 1.0f;
+        }
+        for (int d = 0; d < 3; ++d) {
+            std::vector<char> z(N * c->rsize, 0);
+            if (hipMemcpy(c->u[0][d], d == 2 ? ub.data() : z.data(), N * c->rsize, hipMemcpyHostToDevice) != hipSuccess)
+                rc = MCGB_ERR_HIP;
+        }
+        if (hipMemcpy(c->pid[0], ids.data(), N * 4, hipMemcpyHostToDevice) != hipSuccess) rc = MCGB_ERR_HIP;
+        if (rc) {
+            mcgb_set_error("initial upload failed");
+            mcgb_destroy(c);
+            return rc;
+        }
+    }
+    c->cur = 0;
+    *out = c;
+    return MCGB_OK;
+}
+
+static int set_state_impl(mcgb_ctx* c, int r, int n, const uint32_t* qx, const uint32_t* qy,
+                          const uint32_t* qz, const double* ux, const double* uy, const double* uz,
+                          const double box[3]) {
+    const RepConst& C = c->hC[r];
+    if (n != C.N) {
+        mcgb_set_error("replica %d holds %d particles, %d given", r, C.N, n);
+        return MCGB_ERR_ARG;
+    }
+    for (int d = 0; d < 3; ++d) {
+        if (!(box[d] > 0.0) || mcgb_cell_dim(box[d], c->P[r].cutoff_radius) == 0) {
+            mcgb_set_error("replica %d: box %.4f is below 4 cells of the cutoff", r, box[d]);
+            return MCGB_ERR_GEOMETRY;
+        }
+        if ((double)c->P[r].max_displacement >= 0.5 * box[d]) {
+            mcgb_set_error("replica %d: max_displacement must be below half the box", r);
+            return MCGB_ERR_GEOMETRY;
+        }
+    }
+    const int b = c->cur;
+    const size_t off = C.part_off;
+    std::vector<int> ids(n);
+    for (int i = 0; i < n; ++i) ids[i] = i;
+    HIPCHK(hipMemcpyAsync(c->q[b][0] + off, qx, n * 4, hipMemcpyHostToDevice, c->st));
+    HIPCHK(hipMemcpyAsync(c->q[b][1] + off, qy, n * 4, hipMemcpyHostToDevice, c->st));
+    HIPCHK(hipMemcpyAsync(c->q[b][2] + off, qz, n * 4, hipMemcpyHostToDevice, c->st));
+    HIPCHK(hipMemcpyAsync(c->pid[b] + off, ids.data(), n * 4, hipMemcpyHostToDevice, c->st));
+    std::vector<double> un[3];
+    for (int d = 0; d < 3; ++d) un[d].resize(n);
+    for (int i = 0; i < n; ++i) {
+        double nn = sqrt(ux[i] * ux[i] + uy[i] * uy[i] + uz[i] * uz[i]);
+        if (!(nn > 0.0) || !std::isfinite(nn)) {
+            mcgb_set_error("replica %d: orientation %d has zero or non-finite norm", r, i);
+            return MCGB_ERR_STATE;
+        }
+        un[0][i] = ux[i] / nn; un[1][i] = uy[i] / nn; un[2][i] = uz[i] / nn;
+    }
+    std::vector<float> uf(c->rsize == 4 ? n : 0);
+    for (int d = 0; d < 3; ++d) {
+        if (c->rsize == 8) {
+            HIPCHK(hipMemcpyAsync((double*)c->u[b][d] + off, un[d].data(), n * 8, hipMemcpyHostToDevice, c->st));
+            HIPCHK(hipStreamSynchronize(c->st));
+        } else {
+            for (int i = 0; i < n; ++i) uf[i] = (float)un[d][i];
+            HIPCHK(hipMemcpyAsync((float*)c->u[b][d] + off, uf.data(), n * 4, hipMemcpyHostToDevice, c->st));
+            HIPCHK(hipStreamSynchronize(c->st));
+        }
+    }
+    std::vector<RepDyn> D;
+    int rc = read_dyn(c, D);
+    if (rc) return rc;
+    RepDyn& Dr = D[r];
+    uint32_t t = Dr.t;
+    int sc = Dr.series_count;
+    memset(&Dr, 0, sizeof(Dr));
+    Dr.t = c->state_set[r] ? t : 0u;
+    Dr.series_count = sc;
+    for (int d = 0; d < 3; ++d) Dr.L[d] = box[d];
+    if ((rc = write_dyn(c, D))) return rc;
+    if ((rc = recompute(c, 0))) return rc;
+    c->state_set[r] = 1;
+    return MCGB_OK;
+}
+
+int mcgb_set_state_fixed(mcgb_ctx* c, int r, int n, const uint32_t* qx, const uint32_t* qy,
+                         const uint32_t* qz, const double* ux, const double* uy, const double* uz,
+                         const double box[3]) {
+    int rc = check_ctx(c, r);
+    if (rc) return rc;
+    if (!qx || !qy || !qz || !ux || !uy || !uz || !box) {
+        mcgb_set_error("mcgb_set_state_fixed: null array");
+        return MCGB_ERR_ARG;
+    }
+    return set_state_impl(c, r, n, qx, qy, qz, ux, uy, uz, box);
+}
+
+int mcgb_set_state(mcgb_ctx* c, int r, int n, const double* x, const double* y, const double* z,
+                   const double* ux, const double* uy, const double* uz, const double box[3]) {
+    int rc = check_ctx(c, r);
+    if (rc) return rc;
+    if (!x || !y || !z || !ux || !uy || !uz || !box || n <= 0) {
+        mcgb_set_error("mcgb_set_state: null array");
+        return MCGB_ERR_ARG;
+    }
+    std::vector<uint32_t> q[3];
+    const double* xs[3] = {x, y, z};
+    for (int d = 0; d < 3; ++d) {
+        q[d].resize(n);
+        for (int i = 0; i < n; ++i) {
+            if (!std::isfinite(xs[d][i])) {
+                mcgb_set_error("replica %d: position %d is not finite", r, i);
+                return MCGB_ERR_STATE;
+            }
+            q[d][i] = mcgb_to_fixed(xs[d][i], box[d]);
+        }
+    }
+    return set_state_impl(c, r, n, q[0].data(), q[1].data(), q[2].data(), ux, uy, uz, box);
+}
+
+int mcgb_get_state_fixed(mcgb_ctx* c, int r, int n, uint32_t* qx, uint32_t* qy, uint32_t* qz,
+                         double* ux, double* uy, double* uz, double box[3]) {
+    int rc = check_ctx(c, r);
+    if (rc) return rc;
+    const RepConst& C = c->hC[r];
+    if (n != C.N) {
+        mcgb_set_error("replica %d holds %d particles, %d requested", r, C.N, n);
+        return MCGB_ERR_ARG;
+    }
+    const int b = c->cur;
+    const size_t off = C.part_off;
+    std::vector<uint32_t> q[3];
+    std::vector<int> ids(n);
+    std::vector<double> ud[3];
+    std::vector<float> uf(n);
+    for (int d = 0; d < 3; ++d) {
+        q[d].resize(n);
+        ud[d].resize(n);
+        HIPCHK(hipMemcpyAsync(q[d].data(), c->q[b][d] + off, n * 4, hipMemcpyDeviceToHost, c->st));
+        if (c->rsize == 8) {
+            HIPCHK(hipMemcpyAsync(ud[d].data(), (double*)c->u[b][d] + off, n * 8, hipMemcpyDeviceToHost, c->st));
+            HIPCHK(hipStreamSynchronize(c->st));
+        } else {
+            HIPCHK(hipMemcpyAsync(uf.data(), (float*)c->u[b][d] + off, n * 4, hipMemcpyDeviceToHost, c->st));
+            HIPCHK(hipStreamSynchronize(c->st));
+            for (int i = 0; i < n; ++i) ud[d][i] = uf[i];
+        }
+    }
+    HIPCHK(hipMemcpyAsync(ids.data(), c->pid[b] + off, n * 4, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    for (int i = 0; i < n; ++i) {
+        int p = ids[i];
+        if (p < 0 || p >= n) {
+            mcgb_set_error("corrupt particle id %d at slot %d", p, i);
+            return MCGB_ERR_STATE;
+        }
+        if (qx) qx[p] = q[0][i];
+        if (qy) qy[p] = q[1][i];
+        if (qz) qz[p] = q[2][i];
+        if (ux) ux[p] = ud[0][i];
+        if (uy) uy[p] = ud[1][i];
+        if (uz) uz[p] = ud[2][i];
+    }
+    if (box) {
+        std::vector<RepDyn> D;
+        if ((rc = read_dyn(c, D))) return rc;
+        for (int d = 0; d < 3; ++d) box[d] = D[r].L[d];
+    }
+    return MCGB_OK;
+}
+
+int mcgb_get_state(mcgb_ctx* c, int r, int n, double* x, double* y, double* z, double* ux,
+                   double* uy, double* uz, double box[3]) {
+    int rc = check_ctx(c, r);
+    if (rc) return rc;
+    std::vector<uint32_t> q[3];
+    for (int d = 0; d < 3; ++d) q[d].resize(n > 0 ? n : 1);
+    double L[3];
+    rc = mcgb_get_state_fixed(c, r, n, q[0].data(), q[1].data(), q[2].data(), ux, uy, uz, L);
+    if (rc) return rc;
+    double* xs[3] = {x, y, z};
+    for (int d = 0; d < 3; ++d)
+        if (xs[d])
+            for (int i = 0; i < n; ++i) xs[d][i] = (double)q[d][i] * (L[d] / kTwo32);
+    if (box)
+        for (int d = 0; d < 3; ++d) box[d] = L[d];
+    return MCGB_OK;
+}
+
+static int sweep_impl(mcgb_ctx* c, int n_sweeps, mcgb_move_trace* trace) {
+    for (int r = 0; r < c->nrep; ++r)
+        if (!c->state_set[r]) {
+            mcgb_set_error("replica %d has no configuration (call mcgb_set_state first)", r);
+            return MCGB_ERR_STATE;
+        }
+    int rc = c->prec == 64 ? enqueue_sweeps<double>(c, n_sweeps, trace)
+                           : enqueue_sweeps<float>(c, n_sweeps, trace);
+    if (rc) return rc;
+    std::vector<RepDyn> D;
+    if ((rc = read_dyn(c, D))) return rc;
+    for (int r = 0; r < c->nrep; ++r) {
+        double e = (double)D[r].cap_pairs * 1e9 + (double)D[r].cap_coll * 1e8 + D[r].e_fin;
+        if (!std::isfinite(e)) {
+            mcgb_set_error("replica %d: running energy is not finite", r);
+            return MCGB_ERR_STATE;
+        }
+    }
+    return MCGB_OK;
+}
+
+int mcgb_sweep(mcgb_ctx* c, int n_sweeps) {
+    int rc = check_ctx(c, 0);
+    if (rc) return rc;
+    if (n_sweeps < 0) {
+        mcgb_set_error("n_sweeps must be >= 0");
+        return MCGB_ERR_ARG;
+    }
+    return sweep_impl(c, n_sweeps, nullptr);
+}
+
+int mcgb_sweep_traced(mcgb_ctx* c, mcgb_move_trace* trace, int capacity, uint32_t info[13]) {
+    int rc = check_ctx(c, 0);
+    if (rc) return rc;
+    if (!trace || capacity <= 0) {
+        mcgb_set_error("mcgb_sweep_traced: need a trace buffer");
+        return MCGB_ERR_ARG;
+    }
+    if (c->nrep != 1) {
+        mcgb_set_error("mcgb_sweep_traced: the context must hold exactly one replica");
+        return MCGB_ERR_ARG;
+    }
+    const RepConst& C = c->hC[0];
+    const int ncap = C.ncap[0] * C.ncap[1] * C.ncap[2];
+    const size_t need = (size_t)C.N + (size_t)ncap;     // n_cells * ceil(N / n_cells) <= N + n_cells
+    if (c->dtrace_cap < need) {
+        if (c->dtrace) {
+            (void)hipFree(c->dtrace);
+            c->allocs.erase(std::remove(c->allocs.begin(), c->allocs.end(), (void*)c->dtrace), c->allocs.end());
+        }
+        c->dtrace = nullptr;
+        if ((rc = dalloc_t(c, &c->dtrace, need))) return rc;
+        c->dtrace_cap = need;
+    }
+    if ((rc = sweep_impl(c, 1, c->dtrace))) return rc;
+    std::vector<RepDyn> D;
+    if ((rc = read_dyn(c, D))) return rc;
+    const size_t nrec = (size_t)D[0].ncell * (size_t)D[0].k;
+    if ((size_t)capacity < nrec) {
+        mcgb_set_error("trace capacity %d < %zu records", capacity, nrec);
+        return MCGB_ERR_CAPACITY;
+    }
+    HIPCHK(hipMemcpy(trace, c->dtrace, nrec * sizeof(mcgb_move_trace), hipMemcpyDeviceToHost));
+    if (info) {
+        for (int d = 0; d < 3; ++d) info[d] = D[0].g[d];
+        for (int p = 0; p < 8; ++p) info[3 + p] = D[0].order[p];
+        info[11] = (uint32_t)D[0].k;
+        info[12] = (uint32_t)D[0].ncell;
+    }
+    return MCGB_OK;
+}
+
+int mcgb_get_stats(mcgb_ctx* c, int r, mcgb_stats* out) {
+    int rc = check_ctx(c, r);
+    if (rc) return rc;
+    if (!out) { mcgb_set_error("null stats"); return MCGB_ERR_ARG; }
+    std::vector<RepDyn> D;
+    if ((rc = read_dyn(c, D))) return rc;
+    const RepDyn& d = D[r];
+    memset(out, 0, sizeof(*out));
+    out->sweeps = d.t;
+    out->attempted = d.attempted;
+    out->accepted = d.accepted;
+    out->rejected_left_cell = d.left;
+    out->rejected_hard_core = d.hard;
+    out->candidates = d.cand;
+    out->pairs = d.pairs;
+    out->colloid_evals = d.coll;
+    out->volume_attempted = d.vol_att;
+    out->volume_accepted = d.vol_acc;
+    out->cap_pairs = d.cap_pairs;
+    out->cap_colloid = d.cap_coll;
+    out->energy_finite = d.e_fin;
+    out->energy = (double)d.cap_pairs * 1e9 + (double)d.cap_coll * 1e8 + d.e_fin;
+    for (int k = 0; k < 3; ++k) out->box[k] = d.L[k];
+    out->volume = d.L[0] * d.L[1] * d.L[2];
+    out->density = c->hC[r].N / (out->volume - c->hC[r].vcoll);
+    out->max_drift = d.max_drift;
+    return MCGB_OK;
+}
+
+int mcgb_reset_counters(mcgb_ctx* c, int r) {
+    int rc = check_ctx(c, r);
+    if (rc) return rc;
+    std::vector<RepDyn> D;
+    if ((rc = read_dyn(c, D))) return rc;
+    RepDyn& d = D[r];
+    d.attempted = d.accepted = d.left = d.hard = d.cand = d.pairs = d.coll = d.vol_att = d.vol_acc = 0;
+    d.max_drift = 0.0;
+    d.overflow = 0;
+    return write_dyn(c, D);
+}
+
+int mcgb_total_energy(mcgb_ctx* c, int r, double out[4]) {
+    int rc = check_ctx(c, r);
+    if (rc) return rc;
+    if ((rc = recompute(c, 3))) return rc;
+    std::vector<RepDyn> D;
+    if ((rc = read_dyn(c, D))) return rc;
+    const double* e = D[r].e_res;
+    out[0] = e[0] * 1e9 + e[1] * 1e8 + e[2];
+    out[1] = e[0];
+    out[2] = e[1];
+    out[3] = e[2];
+    return MCGB_OK;
+}
+
+int mcgb_resum_energy(mcgb_ctx* c, int r, double* drift) {
+    int rc = check_ctx(c, r);
+    if (rc) return rc;
+    std::vector<RepDyn> D0;
+    if ((rc = read_dyn(c, D0))) return rc;
+    double run = (double)D0[r].cap_pairs * 1e9 + (double)D0[r].cap_coll * 1e8 + D0[r].e_fin;
+    if ((rc = recompute(c, 0))) return rc;
+    std::vector<RepDyn> D;
+    if ((rc = read_dyn(c, D))) return rc;
+    double now = (double)D[r].cap_pairs * 1e9 + (double)D[r].cap_coll * 1e8 + D[r].e_fin;
+    if (drift) *drift = run - now;
+    return MCGB_OK;
+}
+
+int mcgb_order_parameter(mcgb_ctx* c, int r, double* p2, double director[3]) {
+    int rc = check_ctx(c, r);
+    if (rc) return rc;
+    rc = c->prec == 64 ? enqueue_observe<double>(c, 0, 0) : enqueue_observe<float>(c, 0, 0);
+    if (rc) return rc;
+    std::vector<double> part((size_t)c->nrep * 64 * 6);
+    HIPCHK(hipMemcpyAsync(part.data(), c->obs, part.size() * 8, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    double s[6] = {0, 0, 0, 0, 0, 0};
+    for (int b = 0; b < 64; ++b)
+        for (int k = 0; k < 6; ++k) s[k] += part[((size_t)r * 64 + b) * 6 + k];
+    const double N = c->hC[r].N;
+    double q[6] = {s[0] / N - 1.0 / 3.0, s[1] / N - 1.0 / 3.0, s[2] / N - 1.0 / 3.0, s[3] / N, s[4] / N, s[5] / N};
+    double ev[3], V[9];
+    mcgb_eigen_sym3(q, ev, V);
+    if (p2) *p2 = 1.5 * ev[2];
+    if (director) { director[0] = V[2]; director[1] = V[5]; director[2] = V[8]; }
+    return MCGB_OK;
+}
+
+int mcgb_series(mcgb_ctx* c, int r, int max, double* epp, double* p2, double* dens) {
+    int rc = check_ctx(c, r);
+    if (rc) return rc;
+    std::vector<RepDyn> D;
+    if ((rc = read_dyn(c, D))) return rc;
+    int cnt = D[r].series_count;
+    int avail = std::min(cnt, c->series_cap);
+    int n = std::min(avail, max);
+    if (n <= 0) return 0;
+    std::vector<double> buf((size_t)c->series_cap * kSeriesWidth);
+    HIPCHK(hipMemcpy(buf.data(), c->series + (size_t)r * c->series_cap * kSeriesWidth,
+                     buf.size() * 8, hipMemcpyDeviceToHost));
+    // oldest first: the last `avail` entries of the ring, of which the first n are returned
+    int first = cnt - avail;
+    for (int k = 0; k < n; ++k) {
+        const double* e = &buf[(size_t)((first + k) % c->series_cap) * kSeriesWidth];
+        if (epp) epp[k] = e[0];
+        if (p2) {
+            double q[6] = {e[1] - 1.0 / 3.0, e[2] - 1.0 / 3.0, e[3] - 1.0 / 3.0, e[4], e[5], e[6]};
+            double ev[3];
+            mcgb_eigen_sym3(q, ev, nullptr);
+            p2[k] = 1.5 * ev[2];
+        }
+        if (dens) dens[k] = e[7];
+    }
+    return n;
+}
+
+int mcgb_series_clear(mcgb_ctx* c) {
+    int rc = check_ctx(c, 0);
+    if (rc) return rc;
+    std::vector<RepDyn> D;
+    if ((rc = read_dyn(c, D))) return rc;
+    for (auto& d : D) d.series_count = 0;
+    return write_dyn(c, D);
+}
+
+int mcgb_kernel_times(mcgb_ctx* c, double* ms_pass, double* ms_rebuild, double* ms_sweep) {
+    if (!c) { mcgb_set_error("null context"); return MCGB_ERR_ARG; }
+    if (ms_pass) *ms_pass = c->ms_pass;
+    if (ms_rebuild) *ms_rebuild = c->ms_rebuild;
+    if (ms_sweep) *ms_sweep = c->ms_sweep;
+    return MCGB_OK;
+}
+
+int mcgb_cell_list(mcgb_ctx* c, int r, uint32_t t, int* cell_start, int cell_start_len,
+                   int* pid_sorted, int n, uint32_t shift[3], uint32_t order[8], int dims[3]) {
+    int rc = check_ctx(c, r);
+    if (rc) return rc;
+    const RepConst& C = c->hC[r];
+    if (n != C.N) { mcgb_set_error("replica %d holds %d particles", r, C.N); return MCGB_ERR_ARG; }
+    std::vector<RepDyn> D;
+    if ((rc = read_dyn(c, D))) return rc;
+    const uint32_t t_saved = D[r].t;
+    D[r].t = t;
+    if ((rc = write_dyn(c, D))) return rc;
+    k_begin<<<(c->nrep + 63) / 64, 64, 0, c->st>>>(c->dC, c->dD, c->nrep, 0);
+    rc = c->prec == 64 ? enqueue_rebuild<double>(c) : enqueue_rebuild<float>(c);
+    if (rc) return rc;
+    std::vector<RepDyn> D2;
+    if ((rc = read_dyn(c, D2))) return rc;
+    const int ncell = D2[r].ncell;
+    if (cell_start_len < ncell + 1) {
+        mcgb_set_error("cell_start needs %d entries", ncell + 1);
+        return MCGB_ERR_CAPACITY;
+    }
+    HIPCHK(hipMemcpy(cell_start, c->cell_start + C.cs_off, (ncell + 1) * sizeof(int), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(pid_sorted, c->pid[c->cur] + C.part_off, n * sizeof(int), hipMemcpyDeviceToHost));
+    if (shift) for (int d = 0; d < 3; ++d) shift[d] = D2[r].g[d];
+    if (order) for (int p = 0; p < 8; ++p) order[p] = D2[r].order[p];
+    if (dims) for (int d = 0; d < 3; ++d) dims[d] = D2[r].n[d];
+    D2[r].t = t_saved;
+    return write_dyn(c, D2);
+}
+
+// ---------------------------------------------------------------- batch kernels
+
+static int batch_consts(const mcgb_params* p, RepConst* C) {
+    if (!p) { mcgb_set_error("null parameters"); return MCGB_ERR_ARG; }
+    memset(C, 0, sizeof(*C));
+    mcgb_params q = *p;
+    if (q.system_temperature <= 0.0) q.system_temperature = 1.0;
+    if (q.gb_mu_exponent == 0.0) { mcgb_set_error("mu must be nonzero"); return MCGB_ERR_ARG; }
+    fill_consts(q, C);
+    return MCGB_OK;
+}
+
+int mcgb_pair_energy_batch(const mcgb_params* p, int n, const double* ei, const double* ej,
+                           const double* dr, int precision, double* u, int* capped) {
+    RepConst C;
+    int rc = batch_consts(p, &C);
+    if (rc) return rc;
+    if (n <= 0 || !ei || !ej || !dr || !u || !capped || (precision != 32 && precision != 64)) {
+        mcgb_set_error("mcgb_pair_energy_batch: bad arguments");
+        return MCGB_ERR_ARG;
+    }
+    double *d_ei, *d_ej, *d_dr, *d_u;
+    int* d_c;
+    HIPCHK(hipMalloc(&d_ei, n * 24));
+    HIPCHK(hipMalloc(&d_ej, n * 24));
+    HIPCHK(hipMalloc(&d_dr, n * 24));
+    HIPCHK(hipMalloc(&d_u, n * 8));
+    HIPCHK(hipMalloc(&d_c, n * 4));
+    HIPCHK(hipMemcpy(d_ei, ei, n * 24, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(d_ej, ej, n * 24, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(d_dr, dr, n * 24, hipMemcpyHostToDevice));
+    k_pair_batch<<<(n + 255) / 256, 256>>>(C, n, precision, d_ei, d_ej, d_dr, d_u, d_c);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpy(u, d_u, n * 8, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(capped, d_c, n * 4, hipMemcpyDeviceToHost));
+    HIPCHK(hipFree(d_ei)); HIPCHK(hipFree(d_ej)); HIPCHK(hipFree(d_dr));
+    HIPCHK(hipFree(d_u)); HIPCHK(hipFree(d_c));
+    return MCGB_OK;
+}
+
+int mcgb_colloid_energy_batch(const mcgb_params* p, int n, const double* rel, const double* e,
+                              int precision, double* u, int* capped) {
+    RepConst C;
+    int rc = batch_consts(p, &C);
+    if (rc) return rc;
+    (void)precision;   // the colloid term is evaluated in FP64 in both precisions
+    if (n <= 0 || !rel || !e || !u || !capped) {
+        mcgb_set_error("mcgb_colloid_energy_batch: bad arguments");
+        return MCGB_ERR_ARG;
+    }
+    double *d_r, *d_e, *d_u;
+    int* d_c;
+    HIPCHK(hipMalloc(&d_r, n * 24));
+    HIPCHK(hipMalloc(&d_e, n * 24));
+    HIPCHK(hipMalloc(&d_u, n * 8));
+    HIPCHK(hipMalloc(&d_c, n * 4));
+    HIPCHK(hipMemcpy(d_r, rel, n * 24, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(d_e, e, n * 24, hipMemcpyHostToDevice));
+    k_colloid_batch<<<(n + 255) / 256, 256>>>(C, n, d_r, d_e, d_u, d_c);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpy(u, d_u, n * 8, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(capped, d_c, n * 4, hipMemcpyDeviceToHost));
+    HIPCHK(hipFree(d_r)); HIPCHK(hipFree(d_e)); HIPCHK(hipFree(d_u)); HIPCHK(hipFree(d_c));
+    return MCGB_OK;
+}
+
+int mcgb_philox_batch(int n, const uint32_t* ctr, const uint32_t* key, uint32_t* out) {
+    if (n <= 0 || !ctr || !key || !out) {
+        mcgb_set_error("mcgb_philox_batch: bad arguments");
+        return MCGB_ERR_ARG;
+    }
+    uint32_t *d_c, *d_k, *d_o;
+    HIPCHK(hipMalloc(&d_c, n * 16));
+    HIPCHK(hipMalloc(&d_k, n * 8));
+    HIPCHK(hipMalloc(&d_o, n * 16));
+    HIPCHK(hipMemcpy(d_c, ctr, n * 16, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(d_k, key, n * 8, hipMemcpyHostToDevice));
+    k_philox_batch<<<(n + 255) / 256, 256>>>(n, d_c, d_k, d_o);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpy(out, d_o, n * 16, hipMemcpyDeviceToHost));
+    HIPCHK(hipFree(d_c)); HIPCHK(hipFree(d_k)); HIPCHK(hipFree(d_o));
+    return MCGB_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,226 @@
+// mcgb_device.hpp — device-side data layout, counter-based RNG and pair physics for the
+// Gay-Berne checkerboard sweep on CDNA4 (gfx950).
+//
+// Positions are 32-bit fixed-point fractions of the box (x = q * L / 2^32): the minimum image
+// of a pair is the wrap-around int32 difference of the q's, cell assignment is an exact integer
+// multiply-shift, and an isotropic volume change leaves every q unchanged.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace mcgb {
+
+constexpr int kMaxReplicas = 1024;
+constexpr double kTwo32 = 4294967296.0;
+constexpr uint32_t kSetupTag = 0xFFFF0000u;   // 4th counter word of the per-sweep draws
+
+// ------------------------------------------------------------------ per-replica tables
+
+// Static per-replica constants (uploaded by the host; read with scalar loads).
+struct RepConst {
+    int N;                 // particles
+    int part_off;          // first particle slot in the particle arrays
+    int cell_off;          // first cell slot (counts/accumulators), capacity ncell_cap
+    int cs_off;            // first cell_start slot (ncell_cap + 1 entries)
+    int ncap[3];           // allocated cells per axis (even)
+    int act_off;           // first block of this replica in the pass grid
+    int nact_cap;          // blocks reserved for this replica (= ncap product / 8)
+    uint32_t key0, key1;   // Philox key
+    int colloid_on;        // colloid_wall_option == 1
+    int npt;               // NPT ln V moves
+    int pad0;
+    // physics, FP64 master copies (get_system_const, MCnptColloid.c:2752-2783)
+    double sigma0, four_eps0, chi, chi2, chip, chip2, mu, nu, rc, rc2;
+    double temperature, beta, pressure, dmax, drot;
+    double R, R2, colloid_cut, kappa_prime, one_minus_chi, one_minus_chip;
+    double npt_max_dlnv;
+    double vcoll;          // 4/3 pi R^3 (density convention, MCnptColloid.c:2799-2804)
+    // FP32 copies for the pair math
+    float f_sigma0, f_four_eps0, f_chi, f_chi2, f_chip, f_chip2, f_mu, f_half_nu, f_rc2, f_drot;
+};
+
+// Per-replica state updated on the device.
+struct RepDyn {
+    double L[3];                 // box
+    double s[3];                 // L / 2^32
+    int n[3];                    // cells per axis this sweep
+    int ncell, nact, k;          // cells, active cells per pass, moves per active cell
+    long long K[3];              // fixed-point step amplitude round(d_max 2^32 / L)
+    uint32_t t;                  // sweep index
+    uint32_t g[3];               // grid shift
+    uint32_t order[8];           // sublattice order
+    uint32_t vol[2];             // NPT draws
+    long long cap_pairs, cap_coll;
+    double e_fin;                // running energy = cap_pairs*1e9 + cap_coll*1e8 + e_fin
+    unsigned long long attempted, accepted, left, hard, cand, pairs, coll, vol_att, vol_acc;
+    double moved_abs;            // sum |finite dE| of accepted moves this sweep
+    double max_drift;
+    int resum;                   // re-sum requested for this sweep
+    int overflow;                // a stencil exceeded the LDS staging capacity
+    int series_count;
+    int cells_g0;                // 1: current cell list built with shift 0 for box L
+    // NPT trial box
+    double Lt[3];
+    int nt[3];
+    int trial_ok;
+    int pad1;
+    double e_res[4];             // last full recompute {cap_pairs, cap_coll, fin, hard}
+};
+
+// Per-active-cell accumulators of one pass (each cell is active exactly once per sweep).
+struct CellAcc {
+    int att, acc, left, hard;
+    int dcap, dccap, pairs, coll;
+    long long cand;
+    double dfin, moved_abs;
+    double pad;
+};
+
+// ------------------------------------------------------------------ Philox4x32-10
+
+__host__ __device__ inline void philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
+                                              uint32_t k0, uint32_t k1, uint32_t out[4]) {
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        uint64_t p0 = (uint64_t)0xD2511F53u * c0;
+        uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+        uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
+        uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+        c1 = (uint32_t)p1;
+        c3 = (uint32_t)p0;
+        c0 = n0;
+        c2 = n2;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+__host__ __device__ inline double u01(uint32_t r) { return ((double)r + 0.5) * (1.0 / kTwo32); }
+
+// Symmetric fixed-point step: trunc((2r - 2^32 + 1) * K / 2^33), uniform in (-K/2, K/2).
+__host__ __device__ inline int32_t step_q(uint32_t r, long long K) {
+    long long m = 2LL * (long long)r - 4294967295LL;
+    long long p = m * K;
+    return (int32_t)(p >= 0 ? (p >> 33) : -((-p) >> 33));
+}
+
+__host__ __device__ inline uint32_t cell_coord(uint32_t q, uint32_t g, uint32_t n) {
+    return (uint32_t)(((uint64_t)(uint32_t)(q - g) * n) >> 32);
+}
+
+__host__ __device__ inline int cell_dim(double L, double rc) {
+    int n = 2 * (int)floor(L / (2.0 * rc * (1.0 + 1e-6)));
+    return n >= 4 ? n : 0;
+}
+
+// ------------------------------------------------------------------ GB pair, FP32
+
+// Gay-Berne energy (pij_GB/g_omega, MCnptColloid.c:2607-2681) for unit orientation vectors.
+// Same algebra as the reference, rearranged for the VALU: one reciprocal per g(omega)
+//   g(w) = 1 - (w/2) [ (a+b)^2 (1 - w c) + (a-b)^2 (1 + w c) ] / (1 - w^2 c^2),
+// and eps1^nu * eps2^mu = exp2(mu log2 g(chi') - (nu/2) log2(1 - chi^2 c^2)).
+// Hardware transcendentals: v_rsq, v_rcp, v_log (log2), v_exp (exp2).  *cap = the 1e9 cap.
+__device__ inline float gb_pair_f32(const RepConst& C, float uix, float uiy, float uiz,
+                                    float ujx, float ujy, float ujz, float dx, float dy, float dz,
+                                    float r2, bool* cap) {
+    float rinv = __builtin_amdgcn_rsqf(r2);
+    float r = r2 * rinv;
+    float c = uix * ujx + uiy * ujy + uiz * ujz;
+    float a = (dx * uix + dy * uiy + dz * uiz) * rinv;
+    float b = (dx * ujx + dy * ujy + dz * ujz) * rinv;
+    float ap = a + b, am = a - b;
+    float ap2 = ap * ap, am2 = am * am;
+    float sp = ap2 + am2, sm = ap2 - am2;
+    float c2 = c * c;
+    float d1 = 1.0f - C.f_chi2 * c2;
+    float d2 = 1.0f - C.f_chip2 * c2;
+    float g1 = 1.0f - 0.5f * C.f_chi * (sp - C.f_chi * c * sm) * __builtin_amdgcn_rcpf(d1);
+    float g2 = 1.0f - 0.5f * C.f_chip * (sp - C.f_chip * c * sm) * __builtin_amdgcn_rcpf(d2);
+    float eps = __builtin_amdgcn_exp2f(C.f_mu * __builtin_amdgcn_logf(g2) -
+                                       C.f_half_nu * __builtin_amdgcn_logf(d1));
+    float sigma = C.f_sigma0 * __builtin_amdgcn_rsqf(g1);
+    float dist = r - sigma + C.f_sigma0;
+    *cap = dist < 0.001f;
+    float rho = C.f_sigma0 * __builtin_amdgcn_rcpf(dist);
+    float rho2 = rho * rho;
+    float rho6 = rho2 * rho2 * rho2;
+    return C.f_four_eps0 * eps * rho6 * (rho6 - 1.0f);
+}
+
+// ------------------------------------------------------------------ GB pair, FP64
+
+__device__ inline double g_omega64(double c, double ap2, double am2, double w) {
+    double oe = w * c;
+    return 1.0 - 0.5 * w * (ap2 / (1.0 + oe) + am2 / (1.0 - oe));
+}
+
+// Term for term as pij_GB (MCnptColloid.c:2618-2681) with the inverse norms and cosine of GBij
+// (:1524-1535); returns the finite energy, *cap = the 1e9 cap.
+__device__ inline double gb_pair_f64(const RepConst& C, double uix, double uiy, double uiz,
+                                     double ujx, double ujy, double ujz, double dx, double dy,
+                                     double dz, double r2, bool* cap) {
+    double rij = sqrt(r2);
+    double irij = 1.0 / rij;
+    double iei = 1.0 / sqrt(uix * uix + uiy * uiy + uiz * uiz);
+    double iej = 1.0 / sqrt(ujx * ujx + ujy * ujy + ujz * ujz);
+    double c = iei * iej * (uix * ujx + uiy * ujy + uiz * ujz);
+    double a = irij * iei * (dx * uix + dy * uiy + dz * uiz);
+    double b = irij * iej * (dx * ujx + dy * ujy + dz * ujz);
+    double ap = a + b, am = a - b;
+    double ap2 = ap * ap, am2 = am * am;
+    double eps1 = 1.0 / sqrt(1.0 - C.chi2 * c * c);
+    double ctep1 = pow(eps1, C.nu);
+    double ctep2 = pow(g_omega64(c, ap2, am2, C.chip), C.mu);
+    double schi = g_omega64(c, ap2, am2, C.chi);
+    double sigma = C.sigma0 / sqrt(schi);
+    double dist = rij - sigma + C.sigma0;
+    *cap = dist < 0.001;
+    if (*cap) return 0.0;
+    double e = C.sigma0 / dist;
+    double e2 = e * e;
+    double e6 = e2 * e2 * e2;
+    return C.four_eps0 * (ctep1 * ctep2) * e6 * (e6 - 1.0);
+}
+
+// ------------------------------------------------------------------ colloid-rod, FP64
+
+// GBCollRod (MCnptColloid.c:1758-1854); rel = rod centre - colloid centre.  Evaluated in FP64
+// in both precisions (two calls per move, only near the colloid).  *cap = the 1e8 cap.
+__device__ inline double colloid_f64(const RepConst& C, double rx, double ry, double rz,
+                                     double ex, double ey, double ez, bool* cap) {
+    *cap = false;
+    double d = sqrt(rx * rx + ry * ry + rz * rz);
+    double dij = d - C.R;
+    if (d <= 1e-12) { *cap = true; return 0.0; }
+    if (dij > C.colloid_cut) return 0.0;
+    double cth = (rx / d) * ex + (ry / d) * ey + (rz / d) * ez;
+    cth = cth > 1.0 ? 1.0 : (cth < -1.0 ? -1.0 : cth);
+    double sin2 = 1.0 - cth * cth;
+    double s0 = C.sigma0;
+    double den = C.one_minus_chi, num = 1.0 - C.chi * sin2;
+    if (den <= 0.0) den = 1e-12;
+    if (num <= 1e-12) num = 1e-12;
+    double scr = s0 * sqrt(num / den);
+    double den_e = 1.0 - C.chip * sin2;
+    if (den_e <= 1e-12) den_e = 1e-12;
+    double eps = C.kappa_prime * pow(C.one_minus_chip / den_e, C.mu);
+    double x = dij - scr + s0;
+    if (x <= 1e-10) { *cap = true; return 0.0; }
+    double s3 = s0 * s0 * s0, s9 = s3 * s3 * s3;
+    double x2 = x * x, x3 = x2 * x, x8 = x3 * x3 * x2, x9 = x8 * x;
+    return eps * (s9 / (45.0 * x9) - s3 / (6.0 * x3) - s9 / (40.0 * d * x8) + s3 / (4.0 * d * x2));
+}
+
+// ------------------------------------------------------------------ compensated sums
+
+__device__ inline void two_sum(float& hi, float& lo, float v) {
+    float s = hi + v;
+    float bp = s - hi;
+    float err = (hi - (s - bp)) + (v - bp);
+    hi = s;
+    lo += err;
+}
+
+}  // namespace mcgb

@@ -1,0 +1,472 @@
+/*
+ * mcgb_host.c — host-side C part of libmcgb: parameter file loader, derived constants,
+ * synthetic start configurations, integrated autocorrelation time, 3x3 symmetric eigen solver,
+ * fortSphere.1 restart files, last-error text.  No GPU code.
+ */
+#include "mcgb_internal.h"
+
+#include <ctype.h>
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* ------------------------------------------------------------------ error text */
+
+static __thread char g_err[512];
+
+void mcgb_set_error(const char *fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+}
+
+const char *mcgb_last_error(void) { return g_err; }
+
+/* ------------------------------------------------------------------ parameters */
+
+/* Key names in the reference's order (MCnptColloid.c:4104-4112). */
+static const char *const k_required[MCGB_NUM_REQUIRED_KEYS] = {
+    "mc_ensemble_type", "num_particles", "system_density", "system_temperature",
+    "system_pressure", "gb_sigma_sidetoside", "gb_epsilon0", "gb_kappa",
+    "gb_kappa_prime", "gb_mu_exponent", "gb_nu_exponent", "max_displacement",
+    "max_rotation", "initial_configuration_type", "cutoff_radius",
+    "verlet_radius", "total_mc_cycles", "configuration_save_interval",
+    "statistics_print_interval", "colloid_wall_option", "colloid_interaction_epsilon",
+    "colloid_interaction_amplitude", "colloid_radius"};
+
+void mcgb_params_default(mcgb_params *p)
+{
+    memset(p, 0, sizeof(*p));
+    p->seed = 20261203ull;
+    p->precision = 32;
+    p->npt_max_dlnv = 0.01;
+    p->energy_check_interval = 100;
+    p->lattice_jitter = 0.0;
+    p->colloid_exclusion_radius = -1.0;      /* < 0: colloid_radius + 0.5 */
+}
+
+enum { T_INT, T_DOUBLE, T_FLOAT, T_U64 };
+
+static int parse_value(const char *value, int type, void *dst)
+{
+    /* sscanf(value, "%d"/"%lf"/"%f") as at MCnptColloid.c:3955-4089: leading number accepted */
+    switch (type) {
+    case T_INT: return sscanf(value, "%d", (int *)dst);
+    case T_DOUBLE: return sscanf(value, "%lf", (double *)dst);
+    case T_FLOAT: return sscanf(value, "%f", (float *)dst);
+    default: {
+        unsigned long long v;
+        int n = sscanf(value, "%llu", &v);
+        if (n == 1) *(uint64_t *)dst = (uint64_t)v;
+        return n;
+    }
+    }
+}
+
+int mcgb_params_load(const char *path, mcgb_params *out)
+{
+    mcgb_params p;
+    mcgb_params_default(&p);
+    FILE *f = fopen(path, "r");
+    if (!f) {
+        mcgb_set_error("Error opening the parameters file: %s (make sure the file exists)", path);
+        return MCGB_ERR_IO;
+    }
+    struct { const char *name; int type; void *dst; } keys[] = {
+        {"mc_ensemble_type", T_INT, &p.mc_ensemble_type},
+        {"num_particles", T_INT, &p.num_particles},
+        {"system_density", T_DOUBLE, &p.system_density},
+        {"system_temperature", T_DOUBLE, &p.system_temperature},
+        {"system_pressure", T_DOUBLE, &p.system_pressure},
+        {"gb_sigma_sidetoside", T_DOUBLE, &p.gb_sigma_sidetoside},
+        {"gb_epsilon0", T_DOUBLE, &p.gb_epsilon0},
+        {"gb_kappa", T_DOUBLE, &p.gb_kappa},
+        {"gb_kappa_prime", T_DOUBLE, &p.gb_kappa_prime},
+        {"gb_mu_exponent", T_DOUBLE, &p.gb_mu_exponent},
+        {"gb_nu_exponent", T_DOUBLE, &p.gb_nu_exponent},
+        {"max_displacement", T_FLOAT, &p.max_displacement},
+        {"max_rotation", T_FLOAT, &p.max_rotation},
+        {"initial_configuration_type", T_INT, &p.initial_configuration_type},
+        {"cutoff_radius", T_DOUBLE, &p.cutoff_radius},
+        {"verlet_radius", T_DOUBLE, &p.verlet_radius},
+        {"total_mc_cycles", T_INT, &p.total_mc_cycles},
+        {"configuration_save_interval", T_INT, &p.configuration_save_interval},
+        {"statistics_print_interval", T_INT, &p.statistics_print_interval},
+        {"colloid_wall_option", T_INT, &p.colloid_wall_option},
+        {"colloid_interaction_epsilon", T_DOUBLE, &p.colloid_interaction_epsilon},
+        {"colloid_interaction_amplitude", T_DOUBLE, &p.colloid_interaction_amplitude},
+        {"colloid_radius", T_DOUBLE, &p.colloid_radius},
+        /* optional keys */
+        {"seed", T_U64, &p.seed},
+        {"precision", T_INT, &p.precision},
+        {"npt_max_dlnv", T_DOUBLE, &p.npt_max_dlnv},
+        {"energy_check_interval", T_INT, &p.energy_check_interval},
+        {"lattice_jitter", T_DOUBLE, &p.lattice_jitter},
+        {"colloid_exclusion_radius", T_DOUBLE, &p.colloid_exclusion_radius},
+        {"stage_capacity", T_INT, &p.stage_capacity},
+    };
+    const int nkeys = (int)(sizeof(keys) / sizeof(keys[0]));
+    char line[256], name[256], value[256];
+    int total = 0;
+    /* read_next_parameter, MCnptColloid.c:3892-3906 */
+    while (fgets(line, sizeof(line), f)) {
+        if (line[0] == '\n' || line[0] == '#' || line[0] == '\0') continue;
+        value[0] = '\0';
+        if (sscanf(line, "%255s %255[^\n]", name, value) < 1) continue;
+        int k;
+        for (k = 0; k < nkeys; ++k)
+            if (strcmp(name, keys[k].name) == 0) break;
+        if (k == nkeys) {
+            printf("Warning: Unknown parameter '%s' in configuration file\n", name);
+            continue;
+        }
+        if (parse_value(value, keys[k].type, keys[k].dst) != 1) {
+            fclose(f);
+            mcgb_set_error("Error reading parameter: %s", name);
+            return MCGB_ERR_PARAM;
+        }
+        if (k < MCGB_NUM_REQUIRED_KEYS) {
+            p.keys_read |= 1u << k;
+            total++;
+        }
+    }
+    fclose(f);
+    fflush(stdout);
+    if (total != MCGB_NUM_REQUIRED_KEYS || p.keys_read != (1u << MCGB_NUM_REQUIRED_KEYS) - 1u) {
+        char buf[512];
+        int len = snprintf(buf, sizeof(buf),
+                           "Error: Not all parameters were read correctly. Expected 23, got %d\n"
+                           "Missing parameters:\n", total);
+        for (int i = 0; i < MCGB_NUM_REQUIRED_KEYS && len < (int)sizeof(buf); ++i)
+            if (!(p.keys_read & (1u << i)))
+                len += snprintf(buf + len, sizeof(buf) - (size_t)len, "  - %s%s\n", k_required[i],
+                                (i == 5 || i == 7) ? " cannot generate gb_sigma_endtoend variable." : "");
+        mcgb_set_error("%s", buf);
+        return MCGB_ERR_PARAM;
+    }
+    *out = p;
+    return MCGB_OK;
+}
+
+int mcgb_params_derived(const mcgb_params *p, double out[11])
+{
+    if (!p || !out || p->system_temperature <= 0.0 || p->gb_mu_exponent == 0.0) {
+        mcgb_set_error("mcgb_params_derived: need a parameter block with T* > 0 and mu != 0");
+        return MCGB_ERR_ARG;
+    }
+    /* get_system_const, MCnptColloid.c:2752-2783, in dependency order */
+    double k2 = p->gb_kappa * p->gb_kappa;
+    double chi = (k2 - 1.0) / (k2 + 1.0);
+    double kimu = pow(p->gb_kappa_prime, 1.0 / p->gb_mu_exponent);
+    double sigma_ee = p->gb_sigma_sidetoside * p->gb_kappa;   /* :4130 */
+    out[0] = 1.0 / p->system_temperature;
+    out[1] = 4.0 * p->gb_epsilon0;
+    out[2] = p->cutoff_radius * p->cutoff_radius;
+    out[3] = k2;
+    out[4] = chi;
+    out[5] = chi * chi;
+    out[6] = kimu;
+    out[7] = (kimu - 1.0) / (kimu + 1.0);
+    out[8] = sigma_ee;
+    out[9] = 4.0 * sigma_ee;
+    out[10] = p->colloid_radius * p->colloid_radius;
+    return MCGB_OK;
+}
+
+/* ------------------------------------------------------------------ host Philox and helpers */
+
+void mcgb_philox_host(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4])
+{
+    uint32_t c0 = ctr[0], c1 = ctr[1], c2 = ctr[2], c3 = ctr[3], k0 = key[0], k1 = key[1];
+    for (int r = 0; r < 10; ++r) {
+        uint64_t p0 = (uint64_t)0xD2511F53u * c0, p1 = (uint64_t)0xCD9E8D57u * c2;
+        uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0, n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+        c1 = (uint32_t)p1; c3 = (uint32_t)p0; c0 = n0; c2 = n2;
+        k0 += 0x9E3779B9u; k1 += 0xBB67AE85u;
+    }
+    out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+int mcgb_cell_dim(double L, double rc)
+{
+    int n = 2 * (int)floor(L / (2.0 * rc * (1.0 + 1e-6)));
+    return n >= 4 ? n : 0;
+}
+
+uint32_t mcgb_to_fixed(double x, double L)
+{
+    double f = x / L;
+    f -= floor(f);
+    double q = floor(f * 4294967296.0);
+    if (q >= 4294967296.0) q = 0.0;
+    if (q < 0.0) q = 0.0;
+    return (uint32_t)(uint64_t)q;
+}
+
+/* ------------------------------------------------------------------ synthetic starts */
+
+typedef struct { uint32_t key[2]; uint32_t ctr; uint32_t buf[4]; int used; } host_rng;
+
+static uint32_t rng_next(host_rng *g)
+{
+    if (g->used == 4) {
+        uint32_t c[4] = {g->ctr++, 0u, 0u, 0x5EED5EEDu};
+        mcgb_philox_host(c, g->key, g->buf);
+        g->used = 0;
+    }
+    return g->buf[g->used++];
+}
+
+static double rng_u01(host_rng *g) { return ((double)rng_next(g) + 0.5) * (1.0 / 4294967296.0); }
+
+int mcgb_init_lattice(int n, double density, double colloid_radius, double exclusion_radius,
+                      double jitter, uint64_t seed,
+                      double *x, double *y, double *z, double *ux, double *uy, double *uz,
+                      double box[3])
+{
+    if (n <= 0 || density <= 0.0 || !x || !y || !z || !ux || !uy || !uz || !box) {
+        mcgb_set_error("mcgb_init_lattice: bad arguments");
+        return MCGB_ERR_ARG;
+    }
+    const double pi = 3.14159265358979323846;
+    double V = (double)n / density + (4.0 / 3.0) * pi * colloid_radius * colloid_radius * colloid_radius;
+    double L = cbrt(V);
+    double excl = exclusion_radius < 0.0 ? colloid_radius + 0.5 : exclusion_radius;
+    double c = 0.5 * L;
+    int ns = (int)ceil(cbrt((double)n) - 1e-9);
+    if (ns < 1) ns = 1;
+    int64_t nsites = 0;
+    int64_t *site = NULL;
+    for (;; ++ns) {
+        double a = L / ns;
+        int64_t tot = (int64_t)ns * ns * ns;
+        free(site);
+        site = (int64_t *)malloc(sizeof(int64_t) * (size_t)tot);
+        if (!site) { mcgb_set_error("mcgb_init_lattice: out of memory"); return MCGB_ERR_NOMEM; }
+        nsites = 0;
+        for (int64_t s = 0; s < tot; ++s) {
+            int i = (int)(s % ns), j = (int)((s / ns) % ns), k = (int)(s / ((int64_t)ns * ns));
+            double px = (i + 0.5) * a - c, py = (j + 0.5) * a - c, pz = (k + 0.5) * a - c;
+            if (colloid_radius > 0.0 && px * px + py * py + pz * pz < excl * excl) continue;
+            site[nsites++] = s;
+        }
+        if (nsites >= n) break;
+    }
+    host_rng g = {{(uint32_t)seed, (uint32_t)(seed >> 32)}, 0u, {0, 0, 0, 0}, 4};
+    /* partial Fisher-Yates: the first n entries become a uniform random subset */
+    for (int64_t s = 0; s < n; ++s) {
+        uint64_t r = ((uint64_t)rng_next(&g) << 32) | rng_next(&g);
+        int64_t j = s + (int64_t)(r % (uint64_t)(nsites - s));
+        int64_t t = site[s]; site[s] = site[j]; site[j] = t;
+    }
+    double a = L / ns;
+    for (int p = 0; p < n; ++p) {
+        int64_t s = site[p];
+        int i = (int)(s % ns), j = (int)((s / ns) % ns), k = (int)(s / ((int64_t)ns * ns));
+        double px = (i + 0.5) * a + jitter * (2.0 * rng_u01(&g) - 1.0);
+        double py = (j + 0.5) * a + jitter * (2.0 * rng_u01(&g) - 1.0);
+        double pz = (k + 0.5) * a + jitter * (2.0 * rng_u01(&g) - 1.0);
+        x[p] = px - L * floor(px / L);
+        y[p] = py - L * floor(py / L);
+        z[p] = pz - L * floor(pz / L);
+        /* uniform on the sphere: z = 1 - 2U, phi = 2 pi U' */
+        double zz = 1.0 - 2.0 * rng_u01(&g), phi = 2.0 * pi * rng_u01(&g);
+        double sxy = sqrt(fmax(0.0, 1.0 - zz * zz));
+        ux[p] = sxy * cos(phi);
+        uy[p] = sxy * sin(phi);
+        uz[p] = zz;
+    }
+    free(site);
+    box[0] = box[1] = box[2] = L;
+    return MCGB_OK;
+}
+
+/* ------------------------------------------------------------------ statistics */
+
+int mcgb_tau_int(const double *x, int n, double c, double *tau, int *window)
+{
+    if (!x || n < 4 || !tau) {
+        mcgb_set_error("mcgb_tau_int: need at least 4 samples");
+        return MCGB_ERR_ARG;
+    }
+    double mean = 0.0;
+    for (int i = 0; i < n; ++i) mean += x[i];
+    mean /= n;
+    double c0 = 0.0;
+    for (int i = 0; i < n; ++i) c0 += (x[i] - mean) * (x[i] - mean);
+    c0 /= n;
+    if (c0 <= 0.0) { *tau = 1.0; if (window) *window = 0; return MCGB_OK; }
+    double t = 1.0;
+    int M;
+    for (M = 1; M < n / 2; ++M) {
+        double ct = 0.0;
+        for (int i = 0; i + M < n; ++i) ct += (x[i] - mean) * (x[i + M] - mean);
+        ct /= n;
+        t += 2.0 * ct / c0;
+        if ((double)M >= c * t) break;       /* Sokal's automatic window */
+    }
+    if (t < 1.0 / n) t = 1.0 / n;
+    *tau = t;
+    if (window) *window = M;
+    return MCGB_OK;
+}
+
+/* Symmetric 3x3 eigen-decomposition: closed-form eigenvalues (trigonometric solution of the
+ * characteristic cubic); eigenvectors by the extreme-eigenvalue-first construction: the vector
+ * of the better separated extreme eigenvalue from the largest cross product of two rows of
+ * (A - lambda I), the middle one from the 2x2 projection of A on its orthogonal complement, the
+ * last by a cross product (robust for repeated eigenvalues). */
+static void cross3(const double a[3], const double b[3], double c[3])
+{
+    c[0] = a[1] * b[2] - a[2] * b[1];
+    c[1] = a[2] * b[0] - a[0] * b[2];
+    c[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+static double dot3h(const double a[3], const double b[3]) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+
+static void unit_vector_for(const double A[3][3], double lam, double v[3])
+{
+    double r0[3] = {A[0][0] - lam, A[0][1], A[0][2]};
+    double r1[3] = {A[1][0], A[1][1] - lam, A[1][2]};
+    double r2[3] = {A[2][0], A[2][1], A[2][2] - lam};
+    double c[3][3];
+    cross3(r0, r1, c[0]);
+    cross3(r0, r2, c[1]);
+    cross3(r1, r2, c[2]);
+    int best = 0;
+    double nb = dot3h(c[0], c[0]);
+    for (int k = 1; k < 3; ++k) {
+        double nk = dot3h(c[k], c[k]);
+        if (nk > nb) { nb = nk; best = k; }
+    }
+    if (nb <= 0.0) { v[0] = 1.0; v[1] = 0.0; v[2] = 0.0; return; }
+    double sc = 1.0 / sqrt(nb);
+    for (int i = 0; i < 3; ++i) v[i] = c[best][i] * sc;
+}
+
+int mcgb_eigen_sym3(const double q[6], double evals[3], double evec[9])
+{
+    const double a00 = q[0], a11 = q[1], a22 = q[2], a01 = q[3], a02 = q[4], a12 = q[5];
+    double p1 = a01 * a01 + a02 * a02 + a12 * a12;
+    double m = (a00 + a11 + a22) / 3.0;
+    double l[3];
+    double d0 = a00 - m, d1 = a11 - m, d2 = a22 - m;
+    double p2 = d0 * d0 + d1 * d1 + d2 * d2 + 2.0 * p1;
+    if (p2 <= 0.0) {
+        l[0] = l[1] = l[2] = m;
+    } else {
+        double p = sqrt(p2 / 6.0);
+        double b00 = d0 / p, b11 = d1 / p, b22 = d2 / p, b01 = a01 / p, b02 = a02 / p, b12 = a12 / p;
+        double detb = b00 * (b11 * b22 - b12 * b12) - b01 * (b01 * b22 - b12 * b02) +
+                      b02 * (b01 * b12 - b11 * b02);
+        double r = 0.5 * detb;
+        if (r > 1.0) r = 1.0;
+        if (r < -1.0) r = -1.0;
+        double phi = acos(r) / 3.0;
+        const double two_pi_3 = 2.0943951023931954923;
+        l[2] = m + 2.0 * p * cos(phi);
+        l[0] = m + 2.0 * p * cos(phi + two_pi_3);
+        l[1] = 3.0 * m - l[0] - l[2];
+    }
+    for (int i = 0; i < 2; ++i)
+        for (int j = 0; j < 2 - i; ++j)
+            if (l[j] > l[j + 1]) { double t = l[j]; l[j] = l[j + 1]; l[j + 1] = t; }
+    evals[0] = l[0]; evals[1] = l[1]; evals[2] = l[2];
+    if (!evec) return MCGB_OK;
+    double V[3][3];   /* columns */
+    if (p2 <= 0.0) {
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) evec[3 * i + j] = i == j ? 1.0 : 0.0;
+        return MCGB_OK;
+    }
+    const double A[3][3] = {{a00, a01, a02}, {a01, a11, a12}, {a02, a12, a22}};
+    int ext = (l[2] - l[1] >= l[1] - l[0]) ? 2 : 0;
+    double ve[3];
+    unit_vector_for(A, l[ext], ve);
+    /* orthonormal basis (U, W) of the complement of ve */
+    double U[3], W[3];
+    if (fabs(ve[0]) > fabs(ve[1])) {
+        double inv = 1.0 / sqrt(ve[0] * ve[0] + ve[2] * ve[2]);
+        U[0] = -ve[2] * inv; U[1] = 0.0; U[2] = ve[0] * inv;
+    } else {
+        double inv = 1.0 / sqrt(ve[1] * ve[1] + ve[2] * ve[2]);
+        U[0] = 0.0; U[1] = ve[2] * inv; U[2] = -ve[1] * inv;
+    }
+    cross3(ve, U, W);
+    double AU[3], AW[3];
+    for (int i = 0; i < 3; ++i) {
+        AU[i] = A[i][0] * U[0] + A[i][1] * U[1] + A[i][2] * U[2];
+        AW[i] = A[i][0] * W[0] + A[i][1] * W[1] + A[i][2] * W[2];
+    }
+    double m00 = dot3h(U, AU) - l[1], m01 = dot3h(U, AW), m11 = dot3h(W, AW) - l[1];
+    double w0, w1;
+    if (fabs(m00) >= fabs(m11)) {
+        double n = sqrt(m00 * m00 + m01 * m01);
+        if (n > 0.0) { w0 = -m01 / n; w1 = m00 / n; } else { w0 = 1.0; w1 = 0.0; }
+    } else {
+        double n = sqrt(m11 * m11 + m01 * m01);
+        if (n > 0.0) { w0 = m11 / n; w1 = -m01 / n; } else { w0 = 1.0; w1 = 0.0; }
+    }
+    double vm[3], vo[3];
+    for (int i = 0; i < 3; ++i) vm[i] = w0 * U[i] + w1 * W[i];
+    if (ext == 2) {
+        cross3(vm, ve, vo);           /* eigenvector of l[0] */
+        for (int i = 0; i < 3; ++i) { V[i][2] = ve[i]; V[i][1] = vm[i]; V[i][0] = vo[i]; }
+    } else {
+        cross3(ve, vm, vo);           /* eigenvector of l[2] */
+        for (int i = 0; i < 3; ++i) { V[i][0] = ve[i]; V[i][1] = vm[i]; V[i][2] = vo[i]; }
+    }
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) evec[3 * i + j] = V[i][j];
+    return MCGB_OK;
+}
+
+/* ------------------------------------------------------------------ fortSphere.1 */
+
+int mcgb_write_fort1(const char *path, int n, const double *x, const double *y, const double *z,
+                     const double *ux, const double *uy, const double *uz,
+                     const double box[3], double colloid_radius)
+{
+    FILE *f = fopen(path, "w");
+    if (!f) { mcgb_set_error("cannot open %s for writing", path); return MCGB_ERR_IO; }
+    /* esc_fort1, MCnptColloid.c:3193-3210 */
+    fprintf(f, "%d %15.8lf %15.8lf %15.8lf %15.8lf\n", n, box[0], box[1], box[2], colloid_radius);
+    for (int i = 0; i < n; ++i) {
+        fprintf(f, "%15.8lf %15.8lf %15.8lf\n", x[i], y[i], z[i]);
+        fprintf(f, "%15.8lf %15.8lf %15.8lf\n", 0.0, 0.0, 0.0);
+        fprintf(f, "%15.8lf %15.8lf %15.8lf\n", ux[i], uy[i], uz[i]);
+        fprintf(f, "%15.8lf %15.8lf %15.8lf\n", 0.0, 0.0, 0.0);
+    }
+    if (fclose(f) != 0) { mcgb_set_error("error writing %s", path); return MCGB_ERR_IO; }
+    return MCGB_OK;
+}
+
+int mcgb_read_fort1(const char *path, int n, double *x, double *y, double *z,
+                    double *ux, double *uy, double *uz, double box[3], double *colloid_radius)
+{
+    FILE *f = fopen(path, "r");
+    if (!f) { mcgb_set_error("cannot open %s", path); return MCGB_ERR_IO; }
+    /* read_initial_configuration, MCnptColloid.c:2786-2831 */
+    int ntmp = 0, nval = 0;
+    double b;
+    if (fscanf(f, "%d %lf %lf %lf %lf", &ntmp, &box[0], &box[1], &box[2], colloid_radius) == 5) nval += 5;
+    if (ntmp < n) {
+        fclose(f);
+        mcgb_set_error("%s holds %d particles, %d requested", path, ntmp, n);
+        return MCGB_ERR_IO;
+    }
+    for (int i = 0; i < n; ++i) {
+        if (fscanf(f, "%lf %lf %lf", &x[i], &y[i], &z[i]) == 3) nval += 3;
+        if (fscanf(f, "%lf %lf %lf", &b, &b, &b) == 3) nval += 3;
+        if (fscanf(f, "%lf %lf %lf", &ux[i], &uy[i], &uz[i]) == 3) nval += 3;
+        if (fscanf(f, "%lf %lf %lf", &b, &b, &b) == 3) nval += 3;
+    }
+    fclose(f);
+    if (nval != 12 * n + 5) { mcgb_set_error("error reading %s", path); return MCGB_ERR_IO; }
+    return MCGB_OK;
+}

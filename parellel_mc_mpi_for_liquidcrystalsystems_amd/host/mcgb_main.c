@@ -1,0 +1,100 @@
+/*
+ * mcgb — C driver of the GPU Gay-Berne Monte Carlo (the role of main(), MCnptColloid.c:379-726,
+ * with the serial statistics cadence of :887-1032).
+ *
+ *   mcgb [parametersFile.txt]
+ *
+ * Reads the reference's 23 keys (plus optional ones), builds the start configuration
+ * (initial_configuration_type 0: fortSphere.1 as at MCnptColloid.c:2786-2831; otherwise a
+ * seeded lattice), runs total_mc_cycles checkerboard sweeps on the GPU, prints statistics every
+ * statistics_print_interval sweeps, writes fortSphere.1.BAK every configuration_save_interval
+ * sweeps and fortSphere.1 at the end (:3193-3229).  Every error prints its message and exits
+ * with status 1, as the reference does.
+ */
+#include <stdio.h>
+#include <stdlib.h>
+#include <time.h>
+
+#include "mcgb.h"
+
+static void die(const char *what)
+{
+    fprintf(stderr, "%s\n%s\n", what, mcgb_last_error());
+    exit(1);
+}
+
+static double now_s(void)
+{
+    struct timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return ts.tv_sec + 1e-9 * ts.tv_nsec;
+}
+
+int main(int argc, char **argv)
+{
+    const char *path = argc > 1 ? argv[1] : "parametersFile.txt";
+    mcgb_params p;
+    if (mcgb_params_load(path, &p) != MCGB_OK) die("cannot load parameters");
+    const int N = p.num_particles;
+    double *x = malloc(sizeof(double) * N), *y = malloc(sizeof(double) * N),
+           *z = malloc(sizeof(double) * N), *ux = malloc(sizeof(double) * N),
+           *uy = malloc(sizeof(double) * N), *uz = malloc(sizeof(double) * N);
+    if (!x || !y || !z || !ux || !uy || !uz) die("out of memory");
+    double box[3], R = p.colloid_radius;
+    if (p.initial_configuration_type == 0) {
+        if (mcgb_read_fort1("fortSphere.1", N, x, y, z, ux, uy, uz, box, &R) != MCGB_OK)
+            die("cannot read fortSphere.1");
+    } else {
+        double Rc = p.colloid_wall_option == 1 ? p.colloid_radius : 0.0;
+        if (mcgb_init_lattice(N, p.system_density, Rc, p.colloid_exclusion_radius, p.lattice_jitter,
+                              p.seed, x, y, z, ux, uy, uz, box) != MCGB_OK)
+            die("cannot build the start configuration");
+    }
+    mcgb_ctx *ctx;
+    if (mcgb_create(&p, 1, &ctx) != MCGB_OK) die("cannot create the GPU context");
+    if (mcgb_set_state(ctx, 0, N, x, y, z, ux, uy, uz, box) != MCGB_OK) die("cannot upload the state");
+
+    printf("================================================\n");
+    printf("N = %d  box = %.5f x %.5f x %.5f  T* = %.4f  %s  precision FP%d\n", N, box[0], box[1],
+           box[2], p.system_temperature, p.mc_ensemble_type == 2 ? "NPT" : "NVT", p.precision);
+    printf("================================================\n");
+    printf("%10s %16s %12s %10s %10s %10s\n", "sweep", "E/N", "P2", "density", "acc.ratio", "moves/s");
+
+    const int total = p.total_mc_cycles;
+    const int every = p.statistics_print_interval > 0 ? p.statistics_print_interval : total;
+    const int save = p.configuration_save_interval;
+    double t0 = now_s();
+    uint64_t last_att = 0;
+    double last_t = t0;
+    for (int done = 0; done < total;) {
+        int chunk = every < total - done ? every : total - done;
+        if (mcgb_sweep(ctx, chunk) != MCGB_OK) die("sweep failed");
+        done += chunk;
+        mcgb_stats s;
+        double p2, dir[3];
+        if (mcgb_get_stats(ctx, 0, &s) != MCGB_OK || mcgb_order_parameter(ctx, 0, &p2, dir) != MCGB_OK)
+            die("cannot read statistics");
+        double t = now_s();
+        double rate = (double)(s.attempted - last_att) / (t - last_t);
+        last_att = s.attempted;
+        last_t = t;
+        printf("%10d %16.8f %12.6f %10.6f %10.4f %10.3e\n", done, s.energy / N, p2, s.density,
+               s.attempted ? (double)s.accepted / s.attempted : 0.0, rate);
+        if (save > 0 && (done % save == 0 || done == total)) {
+            if (mcgb_get_state(ctx, 0, N, x, y, z, ux, uy, uz, box) != MCGB_OK) die("cannot read state");
+            if (mcgb_write_fort1("fortSphere.1.BAK", N, x, y, z, ux, uy, uz, box, R) != MCGB_OK)
+                die("cannot write fortSphere.1.BAK");
+        }
+    }
+    if (mcgb_get_state(ctx, 0, N, x, y, z, ux, uy, uz, box) != MCGB_OK) die("cannot read state");
+    if (mcgb_write_fort1("fortSphere.1", N, x, y, z, ux, uy, uz, box, R) != MCGB_OK)
+        die("cannot write fortSphere.1");
+    mcgb_stats s;
+    mcgb_get_stats(ctx, 0, &s);
+    printf("done: %llu trial moves in %.3f s, acceptance %.4f, max FP64 drift %.3e\n",
+           (unsigned long long)s.attempted, now_s() - t0,
+           s.attempted ? (double)s.accepted / s.attempted : 0.0, s.max_drift);
+    mcgb_destroy(ctx);
+    free(x); free(y); free(z); free(ux); free(uy); free(uz);
+    return 0;
+}

@@ -1,0 +1,362 @@
+"""ctypes binding of libmcgb.so (include/mcgb.h) for the tests and the benchmark.
+
+The compute path is the HIP library; this module only marshals arrays.  Loading fails loudly
+when libmcgb.so has not been built (python -m parellel_mc_mpi_for_liquidcrystalsystems_amd.build).
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "libmcgb.so")
+
+ERRORS = {0: "OK", -1: "ARG", -2: "IO", -3: "PARAM", -4: "HIP", -5: "NOMEM", -6: "GEOMETRY",
+          -7: "STATE", -8: "CAPACITY"}
+REQUIRED_KEYS = [
+    "mc_ensemble_type", "num_particles", "system_density", "system_temperature",
+    "system_pressure", "gb_sigma_sidetoside", "gb_epsilon0", "gb_kappa",
+    "gb_kappa_prime", "gb_mu_exponent", "gb_nu_exponent", "max_displacement",
+    "max_rotation", "initial_configuration_type", "cutoff_radius",
+    "verlet_radius", "total_mc_cycles", "configuration_save_interval",
+    "statistics_print_interval", "colloid_wall_option", "colloid_interaction_epsilon",
+    "colloid_interaction_amplitude", "colloid_radius"]
+
+
+class McgbError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"mcgb error {code} ({ERRORS.get(code, '?')}): {msg}")
+        self.code = code
+
+
+class Params(C.Structure):
+    _fields_ = [
+        ("mc_ensemble_type", C.c_int), ("num_particles", C.c_int),
+        ("system_density", C.c_double), ("system_temperature", C.c_double),
+        ("system_pressure", C.c_double), ("gb_sigma_sidetoside", C.c_double),
+        ("gb_epsilon0", C.c_double), ("gb_kappa", C.c_double), ("gb_kappa_prime", C.c_double),
+        ("gb_mu_exponent", C.c_double), ("gb_nu_exponent", C.c_double),
+        ("max_displacement", C.c_float), ("max_rotation", C.c_float),
+        ("initial_configuration_type", C.c_int), ("cutoff_radius", C.c_double),
+        ("verlet_radius", C.c_double), ("total_mc_cycles", C.c_int),
+        ("configuration_save_interval", C.c_int), ("statistics_print_interval", C.c_int),
+        ("colloid_wall_option", C.c_int), ("colloid_interaction_epsilon", C.c_double),
+        ("colloid_interaction_amplitude", C.c_double), ("colloid_radius", C.c_double),
+        ("seed", C.c_uint64), ("precision", C.c_int), ("npt_max_dlnv", C.c_double),
+        ("energy_check_interval", C.c_int), ("lattice_jitter", C.c_double),
+        ("colloid_exclusion_radius", C.c_double), ("stage_capacity", C.c_int),
+        ("keys_read", C.c_uint32)]
+
+
+class MoveTrace(C.Structure):
+    _fields_ = [("pid", C.c_int32), ("outcome", C.c_int32), ("dcap", C.c_int32),
+                ("dccap", C.c_int32), ("dfin", C.c_double), ("scale", C.c_double)]
+
+
+TRACE_DTYPE = np.dtype([("pid", "<i4"), ("outcome", "<i4"), ("dcap", "<i4"), ("dccap", "<i4"),
+                        ("dfin", "<f8"), ("scale", "<f8")])
+
+
+class Stats(C.Structure):
+    _fields_ = [(n, C.c_uint64) for n in (
+        "sweeps", "attempted", "accepted", "rejected_left_cell", "rejected_hard_core",
+        "candidates", "pairs", "colloid_evals", "volume_attempted", "volume_accepted")] + [
+        ("cap_pairs", C.c_int64), ("cap_colloid", C.c_int64), ("energy_finite", C.c_double),
+        ("energy", C.c_double), ("box", C.c_double * 3), ("volume", C.c_double),
+        ("density", C.c_double), ("max_drift", C.c_double)]
+
+    def as_dict(self):
+        d = {n: getattr(self, n) for n, _ in self._fields_}
+        d["box"] = list(self.box)
+        return d
+
+
+_lib = None
+dp = C.POINTER(C.c_double)
+up = C.POINTER(C.c_uint32)
+ip = C.POINTER(C.c_int)
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is not built: run "
+                          "`python -m parellel_mc_mpi_for_liquidcrystalsystems_amd.build`")
+    L = C.CDLL(LIB_PATH)
+    ctx = C.c_void_p
+    sig = {
+        "mcgb_params_default": (None, [C.POINTER(Params)]),
+        "mcgb_params_load": (C.c_int, [C.c_char_p, C.POINTER(Params)]),
+        "mcgb_params_derived": (C.c_int, [C.POINTER(Params), dp]),
+        "mcgb_create": (C.c_int, [C.POINTER(Params), C.c_int, C.POINTER(ctx)]),
+        "mcgb_destroy": (None, [ctx]),
+        "mcgb_last_error": (C.c_char_p, []),
+        "mcgb_device_count": (C.c_int, []),
+        "mcgb_set_state": (C.c_int, [ctx, C.c_int, C.c_int, dp, dp, dp, dp, dp, dp, dp]),
+        "mcgb_set_state_fixed": (C.c_int, [ctx, C.c_int, C.c_int, up, up, up, dp, dp, dp, dp]),
+        "mcgb_get_state": (C.c_int, [ctx, C.c_int, C.c_int, dp, dp, dp, dp, dp, dp, dp]),
+        "mcgb_get_state_fixed": (C.c_int, [ctx, C.c_int, C.c_int, up, up, up, dp, dp, dp, dp]),
+        "mcgb_sweep": (C.c_int, [ctx, C.c_int]),
+        "mcgb_sweep_traced": (C.c_int, [ctx, C.POINTER(MoveTrace), C.c_int, up]),
+        "mcgb_get_stats": (C.c_int, [ctx, C.c_int, C.POINTER(Stats)]),
+        "mcgb_reset_counters": (C.c_int, [ctx, C.c_int]),
+        "mcgb_total_energy": (C.c_int, [ctx, C.c_int, dp]),
+        "mcgb_resum_energy": (C.c_int, [ctx, C.c_int, dp]),
+        "mcgb_order_parameter": (C.c_int, [ctx, C.c_int, dp, dp]),
+        "mcgb_series": (C.c_int, [ctx, C.c_int, C.c_int, dp, dp, dp]),
+        "mcgb_series_clear": (C.c_int, [ctx]),
+        "mcgb_kernel_times": (C.c_int, [ctx, dp, dp, dp]),
+        "mcgb_cell_list": (C.c_int, [ctx, C.c_int, C.c_uint32, ip, C.c_int, ip, C.c_int, up, up, ip]),
+        "mcgb_pair_energy_batch": (C.c_int, [C.POINTER(Params), C.c_int, dp, dp, dp, C.c_int, dp, ip]),
+        "mcgb_colloid_energy_batch": (C.c_int, [C.POINTER(Params), C.c_int, dp, dp, C.c_int, dp, ip]),
+        "mcgb_philox_batch": (C.c_int, [C.c_int, up, up, up]),
+        "mcgb_init_lattice": (C.c_int, [C.c_int, C.c_double, C.c_double, C.c_double, C.c_double,
+                                        C.c_uint64, dp, dp, dp, dp, dp, dp, dp]),
+        "mcgb_tau_int": (C.c_int, [dp, C.c_int, C.c_double, dp, ip]),
+        "mcgb_eigen_sym3": (C.c_int, [dp, dp, dp]),
+        "mcgb_write_fort1": (C.c_int, [C.c_char_p, C.c_int, dp, dp, dp, dp, dp, dp, dp, C.c_double]),
+        "mcgb_read_fort1": (C.c_int, [C.c_char_p, C.c_int, dp, dp, dp, dp, dp, dp, dp, dp]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def check(rc):
+    if rc is not None and rc < 0:
+        raise McgbError(rc, lib().mcgb_last_error().decode(errors="replace"))
+    return rc
+
+
+def _d(a):
+    return a.ctypes.data_as(dp)
+
+
+def _u(a):
+    return a.ctypes.data_as(up)
+
+
+def _i(a):
+    return a.ctypes.data_as(ip)
+
+
+def default_params(**kw):
+    p = Params()
+    lib().mcgb_params_default(C.byref(p))
+    for k, v in kw.items():
+        if not hasattr(p, k):
+            raise KeyError(k)
+        setattr(p, k, v)
+    return p
+
+
+def gb_params(N, density, temperature, colloid_radius=0.0, precision=32, seed=1, npt=False,
+              pressure=0.0, kappa=3.0, kappa_prime=5.0, mu=2.0, nu=1.0, cutoff=4.0,
+              max_displacement=0.1, max_rotation=0.1, **kw):
+    """Parameter block of the benchmark family GB(kappa, kappa', mu, nu), sigma0 = eps0 = 1."""
+    return default_params(
+        mc_ensemble_type=2 if npt else 1, num_particles=N, system_density=density,
+        system_temperature=temperature, system_pressure=pressure, gb_sigma_sidetoside=1.0,
+        gb_epsilon0=1.0, gb_kappa=kappa, gb_kappa_prime=kappa_prime, gb_mu_exponent=mu,
+        gb_nu_exponent=nu, max_displacement=max_displacement, max_rotation=max_rotation,
+        initial_configuration_type=1, cutoff_radius=cutoff, verlet_radius=cutoff + 1.0,
+        total_mc_cycles=0, configuration_save_interval=0, statistics_print_interval=0,
+        colloid_wall_option=1 if colloid_radius > 0 else 0, colloid_interaction_epsilon=0.0,
+        colloid_interaction_amplitude=0.0, colloid_radius=colloid_radius, seed=seed,
+        precision=precision, keys_read=(1 << 23) - 1, **kw)
+
+
+def load_params(path):
+    p = Params()
+    check(lib().mcgb_params_load(path.encode(), C.byref(p)))
+    return p
+
+
+def derived(p):
+    out = np.empty(11)
+    check(lib().mcgb_params_derived(C.byref(p), _d(out)))
+    return out
+
+
+def device_count():
+    return lib().mcgb_device_count()
+
+
+def init_lattice(N, density, colloid_radius=0.0, exclusion_radius=-1.0, jitter=0.0, seed=1):
+    arrs = [np.empty(N) for _ in range(6)]
+    box = np.empty(3)
+    check(lib().mcgb_init_lattice(N, density, colloid_radius, exclusion_radius, jitter, seed,
+                                  *(_d(a) for a in arrs), _d(box)))
+    return arrs[:3], arrs[3:], box
+
+
+def tau_int(x, c=5.0):
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    tau = C.c_double()
+    w = C.c_int()
+    check(lib().mcgb_tau_int(_d(x), x.shape[0], c, C.byref(tau), C.byref(w)))
+    return tau.value, w.value
+
+
+def eigen_sym3(q):
+    q = np.ascontiguousarray(q, dtype=np.float64)
+    ev = np.empty(3)
+    V = np.empty(9)
+    check(lib().mcgb_eigen_sym3(_d(q), _d(ev), _d(V)))
+    return ev, V.reshape(3, 3)
+
+
+def pair_energy_batch(p, ei, ej, dr, precision):
+    ei, ej, dr = (np.ascontiguousarray(a, dtype=np.float64).reshape(-1, 3) for a in (ei, ej, dr))
+    n = ei.shape[0]
+    u = np.empty(n)
+    cap = np.empty(n, dtype=np.int32)
+    check(lib().mcgb_pair_energy_batch(C.byref(p), n, _d(ei), _d(ej), _d(dr), precision, _d(u), _i(cap)))
+    return u, cap
+
+
+def colloid_energy_batch(p, rel, e, precision=64):
+    rel, e = (np.ascontiguousarray(a, dtype=np.float64).reshape(-1, 3) for a in (rel, e))
+    n = rel.shape[0]
+    u = np.empty(n)
+    cap = np.empty(n, dtype=np.int32)
+    check(lib().mcgb_colloid_energy_batch(C.byref(p), n, _d(rel), _d(e), precision, _d(u), _i(cap)))
+    return u, cap
+
+
+def philox_batch(ctr, key):
+    ctr = np.ascontiguousarray(ctr, dtype=np.uint32).reshape(-1, 4)
+    key = np.ascontiguousarray(key, dtype=np.uint32).reshape(-1, 2)
+    out = np.empty_like(ctr)
+    check(lib().mcgb_philox_batch(ctr.shape[0], _u(ctr), _u(key), _u(out)))
+    return out
+
+
+class Context:
+    """One GPU context holding one or more replicas (mcgb_create/mcgb_destroy)."""
+
+    def __init__(self, params):
+        if isinstance(params, Params):
+            params = [params]
+        arr = (Params * len(params))(*params)
+        self.params = list(params)
+        self.nrep = len(params)
+        h = C.c_void_p()
+        check(lib().mcgb_create(arr, self.nrep, C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            lib().mcgb_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def set_state(self, pos, u, box, replica=0):
+        pos = [np.ascontiguousarray(a, dtype=np.float64) for a in pos]
+        u = [np.ascontiguousarray(a, dtype=np.float64) for a in u]
+        box = np.ascontiguousarray(box, dtype=np.float64)
+        check(lib().mcgb_set_state(self.h, replica, pos[0].shape[0], *(_d(a) for a in pos),
+                                   *(_d(a) for a in u), _d(box)))
+
+    def set_state_fixed(self, q, u, box, replica=0):
+        q = [np.ascontiguousarray(a, dtype=np.uint32) for a in q]
+        u = [np.ascontiguousarray(a, dtype=np.float64) for a in u]
+        box = np.ascontiguousarray(box, dtype=np.float64)
+        check(lib().mcgb_set_state_fixed(self.h, replica, q[0].shape[0], *(_u(a) for a in q),
+                                         *(_d(a) for a in u), _d(box)))
+
+    def get_state(self, replica=0):
+        n = self.params[replica].num_particles
+        pos = [np.empty(n) for _ in range(3)]
+        u = [np.empty(n) for _ in range(3)]
+        box = np.empty(3)
+        check(lib().mcgb_get_state(self.h, replica, n, *(_d(a) for a in pos), *(_d(a) for a in u), _d(box)))
+        return pos, u, box
+
+    def get_state_fixed(self, replica=0):
+        n = self.params[replica].num_particles
+        q = [np.empty(n, dtype=np.uint32) for _ in range(3)]
+        u = [np.empty(n) for _ in range(3)]
+        box = np.empty(3)
+        check(lib().mcgb_get_state_fixed(self.h, replica, n, *(_u(a) for a in q), *(_d(a) for a in u),
+                                         _d(box)))
+        return q, u, box
+
+    def sweep(self, n=1):
+        check(lib().mcgb_sweep(self.h, n))
+
+    def sweep_traced(self):
+        p = self.params[0]
+        cap = p.num_particles + 4 * p.num_particles + 100000
+        buf = (MoveTrace * cap)()
+        info = np.zeros(13, dtype=np.uint32)
+        check(lib().mcgb_sweep_traced(self.h, buf, cap, _u(info)))
+        nrec = int(info[11]) * int(info[12])
+        arr = np.frombuffer(buf, dtype=TRACE_DTYPE, count=nrec).copy()
+        return arr, info
+
+    def stats(self, replica=0):
+        s = Stats()
+        check(lib().mcgb_get_stats(self.h, replica, C.byref(s)))
+        return s.as_dict()
+
+    def reset_counters(self, replica=0):
+        check(lib().mcgb_reset_counters(self.h, replica))
+
+    def total_energy(self, replica=0):
+        out = np.empty(4)
+        check(lib().mcgb_total_energy(self.h, replica, _d(out)))
+        return out
+
+    def resum_energy(self, replica=0):
+        d = C.c_double()
+        check(lib().mcgb_resum_energy(self.h, replica, C.byref(d)))
+        return d.value
+
+    def order_parameter(self, replica=0):
+        p2 = C.c_double()
+        d = np.empty(3)
+        check(lib().mcgb_order_parameter(self.h, replica, C.byref(p2), _d(d)))
+        return p2.value, d
+
+    def series(self, replica=0, max_n=1 << 16):
+        e, p2, rho = np.empty(max_n), np.empty(max_n), np.empty(max_n)
+        n = check(lib().mcgb_series(self.h, replica, max_n, _d(e), _d(p2), _d(rho)))
+        return e[:n], p2[:n], rho[:n]
+
+    def series_clear(self):
+        check(lib().mcgb_series_clear(self.h))
+
+    def kernel_times(self):
+        a, b, c = C.c_double(), C.c_double(), C.c_double()
+        check(lib().mcgb_kernel_times(self.h, C.byref(a), C.byref(b), C.byref(c)))
+        return {"ms_pass": a.value, "ms_rebuild": b.value, "ms_sweep": c.value}
+
+    def cell_list(self, t, replica=0):
+        p = self.params[replica]
+        n = p.num_particles
+        cs = np.empty(2_000_000 if n > 100000 else 200000, dtype=np.int32)
+        ids = np.empty(n, dtype=np.int32)
+        shift = np.empty(3, dtype=np.uint32)
+        order = np.empty(8, dtype=np.uint32)
+        dims = np.empty(3, dtype=np.int32)
+        check(lib().mcgb_cell_list(self.h, replica, t, _i(cs), cs.shape[0], _i(ids), n, _u(shift),
+                                   _u(order), _i(dims)))
+        nc = int(np.prod(dims))
+        return cs[:nc + 1].copy(), ids, shift, order, dims

@@ -76,8 +76,12 @@ def u01(r):
     return (float(r) + 0.5) / TWO32
 
 
-def replay_sweep(c, q, u, L, seed, replica, t, trace, info, check_tol=1e-9):
-    """Replays one sweep on copies of (q, u); checks every trace record; returns final state."""
+def replay_sweep(c, q, u, L, seed, replica, t, trace, info, check_tol=1e-9, fp32=False):
+    """Replays one sweep on copies of (q, u); checks every trace record; returns final state.
+
+    fp32=True mirrors the FP32 kernel's storage: the rotation vector and the stored orientation
+    are rounded to float32; dE is checked to check_tol relative to the sum of |pair terms| and
+    an accept/reject within that tolerance of the threshold follows the record."""
     q = [a.astype(np.uint32).copy() for a in q]
     u = [a.astype(np.float64).copy() for a in u]
     N = q[0].shape[0]
@@ -123,9 +127,14 @@ def replay_sweep(c, q, u, L, seed, replica, t, trace, info, check_tol=1e-9):
                 phi = 2.0 * np.pi * u01(B[1])
                 sxy = np.sqrt(max(0.0, 1.0 - zz * zz))
                 ui = np.array([u[d][i] for d in range(3)])
-                un = ui + c.max_rotation * np.array([sxy * np.cos(phi), sxy * np.sin(phi), zz])
+                v = np.array([sxy * np.cos(phi), sxy * np.sin(phi), zz])
+                if fp32:
+                    v = v.astype(np.float32).astype(np.float64)
+                un = ui + c.max_rotation * v
                 nn = un @ un
                 un = ui.copy() if nn <= 1e-12 else un / np.sqrt(nn)
+                if fp32:
+                    un = un.astype(np.float32).astype(np.float64)
                 x = [q[d].astype(np.float64) * s[d] for d in range(3)]
                 xi = np.array([x[d][i] for d in range(3)])
                 xn = np.array([qn[d] * s[d] for d in range(3)])
@@ -146,7 +155,7 @@ def replay_sweep(c, q, u, L, seed, replica, t, trace, info, check_tol=1e-9):
                 dE = (cn_ - co) * 1e9 + dccap * 1e8 + dfin
                 thr = -np.log(u01(B[2])) * c.temperature
                 accept = dE <= thr
-                if abs(dE - thr) > 1e-9 * max(abs(thr), 1.0):
+                if abs(dE - thr) > check_tol * max(abs(thr), rec["scale"], 1.0):
                     assert rec["outcome"] == (4 if accept else 3)
                 else:
                     accept = rec["outcome"] == 4

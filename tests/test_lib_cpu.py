@@ -1,0 +1,165 @@
+"""libmcgb host side without a GPU: exported symbols, parameter loader against the reference's
+own loader, derived constants, start configurations, tau_int, eigen solver, fortSphere.1."""
+import json
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+from parellel_mc_mpi_for_liquidcrystalsystems_amd import mcgb as m
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLD = os.path.join(REPO, "tests", "golden")
+with open(os.path.join(GOLD, "ref_loader.json")) as f:
+    LOADER = json.load(f)
+
+
+def test_library_exports_every_declared_symbol():
+    hdr = open(os.path.join(REPO, "include", "mcgb.h")).read()
+    names = sorted(set(re.findall(r"\b(mcgb_[a-z0-9_]+)\s*\(", hdr)))
+    assert len(names) >= 25
+    L = m.lib()
+    for n in names:
+        assert hasattr(L, n), n
+
+
+@pytest.mark.parametrize("fname", ["params_cfg0.txt", "params_cfg1.txt", "params_unknown.txt"])
+def test_loader_matches_reference_loader(fname, capfd):
+    ref = LOADER["loader"][fname]["values"]
+    p = m.load_params(os.path.join(GOLD, fname))
+    for k, v in ref.items():
+        if k == "gb_sigma_endtoend":
+            assert p.gb_sigma_sidetoside * p.gb_kappa == pytest.approx(v, rel=1e-15)
+        else:
+            assert getattr(p, k) == pytest.approx(v, rel=1e-7 if k.startswith("max_") else 0), k
+    assert p.keys_read == (1 << 23) - 1
+
+
+def test_loader_optional_and_unknown_keys(capfd):
+    p = m.load_params(os.path.join(GOLD, "params_unknown.txt"))
+    out = capfd.readouterr().out
+    assert p.seed == 12345                       # optional key recognised
+    assert "Warning: Unknown parameter 'foo_bar' in configuration file" in out
+    assert "seed" not in out                     # the reference warned about it; we know it
+
+
+def test_loader_missing_keys_message():
+    with pytest.raises(m.McgbError) as e:
+        m.load_params(os.path.join(GOLD, "params_missing.txt"))
+    msg = str(e.value)
+    ref = LOADER["loader"]["params_missing.txt"]["stdout"]
+    assert e.value.code == -3
+    for line in ref:
+        assert line.strip() in msg
+
+
+def test_loader_malformed_value():
+    with pytest.raises(m.McgbError) as e:
+        m.load_params(os.path.join(GOLD, "params_malformed.txt"))
+    assert "Error reading parameter: num_particles" in str(e.value)
+
+
+def test_loader_missing_file():
+    with pytest.raises(m.McgbError) as e:
+        m.load_params("/nonexistent/parametersFile.txt")
+    assert e.value.code == -2
+
+
+def test_derived_constants_match_reference_second_call():
+    ref = LOADER["constants"]["gb3521"]["second_call"]
+    p = m.gb_params(1000, 0.3, 1.0, colloid_radius=5.0)
+    d = m.derived(p)
+    names = ["gb_beta", "gb_4eps0", "cutoff_radius_sq", "gb_kappa_sq", "gb_chi", "gb_chi_sq",
+             "gb_k_imu", "gb_chi_prime", None, "colloid_rod_cutoff", "colloid_radius_sq"]
+    for v, n in zip(d, names):
+        if n:
+            assert v == pytest.approx(ref[n], rel=1e-15, abs=1e-15), n
+    assert d[8] == 3.0
+
+
+def test_init_lattice_geometry():
+    (x, y, z), (ux, uy, uz), box = m.init_lattice(8000, 0.30, seed=4)
+    L = (8000 / 0.3) ** (1 / 3)
+    assert box[0] == pytest.approx(L, rel=1e-14)
+    a = L / 20                                  # 20^3 simple cubic lattice
+    for c in (x, y, z):
+        k = c / a - 0.5
+        assert np.allclose(k, np.round(k), atol=1e-9)
+    assert np.allclose(ux ** 2 + uy ** 2 + uz ** 2, 1.0)
+    assert abs(uz.mean()) < 0.03 and abs((uz ** 2).mean() - 1 / 3) < 0.02
+    pts = np.round(np.stack([x, y, z], 1) / a - 0.5).astype(int)
+    assert len({tuple(p) for p in pts}) == 8000
+
+
+def test_init_lattice_colloid_exclusion():
+    N, R = 20000, 5.0
+    (x, y, z), _, box = m.init_lattice(N, 0.32, colloid_radius=R, exclusion_radius=5.5, jitter=0.05, seed=2)
+    c = box[0] / 2
+    d = np.sqrt((x - c) ** 2 + (y - c) ** 2 + (z - c) ** 2)
+    assert d.min() >= 5.5 - 0.05 * np.sqrt(3) - 1e-9
+    assert box[0] ** 3 == pytest.approx(N / 0.32 + 4 / 3 * np.pi * R ** 3, rel=1e-12)
+
+
+def test_tau_int_ar1():
+    rng = np.random.default_rng(0)
+    for phi in (0.0, 0.5, 0.9):
+        n = 200000
+        e = rng.normal(size=n)
+        x = np.empty(n)
+        x[0] = e[0]
+        for i in range(1, n):
+            x[i] = phi * x[i - 1] + e[i]
+        tau, w = m.tau_int(x)
+        want = (1 + phi) / (1 - phi)
+        assert tau == pytest.approx(want, rel=0.08), (phi, tau, want)
+
+
+def test_eigen_sym3_against_numpy():
+    rng = np.random.default_rng(1)
+    for _ in range(200):
+        A = rng.normal(size=(3, 3))
+        A = A + A.T
+        if _ % 7 == 0:
+            A = np.diag(rng.normal(size=3))
+        if _ % 11 == 0:
+            v = rng.normal(size=3)
+            A = np.outer(v, v) - np.eye(3) * 0.2         # repeated eigenvalue
+        q = [A[0, 0], A[1, 1], A[2, 2], A[0, 1], A[0, 2], A[1, 2]]
+        ev, V = m.eigen_sym3(q)
+        assert np.allclose(ev, np.linalg.eigvalsh(A), atol=1e-10)
+        for k in range(3):
+            assert np.allclose(A @ V[:, k], ev[k] * V[:, k], atol=1e-8)
+
+
+def test_fort1_round_trip(tmp_path):
+    (x, y, z), (ux, uy, uz), box = m.init_lattice(500, 0.2, seed=9)
+    path = str(tmp_path / "fortSphere.1")
+    L = m.lib()
+    m.check(L.mcgb_write_fort1(path.encode(), 500, *(m._d(a) for a in (x, y, z, ux, uy, uz)),
+                               m._d(box), 5.0))
+    lines = open(path).read().splitlines()
+    assert len(lines) == 1 + 4 * 500 and lines[2].split() == ["0.00000000"] * 3
+    arrs = [np.empty(500) for _ in range(6)]
+    b2 = np.empty(3)
+    R = np.empty(1)
+    m.check(L.mcgb_read_fort1(path.encode(), 500, *(m._d(a) for a in arrs), m._d(b2), m._d(R)))
+    assert np.allclose(arrs[0], x, atol=1e-8) and np.allclose(arrs[5], uz, atol=1e-8)
+    assert R[0] == 5.0 and np.allclose(b2, box, atol=1e-8)
+
+
+def test_create_without_gpu_fails_loudly():
+    if m.device_count() > 0:
+        pytest.skip("a GPU is visible")
+    with pytest.raises(m.McgbError) as e:
+        m.Context(m.gb_params(8000, 0.3, 1.2))
+    assert e.value.code == -4
+
+
+def test_driver_reports_missing_parameter_file(tmp_path):
+    drv = os.path.join(REPO, "parellel_mc_mpi_for_liquidcrystalsystems_amd", "mcgb")
+    if not os.path.exists(drv):
+        pytest.skip("driver not built")
+    p = subprocess.run([drv, str(tmp_path / "nope.txt")], capture_output=True, text=True)
+    assert p.returncode == 1 and "Error opening the parameters file" in p.stderr

@@ -62,6 +62,9 @@
 
 void orc_consts_init(orc_consts *c)
 {
+    /* the move sizes are float globals in the reference (MCnptColloid.c:108-109, read with %f) */
+    c->max_displacement = (double)(float)c->max_displacement;
+    c->max_rotation     = (double)(float)c->max_rotation;
     /* get_system_const, MCnptColloid.c:2752-2783, in dependency order */
     c->beta        = 1.0 / c->temperature;
     c->four_eps0   = 4.0 * c->epsilon0;

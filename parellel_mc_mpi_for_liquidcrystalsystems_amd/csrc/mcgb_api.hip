@@ -145,6 +145,13 @@ void fill_consts(const mcgb_params& p, RepConst* C) {
     C->f_half_nu = (float)(0.5 * C->nu);
     C->f_rc2 = (float)C->rc2;
     C->f_drot = (float)C->drot;
+    C->f_g1c0 = (float)((1.0 - C->chi) * (1.0 - C->chi));
+    C->f_g1c1 = (float)(C->chi * (1.0 - C->chi));
+    C->f_g1d0 = (float)(1.0 - C->chi2);
+    C->f_g2c0 = (float)((1.0 - C->chip) * (1.0 - C->chip));
+    C->f_g2c1 = (float)(C->chip * (1.0 - C->chip));
+    C->f_g2d0 = (float)(1.0 - C->chip2);
+    C->f_rc2_pre = (float)(C->rc2 * (1.0 + 1e-5));
 }
 
 int check_params(const mcgb_params& p, int r) {

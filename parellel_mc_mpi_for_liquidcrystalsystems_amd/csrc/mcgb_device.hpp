@@ -38,6 +38,9 @@ struct RepConst {
     double vcoll;          // 4/3 pi R^3 (density convention, MCnptColloid.c:2799-2804)
     // FP32 copies for the pair math
     float f_sigma0, f_four_eps0, f_chi, f_chi2, f_chip, f_chip2, f_mu, f_half_nu, f_rc2, f_drot;
+    // g(om) = (c0 + c1 (|p|^2+|q|^2)/r^2 + om^2 (r.w)^2/r^2) / (d0 + om^2 |w|^2)
+    float f_g1c0, f_g1c1, f_g1d0, f_g2c0, f_g2c1, f_g2d0;
+    float f_rc2_pre;       // candidate pre-filter, slightly above rc^2 (final test in FP64)
 };
 
 // Per-replica state updated on the device.
@@ -117,35 +120,48 @@ __host__ __device__ inline int cell_dim(double L, double rc) {
 
 // ------------------------------------------------------------------ GB pair, FP32
 
-// Gay-Berne energy (pij_GB/g_omega, MCnptColloid.c:2607-2681) for unit orientation vectors.
-// Same algebra as the reference, rearranged for the VALU: one reciprocal per g(omega)
-//   g(w) = 1 - (w/2) [ (a+b)^2 (1 - w c) + (a-b)^2 (1 + w c) ] / (1 - w^2 c^2),
-// and eps1^nu * eps2^mu = exp2(mu log2 g(chi') - (nu/2) log2(1 - chi^2 c^2)).
-// Hardware transcendentals: v_rsq, v_rcp, v_log (log2), v_exp (exp2).  *cap = the 1e9 cap.
+// Gay-Berne energy (pij_GB/g_omega, MCnptColloid.c:2607-2681) for unit orientation vectors,
+// rewritten so that no step cancels.  With p = e_i x r, q = e_j x r, w = e_i x e_j
+// (Lagrange and Binet-Cauchy identities: 1 - a^2 = |p|^2/r^2, c - ab = p.q/r^2, 1 - c^2 = |w|^2,
+// |p x q|^2 = (r.w)^2 r^2):
+//   g(om) = [ (1-om)^2 + om(1-om)(|p|^2 + |q|^2)/r^2 + om^2 (r.w)^2/r^2 ] / [ (1-om^2) + om^2 |w|^2 ],
+// a ratio of sums of non-negative terms for 0 <= om < 1 (the reference's form subtracts two
+// nearly equal numbers near end-to-end contact).  eps1^nu eps2^mu =
+// exp2(mu (log2 N' - log2 D') - (nu/2) log2 D), D = 1 - chi^2 c^2.  The distance r comes from
+// the exact FP64 r^2 (r2d) with one FP64 Newton step on the hardware rsqrt, and
+// r - sigma + sigma0 is formed in FP64: the remaining error is that of sigma (a few FP32 ulps).
+// Transcendentals: v_rsq x2, v_rcp x2, v_log x3, v_exp x1.  *cap = the 1e9 cap.
 __device__ inline float gb_pair_f32(const RepConst& C, float uix, float uiy, float uiz,
                                     float ujx, float ujy, float ujz, float dx, float dy, float dz,
-                                    float r2, bool* cap) {
-    float rinv = __builtin_amdgcn_rsqf(r2);
-    float r = r2 * rinv;
-    float c = uix * ujx + uiy * ujy + uiz * ujz;
-    float a = (dx * uix + dy * uiy + dz * uiz) * rinv;
-    float b = (dx * ujx + dy * ujy + dz * ujz) * rinv;
-    float ap = a + b, am = a - b;
-    float ap2 = ap * ap, am2 = am * am;
-    float sp = ap2 + am2, sm = ap2 - am2;
-    float c2 = c * c;
-    float d1 = 1.0f - C.f_chi2 * c2;
-    float d2 = 1.0f - C.f_chip2 * c2;
-    float g1 = 1.0f - 0.5f * C.f_chi * (sp - C.f_chi * c * sm) * __builtin_amdgcn_rcpf(d1);
-    float g2 = 1.0f - 0.5f * C.f_chip * (sp - C.f_chip * c * sm) * __builtin_amdgcn_rcpf(d2);
-    float eps = __builtin_amdgcn_exp2f(C.f_mu * __builtin_amdgcn_logf(g2) -
-                                       C.f_half_nu * __builtin_amdgcn_logf(d1));
-    float sigma = C.f_sigma0 * __builtin_amdgcn_rsqf(g1);
-    float dist = r - sigma + C.f_sigma0;
-    *cap = dist < 0.001f;
-    float rho = C.f_sigma0 * __builtin_amdgcn_rcpf(dist);
-    float rho2 = rho * rho;
-    float rho6 = rho2 * rho2 * rho2;
+                                    double r2d, bool* cap) {
+    const float r2 = (float)r2d;
+    const float rinv = __builtin_amdgcn_rsqf(r2);
+    const float ir2 = rinv * rinv;
+    // w = e_i x e_j, p = e_i x dr, q = e_j x dr
+    const float wx = uiy * ujz - uiz * ujy, wy = uiz * ujx - uix * ujz, wz = uix * ujy - uiy * ujx;
+    const float px = uiy * dz - uiz * dy, py = uiz * dx - uix * dz, pz = uix * dy - uiy * dx;
+    const float qx = ujy * dz - ujz * dy, qy = ujz * dx - ujx * dz, qz = ujx * dy - ujy * dx;
+    const float w2 = wx * wx + wy * wy + wz * wz;
+    const float tau = dx * wx + dy * wy + dz * wz;
+    const float ab = (px * px + py * py + pz * pz + qx * qx + qy * qy + qz * qz) * ir2;
+    const float tt = tau * tau * ir2;
+    // shape g(chi) and energy g(chi') as N / D
+    const float n1 = C.f_g1c0 + C.f_g1c1 * ab + C.f_chi2 * tt;
+    const float d1 = C.f_g1d0 + C.f_chi2 * w2;
+    const float n2 = C.f_g2c0 + C.f_g2c1 * ab + C.f_chip2 * tt;
+    const float d2 = C.f_g2d0 + C.f_chip2 * w2;
+    const float eps = __builtin_amdgcn_exp2f(
+        C.f_mu * (__builtin_amdgcn_logf(n2) - __builtin_amdgcn_logf(d2)) -
+        C.f_half_nu * __builtin_amdgcn_logf(d1));
+    const float sigma = C.f_sigma0 * __builtin_amdgcn_rsqf(n1 * __builtin_amdgcn_rcpf(d1));
+    // r in FP64: one Newton step on the FP32 reciprocal square root
+    const double y = (double)rinv;
+    const double y1 = y * (1.5 - 0.5 * r2d * y * y);
+    const double dist = (r2d * y1 - (double)sigma) + C.sigma0;
+    *cap = dist < 0.001;
+    const float rho = C.f_sigma0 * __builtin_amdgcn_rcpf((float)dist);
+    const float rho2 = rho * rho;
+    const float rho6 = rho2 * rho2 * rho2;
     return C.f_four_eps0 * eps * rho6 * (rho6 - 1.0f);
 }
 

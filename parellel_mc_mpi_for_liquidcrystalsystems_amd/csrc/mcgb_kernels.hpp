@@ -245,20 +245,32 @@ struct PassLds {
     }
 };
 
+// Pair energy of the moved particle (orientation ui) and stencil particle j at fixed-point
+// separation (ox, oy, oz) = q_j - q_i.  Returns false when the pair is beyond the cutoff (the
+// decision is taken on the FP64 r^2 in both precisions).
 template <typename Real>
-__device__ inline Real pair_energy(const RepConst& C, Real uix, Real uiy, Real uiz, Real ujx,
-                                   Real ujy, Real ujz, Real dx, Real dy, Real dz, Real r2, bool* cap);
+__device__ inline bool pair_energy(const RepConst& C, const RepDyn& D, Real uix, Real uiy, Real uiz,
+                                   Real ujx, Real ujy, Real ujz, int ox, int oy, int oz, Real* u,
+                                   bool* cap);
 template <>
-__device__ inline float pair_energy<float>(const RepConst& C, float uix, float uiy, float uiz,
-                                           float ujx, float ujy, float ujz, float dx, float dy,
-                                           float dz, float r2, bool* cap) {
-    return gb_pair_f32(C, uix, uiy, uiz, ujx, ujy, ujz, dx, dy, dz, r2, cap);
+__device__ inline bool pair_energy<float>(const RepConst& C, const RepDyn& D, float uix, float uiy,
+                                          float uiz, float ujx, float ujy, float ujz, int ox,
+                                          int oy, int oz, float* u, bool* cap) {
+    const double dx = (double)ox * D.s[0], dy = (double)oy * D.s[1], dz = (double)oz * D.s[2];
+    const double r2 = dx * dx + dy * dy + dz * dz;
+    if (!(r2 < C.rc2)) return false;
+    *u = gb_pair_f32(C, uix, uiy, uiz, ujx, ujy, ujz, (float)dx, (float)dy, (float)dz, r2, cap);
+    return true;
 }
 template <>
-__device__ inline double pair_energy<double>(const RepConst& C, double uix, double uiy, double uiz,
-                                             double ujx, double ujy, double ujz, double dx,
-                                             double dy, double dz, double r2, bool* cap) {
-    return gb_pair_f64(C, uix, uiy, uiz, ujx, ujy, ujz, dx, dy, dz, r2, cap);
+__device__ inline bool pair_energy<double>(const RepConst& C, const RepDyn& D, double uix,
+                                           double uiy, double uiz, double ujx, double ujy,
+                                           double ujz, int ox, int oy, int oz, double* u, bool* cap) {
+    const double dx = (double)ox * D.s[0], dy = (double)oy * D.s[1], dz = (double)oz * D.s[2];
+    const double r2 = dx * dx + dy * dy + dz * dz;
+    if (!(r2 < C.rc2)) return false;
+    *u = gb_pair_f64(C, uix, uiy, uiz, ujx, ujy, ujz, dx, dy, dz, r2, cap);
+    return true;
 }
 
 // Per-lane accumulator of a finite energy difference: compensated in FP32, plain in FP64.
@@ -386,8 +398,8 @@ __global__ __launch_bounds__(64) void k_pass(PassArgs P) {
     int dcap_sum = 0, dccap_sum = 0;
     long long n_cand = 0;
     double dfin_sum = 0.0, moved = 0.0;
-    const Real sx = (Real)D.s[0], sy = (Real)D.s[1], sz = (Real)D.s[2];
-    const Real rc2 = sizeof(Real) == 4 ? (Real)C.f_rc2 : (Real)C.rc2;
+    const float sx = (float)D.s[0], sy = (float)D.s[1], sz = (float)D.s[2];
+    const float rc2 = C.f_rc2_pre;       // pre-filter; the cutoff itself is decided in FP64
 
     if (nc == 0) {
         if (P.trace)
@@ -478,11 +490,11 @@ __global__ __launch_bounds__(64) void k_pass(PassArgs P) {
                                 if (e < S && e != li) {
                                     const int ox = (int)(sqx[e] - qix), oy = (int)(sqy[e] - qiy),
                                               oz = (int)(sqz[e] - qiz);
-                                    const Real fx = (Real)ox * sx, fy = (Real)oy * sy, fz = (Real)oz * sz;
-                                    const Real hx2 = (Real)(ox - dqx) * sx, hy2 = (Real)(oy - dqy) * sy,
-                                               hz2 = (Real)(oz - dqz) * sz;
-                                    const Real r2o = fx * fx + fy * fy + fz * fz;
-                                    const Real r2n = hx2 * hx2 + hy2 * hy2 + hz2 * hz2;
+                                    const float fx = (float)ox * sx, fy = (float)oy * sy, fz = (float)oz * sz;
+                                    const float hx2 = (float)(ox - dqx) * sx, hy2 = (float)(oy - dqy) * sy,
+                                                hz2 = (float)(oz - dqz) * sz;
+                                    const float r2o = fx * fx + fy * fy + fz * fz;
+                                    const float r2n = hx2 * hx2 + hy2 * hy2 + hz2 * hz2;
                                     keep = (r2o < rc2) | (r2n < rc2);
                                 }
                                 const unsigned long long mask = __ballot(keep);
@@ -495,15 +507,11 @@ __global__ __launch_bounds__(64) void k_pass(PassArgs P) {
                                 const int e = slist[it >> 1];
                                 const bool nw = it & 1;
                                 const uint32_t px = nw ? qnx : qix, py = nw ? qny : qiy, pz = nw ? qnz : qiz;
-                                const Real fx = (Real)(int)(sqx[e] - px) * sx;
-                                const Real fy = (Real)(int)(sqy[e] - py) * sy;
-                                const Real fz = (Real)(int)(sqz[e] - pz) * sz;
-                                const Real r2 = fx * fx + fy * fy + fz * fz;
-                                if (r2 < rc2) {
-                                    bool cp;
-                                    const Real u = pair_energy<Real>(C, nw ? unx : uix, nw ? uny : uiy,
-                                                                     nw ? unz : uiz, sux[e], suy[e], suz[e],
-                                                                     fx, fy, fz, r2, &cp);
+                                bool cp;
+                                Real u;
+                                if (pair_energy<Real>(C, D, nw ? unx : uix, nw ? uny : uiy, nw ? unz : uiz,
+                                                      sux[e], suy[e], suz[e], (int)(sqx[e] - px),
+                                                      (int)(sqy[e] - py), (int)(sqz[e] - pz), &u, &cp)) {
                                     ++pr;
                                     if (cp) capd += nw ? 1 : -1;
                                     else {
@@ -522,15 +530,12 @@ __global__ __launch_bounds__(64) void k_pass(PassArgs P) {
 #pragma unroll
                                 for (int st = 0; st < 2; ++st) {
                                     const bool nw = st == 1;
-                                    const Real fx = (Real)(int)(jx - (nw ? qnx : qix)) * sx;
-                                    const Real fy = (Real)(int)(jy - (nw ? qny : qiy)) * sy;
-                                    const Real fz = (Real)(int)(jz - (nw ? qnz : qiz)) * sz;
-                                    const Real r2 = fx * fx + fy * fy + fz * fz;
-                                    if (r2 < rc2) {
-                                        bool cp;
-                                        const Real u = pair_energy<Real>(C, nw ? unx : uix, nw ? uny : uiy,
-                                                                         nw ? unz : uiz, ujx, ujy, ujz,
-                                                                         fx, fy, fz, r2, &cp);
+                                    bool cp;
+                                    Real u;
+                                    if (pair_energy<Real>(C, D, nw ? unx : uix, nw ? uny : uiy, nw ? unz : uiz,
+                                                          ujx, ujy, ujz, (int)(jx - (nw ? qnx : qix)),
+                                                          (int)(jy - (nw ? qny : qiy)),
+                                                          (int)(jz - (nw ? qnz : qiz)), &u, &cp)) {
                                         ++pr;
                                         if (cp) capd += nw ? 1 : -1;
                                         else {
@@ -890,11 +895,11 @@ __global__ void k_pair_batch(RepConst C, int n, int prec, const double* __restri
         // the FP32 kernel takes unit vectors (the sweep stores normalised orientations)
         double ni = 1.0 / sqrt(ei[3 * k] * ei[3 * k] + ei[3 * k + 1] * ei[3 * k + 1] + ei[3 * k + 2] * ei[3 * k + 2]);
         double nj = 1.0 / sqrt(ej[3 * k] * ej[3 * k] + ej[3 * k + 1] * ej[3 * k + 1] + ej[3 * k + 2] * ej[3 * k + 2]);
-        float dx = (float)dr[3 * k], dy = (float)dr[3 * k + 1], dz = (float)dr[3 * k + 2];
+        const double ddx = dr[3 * k], ddy = dr[3 * k + 1], ddz = dr[3 * k + 2];
         v = (double)gb_pair_f32(C, (float)(ei[3 * k] * ni), (float)(ei[3 * k + 1] * ni),
                                 (float)(ei[3 * k + 2] * ni), (float)(ej[3 * k] * nj),
-                                (float)(ej[3 * k + 1] * nj), (float)(ej[3 * k + 2] * nj), dx, dy, dz,
-                                dx * dx + dy * dy + dz * dz, &cp);
+                                (float)(ej[3 * k + 1] * nj), (float)(ej[3 * k + 2] * nj), (float)ddx,
+                                (float)ddy, (float)ddz, ddx * ddx + ddy * ddy + ddz * ddz, &cp);
     }
     capped[k] = cp ? 1 : 0;
     u[k] = cp ? 1.0e9 : v;

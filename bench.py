@@ -73,6 +73,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-sweeps", type=int, default=1)
     ap.add_argument("--n", type=int, default=256000)
+    ap.add_argument("--relax", type=int, default=200,
+                    help="untimed sweeps from the lattice start before tuning and warm-up")
+    ap.add_argument("--waves", type=int, default=0, help="wavefronts per cell (0: autotune)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the sweep is not "
@@ -84,8 +87,18 @@ def main():
     N, rho, T, R = args.n, 0.32, 1.0, 5.0
     p, pos, u, box = workload(N, rho, T, R)
     seed = int(p.seed)
+    if args.waves:
+        p.waves_per_cell = args.waves
     with m.Context(p) as ctx:
         ctx.set_state(pos, u, box)
+        t_relax = time.perf_counter()
+        if args.relax:
+            ctx.sweep(args.relax)
+        t_relax = time.perf_counter() - t_relax
+        tune = None
+        if not args.waves:
+            best, ms = ctx.autotune(4)
+            tune = {"waves_per_cell": best, "ms_per_pass": ms}
         if args.warmup:
             ctx.sweep(args.warmup)
         ctx.reset_counters()
@@ -127,6 +140,10 @@ def main():
         "ms_sweep_device": kt["ms_sweep"],
         "ess_per_s": {"E/N": ess_per_s(e, dt), "P2": ess_per_s(p2, dt), "samples": int(len(e))},
         "max_fp64_drift": st["max_drift"],
+        "resummations_timed": st["resummations"],
+        "relax_sweeps": args.relax,
+        "relax_s": t_relax,
+        "autotune": tune,
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(pos, u, box, N, rho, T, R, seed, sweeps=args.cpu_sweeps)

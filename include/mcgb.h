@@ -73,6 +73,7 @@ typedef struct mcgb_params {
     double   lattice_jitter;                /* init: uniform jitter amplitude of sites     */
     double   colloid_exclusion_radius;      /* init: sites closer than this are removed    */
     int      stage_capacity;                /* particles staged in LDS per stencil, 0: auto */
+    int      waves_per_cell;                /* wavefronts per active cell: 1, 2, 4 or 8     */
 
     uint32_t keys_read;                     /* bit i set: required key i was read          */
 } mcgb_params;
@@ -114,6 +115,7 @@ typedef struct mcgb_stats {
     double   volume;
     double   density;              /* N / (V - V_colloid), MCnptColloid.c:2799-2804          */
     double   max_drift;            /* largest |running - recomputed| seen at a re-summation  */
+    uint64_t resummations;         /* FP64 re-summations of the running energy              */
 } mcgb_stats;
 
 typedef struct mcgb_ctx mcgb_ctx;
@@ -211,6 +213,12 @@ int mcgb_order_parameter(mcgb_ctx *ctx, int replica, double *p2, double director
 int mcgb_series(mcgb_ctx *ctx, int replica, int max,
                 double *energy_per_particle, double *p2, double *density);
 int mcgb_series_clear(mcgb_ctx *ctx);
+
+/* Autotuner: times `sweeps` sweeps with each supported number of wavefronts per active cell
+ * (1, 2, 4, 8; the LDS staging capacity follows) and keeps the fastest.  The timed sweeps are
+ * ordinary sweeps of the simulation (they advance it).  No counterpart in the reference.
+ * ms_per_pass (may be NULL) receives the measured pass times in that order. */
+int mcgb_autotune(mcgb_ctx *ctx, int sweeps, int *best_waves, double ms_per_pass[4]);
 
 /* Average device time per launch of the sweep-pass kernel, the cell-list rebuild and the whole
  * sweep, measured with HIP events on the context's stream over the last mcgb_sweep() call. */

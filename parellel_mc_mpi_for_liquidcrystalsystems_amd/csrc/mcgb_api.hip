@@ -54,6 +54,8 @@ struct mcgb_ctx {
     std::vector<int> state_set;
     int nblk = 0, Ntot = 0, Nmax = 0, ncell_tot = 0, ncs_tot = 0, ncell_max = 0;
     int cap = 0;
+    int cap_request = 0;
+    int waves = 4;
     size_t lds = 0;
     int resum_interval = 100;
     int series_cap = kSeriesCapDefault;
@@ -70,6 +72,10 @@ struct mcgb_ctx {
     int *cell_id = nullptr, *tmp = nullptr, *count = nullptr, *cell_start = nullptr;
     int *cell_start2 = nullptr, *list2 = nullptr;
     CellAcc* acc = nullptr;
+    MoveDraw* draws = nullptr;
+    int rec_max = 0;
+    int* ovf_count = nullptr;
+    int* ovf_list = nullptr;
     double* efin = nullptr;
     int *ecap = nullptr, *eccap = nullptr, *ehard = nullptr;
     double* obs = nullptr;
@@ -78,6 +84,9 @@ struct mcgb_ctx {
     size_t dtrace_cap = 0;
     std::vector<void*> allocs;
     Events events;
+    hipEvent_t tev[22] = {};
+    hipGraphExec_t graph[2] = {nullptr, nullptr};
+    bool use_graph = true;
     bool timing = true;
     double ms_pass = 0, ms_rebuild = 0, ms_sweep = 0;
 };
@@ -233,74 +242,146 @@ int enqueue_observe(mcgb_ctx* c, int advance, int record) {
     return MCGB_OK;
 }
 
+template <typename Real, bool TR>
+void launch_pass_t(mcgb_ctx* c, const PassArgs& P) {
+    switch (c->waves) {
+    case 1: k_pass<Real, 1, false, TR><<<c->nblk, 64, c->lds, c->st>>>(P); break;
+    case 2: k_pass<Real, 2, false, TR><<<c->nblk, 128, c->lds, c->st>>>(P); break;
+    case 8: k_pass<Real, 8, false, TR><<<c->nblk, 512, c->lds, c->st>>>(P); break;
+    default: k_pass<Real, 4, false, TR><<<c->nblk, 256, c->lds, c->st>>>(P); break;
+    }
+    // cells whose stencil exceeded the staging capacity: one single-wavefront workgroup per
+    // overflow-list entry reads the stencil from HBM (with an empty list every workgroup exits
+    // at once); only the stencil table, draws and partials live in LDS
+    PassArgs Q = P;
+    Q.cap = 32;
+    k_pass<Real, 1, true, TR><<<c->nblk, 64, PassLds<Real, 1>::bytes(32), c->st>>>(Q);
+}
+
+template <typename Real>
+void launch_pass(mcgb_ctx* c, const PassArgs& P) {
+    if (P.trace) launch_pass_t<Real, true>(c, P);
+    else launch_pass_t<Real, false>(c, P);
+}
+
+template <typename Real>
+size_t pass_lds(int waves, int cap) {
+    switch (waves) {
+    case 1: return PassLds<Real, 1>::bytes(cap);
+    case 2: return PassLds<Real, 2>::bytes(cap);
+    case 8: return PassLds<Real, 8>::bytes(cap);
+    default: return PassLds<Real, 4>::bytes(cap);
+    }
+}
+
+// One sweep of every replica.  ev (may be null): 11 events recorded at the sweep start, after
+// the rebuild, after each of the 8 passes and at the end.
+template <typename Real>
+int enqueue_one(mcgb_ctx* c, mcgb_move_trace* trace, const hipEvent_t* ev) {
+    int rc;
+    if (ev) HIPCHK(hipEventRecord(ev[0], c->st));
+    k_begin<<<(c->nrep + 63) / 64, 64, 0, c->st>>>(c->dC, c->dD, c->nrep, c->resum_interval);
+    k_draws<<<dim3((c->rec_max + 255) / 256, c->nrep), 256, 0, c->st>>>(c->dC, c->dD, c->draws);
+    k_zero8<<<1, 64, 0, c->st>>>(c->ovf_count);
+    if ((rc = enqueue_rebuild<Real>(c))) return rc;
+    if (ev) HIPCHK(hipEventRecord(ev[1], c->st));
+    PassArgs P;
+    P.RC = c->dC; P.RD = c->dD; P.blk_rep = c->dBlk;
+    const int b = c->cur;
+    P.qx = c->q[b][0]; P.qy = c->q[b][1]; P.qz = c->q[b][2];
+    P.ux = c->u[b][0]; P.uy = c->u[b][1]; P.uz = c->u[b][2];
+    P.pid = c->pid[b];
+    P.cell_start = c->cell_start;
+    P.acc = c->acc;
+    P.trace = trace;
+    P.draws = c->draws;
+    P.ovf_count = c->ovf_count;
+    P.ovf_list = c->ovf_list;
+    P.nblk = c->nblk;
+    P.cap = c->cap;
+    for (int p = 0; p < 8; ++p) {
+        P.pass = p;
+        launch_pass<Real>(c, P);
+        if (ev) HIPCHK(hipEventRecord(ev[2 + p], c->st));
+    }
+    HIPCHK(hipGetLastError());
+    k_end_moves<<<c->nrep, 256, 0, c->st>>>(c->dC, c->dD, c->acc);
+    if ((rc = enqueue_energy<Real>(c, 1))) return rc;
+    if (c->any_npt && (rc = enqueue_npt<Real>(c))) return rc;
+    if ((rc = enqueue_observe<Real>(c, 1, 1))) return rc;
+    if (ev) HIPCHK(hipEventRecord(ev[10], c->st));
+    return MCGB_OK;
+}
+
+// The sweep sequence of two consecutive sweeps (the particle buffers swap once per sweep, so
+// two sweeps return to the starting buffer) captured as a HIP graph for the current buffer.
+template <typename Real>
+int build_graph(mcgb_ctx* c) {
+    const int par = c->cur;
+    if (c->graph[par]) return MCGB_OK;
+    hipGraph_t g;
+    HIPCHK(hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
+    int rc = enqueue_one<Real>(c, nullptr, nullptr);
+    if (!rc) rc = enqueue_one<Real>(c, nullptr, nullptr);
+    hipError_t ec = hipStreamEndCapture(c->st, &g);
+    if (rc) return rc;
+    if (ec != hipSuccess) {
+        mcgb_set_error("graph capture failed: %s", hipGetErrorString(ec));
+        return MCGB_ERR_HIP;
+    }
+    hipError_t ei = hipGraphInstantiate(&c->graph[par], g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (ei != hipSuccess) {
+        c->graph[par] = nullptr;
+        mcgb_set_error("graph instantiation failed: %s", hipGetErrorString(ei));
+        return MCGB_ERR_HIP;
+    }
+    return MCGB_OK;
+}
+
 template <typename Real>
 int enqueue_sweeps(mcgb_ctx* c, int n_sweeps, mcgb_move_trace* trace) {
     int rc;
-    hipEvent_t e0 = nullptr, e1 = nullptr;
     const bool tm = c->timing;
-    double tp = 0, tr = 0, tsw = 0;
-    int evi = 0;
-    std::vector<int> marks;   // event indices: per sweep 11 marks
-    for (int s = 0; s < n_sweeps; ++s) {
-        if (tm) {
-            if (c->events.get(evi, &e0)) return MCGB_ERR_HIP;
-            HIPCHK(hipEventRecord(e0, c->st));
+    int timed_sets = 0;   // event sets (of 11) holding the last sweeps, in c->tev
+    if (n_sweeps <= 0) return MCGB_OK;
+    // graph replay for the bulk; with timing on, the last two sweeps are launched one kernel
+    // at a time between HIP events (events inside a graph cannot be timed)
+    const int tail = tm ? std::min(n_sweeps, 2) : 0;
+    int bulk = n_sweeps - tail;
+    if (trace || !c->use_graph || bulk < 2) {
+        for (int s = 0; s < bulk; ++s)
+            if ((rc = enqueue_one<Real>(c, trace, nullptr))) return rc;
+    } else {
+        if (bulk & 1) {
+            if ((rc = enqueue_one<Real>(c, nullptr, nullptr))) return rc;
+            --bulk;
         }
-        k_begin<<<(c->nrep + 63) / 64, 64, 0, c->st>>>(c->dC, c->dD, c->nrep, c->resum_interval);
-        if ((rc = enqueue_rebuild<Real>(c))) return rc;
-        if (tm) {
-            if (c->events.get(evi + 1, &e1)) return MCGB_ERR_HIP;
-            HIPCHK(hipEventRecord(e1, c->st));
-        }
-        PassArgs P;
-        P.RC = c->dC; P.RD = c->dD; P.blk_rep = c->dBlk;
-        const int b = c->cur;
-        P.qx = c->q[b][0]; P.qy = c->q[b][1]; P.qz = c->q[b][2];
-        P.ux = c->u[b][0]; P.uy = c->u[b][1]; P.uz = c->u[b][2];
-        P.pid = c->pid[b];
-        P.cell_start = c->cell_start;
-        P.acc = c->acc;
-        P.trace = trace;
-        P.cap = c->cap;
-        for (int p = 0; p < 8; ++p) {
-            P.pass = p;
-            k_pass<Real><<<c->nblk, 64, c->lds, c->st>>>(P);
-            if (tm) {
-                hipEvent_t ep;
-                if (c->events.get(evi + 2 + p, &ep)) return MCGB_ERR_HIP;
-                HIPCHK(hipEventRecord(ep, c->st));
-            }
-        }
-        HIPCHK(hipGetLastError());
-        k_end_moves<<<c->nrep, 256, 0, c->st>>>(c->dC, c->dD, c->acc);
-        if ((rc = enqueue_energy<Real>(c, 1))) return rc;
-        if (c->any_npt && (rc = enqueue_npt<Real>(c))) return rc;
-        if ((rc = enqueue_observe<Real>(c, 1, 1))) return rc;
-        if (tm) {
-            hipEvent_t ee;
-            if (c->events.get(evi + 10, &ee)) return MCGB_ERR_HIP;
-            HIPCHK(hipEventRecord(ee, c->st));
-            marks.push_back(evi);
-            evi += 11;
-        }
+        if ((rc = build_graph<Real>(c))) return rc;
+        const int par = c->cur;
+        for (int i = 0; i < bulk / 2; ++i) HIPCHK(hipGraphLaunch(c->graph[par], c->st));
     }
+    for (int s = 0; s < tail; ++s)
+        if ((rc = enqueue_one<Real>(c, trace, c->tev + 11 * s))) return rc;
+    timed_sets = tail;
     HIPCHK(hipStreamSynchronize(c->st));
-    if (tm && !marks.empty()) {
-        for (int m : marks) {
+    if (timed_sets > 0) {
+        double tp = 0, tr = 0, tsw = 0;
+        for (int k = 0; k < timed_sets; ++k) {
+            const hipEvent_t* e = c->tev + 11 * k;
             float ms;
-            HIPCHK(hipEventElapsedTime(&ms, c->events.ev[m], c->events.ev[m + 1]));
+            HIPCHK(hipEventElapsedTime(&ms, e[0], e[1]));
             tr += ms;
             for (int p = 0; p < 8; ++p) {
-                HIPCHK(hipEventElapsedTime(&ms, c->events.ev[m + 1 + p], c->events.ev[m + 2 + p]));
+                HIPCHK(hipEventElapsedTime(&ms, e[1 + p], e[2 + p]));
                 tp += ms;
             }
-            HIPCHK(hipEventElapsedTime(&ms, c->events.ev[m], c->events.ev[m + 10]));
+            HIPCHK(hipEventElapsedTime(&ms, e[0], e[10]));
             tsw += ms;
         }
-        const double ns = (double)marks.size();
-        c->ms_rebuild = tr / ns;
-        c->ms_pass = tp / (8.0 * ns);
-        c->ms_sweep = tsw / ns;
+        c->ms_rebuild = tr / timed_sets;
+        c->ms_pass = tp / (8.0 * timed_sets);
+        c->ms_sweep = tsw / timed_sets;
     }
     return MCGB_OK;
 }
@@ -355,6 +436,10 @@ void mcgb_destroy(mcgb_ctx* c) {
     if (!c) return;
     (void)hipSetDevice(c->device);
     if (c->st) (void)hipStreamSynchronize(c->st);
+    for (int k = 0; k < 2; ++k)
+        if (c->graph[k]) (void)hipGraphExecDestroy(c->graph[k]);
+    for (int k = 0; k < 22; ++k)
+        if (c->tev[k]) (void)hipEventDestroy(c->tev[k]);
     for (void* p : c->allocs) (void)hipFree(p);
     if (c->st) (void)hipStreamDestroy(c->st);
     delete c;
@@ -393,6 +478,7 @@ int mcgb_create(const mcgb_params* p, int n_replicas, mcgb_ctx** out) {
     c->resum_interval = p[0].energy_check_interval;
     HIPCHK(hipGetDevice(&c->device));
     HIPCHK(hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking));
+    for (int k = 0; k < 22; ++k) HIPCHK(hipEventCreate(&c->tev[k]));
 
     // ---- layout of every replica
     c->hC.resize(n_replicas);
@@ -431,6 +517,8 @@ int mcgb_create(const mcgb_params* p, int n_replicas, mcgb_ctx** out) {
         C.nact_cap = ncell / 8;
         C.key0 = (uint32_t)pr.seed;
         C.key1 = (uint32_t)(pr.seed >> 32) + (uint32_t)r;
+        C.rec_off = c->Ntot + c->ncell_tot;
+        c->rec_max = std::max(c->rec_max, pr.num_particles + ncell);
         c->Ntot += C.N;
         c->Nmax = std::max(c->Nmax, C.N);
         c->ncell_tot += ncell;
@@ -446,15 +534,23 @@ int mcgb_create(const mcgb_params* p, int n_replicas, mcgb_ctx** out) {
         memset(&Dr, 0, sizeof(Dr));
         for (int d = 0; d < 3; ++d) Dr.L[d] = L;
     }
-    // LDS staging capacity: 1.25x the mean 3x3x3 stencil plus slack, within 64 KiB
-    const size_t per = 12 + 3 * c->rsize + 2;
-    const int capmax = (int)((65536 - 256 - 64 * (4 + 12 + 3 * c->rsize + 8) - 64) / per) & ~31;
-    int cap = (int)(27.0 * occ_max * 1.25 + 96.0);
+    // wavefronts per active cell and LDS staging capacity (1.15x the mean 3x3x3 stencil plus
+    // 64, within 64 KiB; a larger stencil takes the HBM path)
+    c->waves = p[0].waves_per_cell;
+    if (c->waves != 1 && c->waves != 2 && c->waves != 4 && c->waves != 8) {
+        mcgb_set_error("waves_per_cell must be 1, 2, 4 or 8 (got %d)", c->waves);
+        mcgb_destroy(c);
+        return MCGB_ERR_ARG;
+    }
+    int cap = (int)(27.0 * occ_max * 1.15 + 64.0);
     cap = (cap + 31) & ~31;
     if (p[0].stage_capacity > 0) cap = (p[0].stage_capacity + 31) & ~31;
-    cap = std::max(32, std::min(cap, capmax));
+    cap = std::max(32, cap);
+    c->cap_request = cap;
+    while (cap > 32 && (c->rsize == 8 ? pass_lds<double>(c->waves, cap) : pass_lds<float>(c->waves, cap)) > 65536)
+        cap -= 32;
     c->cap = cap;
-    c->lds = c->rsize == 8 ? PassLds<double>::bytes(cap) : PassLds<float>::bytes(cap);
+    c->lds = c->rsize == 8 ? pass_lds<double>(c->waves, cap) : pass_lds<float>(c->waves, cap);
 
     // ---- device memory
     int rc = 0;
@@ -471,6 +567,9 @@ int mcgb_create(const mcgb_params* p, int n_replicas, mcgb_ctx** out) {
     if (!rc) rc = dalloc_t(c, &c->cell_start, c->ncs_tot);
     if (!rc) rc = dalloc_t(c, &c->cell_start2, c->ncs_tot);
     if (!rc) rc = dalloc_t(c, &c->acc, c->ncell_tot);
+    if (!rc) rc = dalloc_t(c, &c->draws, (size_t)c->Ntot + c->ncell_tot);
+    if (!rc) rc = dalloc_t(c, &c->ovf_count, 8);
+    if (!rc) rc = dalloc_t(c, &c->ovf_list, (size_t)8 * c->nblk);
     if (!rc) rc = dalloc_t(c, &c->efin, N);
     if (!rc) rc = dalloc_t(c, &c->ecap, N);
     if (!rc) rc = dalloc_t(c, &c->eccap, N);
@@ -786,6 +885,7 @@ int mcgb_get_stats(mcgb_ctx* c, int r, mcgb_stats* out) {
     out->volume = d.L[0] * d.L[1] * d.L[2];
     out->density = c->hC[r].N / (out->volume - c->hC[r].vcoll);
     out->max_drift = d.max_drift;
+    out->resummations = d.resums;
     return MCGB_OK;
 }
 
@@ -798,6 +898,7 @@ int mcgb_reset_counters(mcgb_ctx* c, int r) {
     d.attempted = d.accepted = d.left = d.hard = d.cand = d.pairs = d.coll = d.vol_att = d.vol_acc = 0;
     d.max_drift = 0.0;
     d.overflow = 0;
+    d.resums = 0;
     return write_dyn(c, D);
 }
 
@@ -884,6 +985,39 @@ int mcgb_series_clear(mcgb_ctx* c) {
     if ((rc = read_dyn(c, D))) return rc;
     for (auto& d : D) d.series_count = 0;
     return write_dyn(c, D);
+}
+
+static int set_waves(mcgb_ctx* c, int waves) {
+    for (int k = 0; k < 2; ++k)
+        if (c->graph[k]) { (void)hipGraphExecDestroy(c->graph[k]); c->graph[k] = nullptr; }
+    c->waves = waves;
+    int cap = c->cap_request;
+    while (cap > 32 && (c->rsize == 8 ? pass_lds<double>(waves, cap) : pass_lds<float>(waves, cap)) > 65536)
+        cap -= 32;
+    c->cap = cap;
+    c->lds = c->rsize == 8 ? pass_lds<double>(waves, cap) : pass_lds<float>(waves, cap);
+    return MCGB_OK;
+}
+
+int mcgb_autotune(mcgb_ctx* c, int sweeps, int* best_waves, double ms_per_pass[4]) {
+    int rc = check_ctx(c, 0);
+    if (rc) return rc;
+    if (sweeps <= 0) { mcgb_set_error("mcgb_autotune: sweeps must be > 0"); return MCGB_ERR_ARG; }
+    const int cand[4] = {1, 2, 4, 8};
+    const bool tm = c->timing;
+    c->timing = true;
+    double best = 1e300;
+    int bw = c->waves;
+    for (int k = 0; k < 4; ++k) {
+        set_waves(c, cand[k]);
+        if ((rc = sweep_impl(c, sweeps, nullptr))) { c->timing = tm; return rc; }
+        if (ms_per_pass) ms_per_pass[k] = c->ms_pass;
+        if (c->ms_pass < best) { best = c->ms_pass; bw = cand[k]; }
+    }
+    set_waves(c, bw);
+    c->timing = tm;
+    if (best_waves) *best_waves = bw;
+    return MCGB_OK;
 }
 
 int mcgb_kernel_times(mcgb_ctx* c, double* ms_pass, double* ms_rebuild, double* ms_sweep) {

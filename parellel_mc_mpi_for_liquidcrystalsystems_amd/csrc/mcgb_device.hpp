@@ -29,7 +29,7 @@ struct RepConst {
     uint32_t key0, key1;   // Philox key
     int colloid_on;        // colloid_wall_option == 1
     int npt;               // NPT ln V moves
-    int pad0;
+    int rec_off;           // first per-move draw record (capacity N + ncell_cap)
     // physics, FP64 master copies (get_system_const, MCnptColloid.c:2752-2783)
     double sigma0, four_eps0, chi, chi2, chip, chip2, mu, nu, rc, rc2;
     double temperature, beta, pressure, dmax, drot;
@@ -69,6 +69,28 @@ struct RepDyn {
     int trial_ok;
     int pad1;
     double e_res[4];             // last full recompute {cap_pairs, cap_coll, fin, hard}
+    unsigned long long resums;   // re-summations of the running energy
+};
+
+// Draws of one trial move (k_draws): particle selector, fixed-point step, rotation vector,
+// Metropolis threshold -log(U) T.
+struct MoveDraw {
+    uint32_t j;
+    int32_t dq[3];
+    double thr;
+    double v[3];
+};
+
+// FP32 constants of the pair function, held in registers by the pass kernel.
+struct GBf {
+    float sigma0, four_eps0, chi2, chip2, mu, half_nu;
+    float g1c0, g1c1, g1d0, g2c0, g2c1, g2d0;
+    double sigma0d;
+};
+
+// FP64 constants of the pair function.
+struct GBd {
+    double sigma0, four_eps0, chi, chi2, chip, mu, nu;
 };
 
 // Per-active-cell accumulators of one pass (each cell is active exactly once per sweep).
@@ -131,7 +153,7 @@ __host__ __device__ inline int cell_dim(double L, double rc) {
 // the exact FP64 r^2 (r2d) with one FP64 Newton step on the hardware rsqrt, and
 // r - sigma + sigma0 is formed in FP64: the remaining error is that of sigma (a few FP32 ulps).
 // Transcendentals: v_rsq x2, v_rcp x2, v_log x3, v_exp x1.  *cap = the 1e9 cap.
-__device__ inline float gb_pair_f32(const RepConst& C, float uix, float uiy, float uiz,
+__device__ inline float gb_pair_f32(const GBf& C, float uix, float uiy, float uiz,
                                     float ujx, float ujy, float ujz, float dx, float dy, float dz,
                                     double r2d, bool* cap) {
     const float r2 = (float)r2d;
@@ -146,23 +168,23 @@ __device__ inline float gb_pair_f32(const RepConst& C, float uix, float uiy, flo
     const float ab = (px * px + py * py + pz * pz + qx * qx + qy * qy + qz * qz) * ir2;
     const float tt = tau * tau * ir2;
     // shape g(chi) and energy g(chi') as N / D
-    const float n1 = C.f_g1c0 + C.f_g1c1 * ab + C.f_chi2 * tt;
-    const float d1 = C.f_g1d0 + C.f_chi2 * w2;
-    const float n2 = C.f_g2c0 + C.f_g2c1 * ab + C.f_chip2 * tt;
-    const float d2 = C.f_g2d0 + C.f_chip2 * w2;
+    const float n1 = C.g1c0 + C.g1c1 * ab + C.chi2 * tt;
+    const float d1 = C.g1d0 + C.chi2 * w2;
+    const float n2 = C.g2c0 + C.g2c1 * ab + C.chip2 * tt;
+    const float d2 = C.g2d0 + C.chip2 * w2;
     const float eps = __builtin_amdgcn_exp2f(
-        C.f_mu * (__builtin_amdgcn_logf(n2) - __builtin_amdgcn_logf(d2)) -
-        C.f_half_nu * __builtin_amdgcn_logf(d1));
-    const float sigma = C.f_sigma0 * __builtin_amdgcn_rsqf(n1 * __builtin_amdgcn_rcpf(d1));
+        C.mu * (__builtin_amdgcn_logf(n2) - __builtin_amdgcn_logf(d2)) -
+        C.half_nu * __builtin_amdgcn_logf(d1));
+    const float sigma = C.sigma0 * __builtin_amdgcn_rsqf(n1 * __builtin_amdgcn_rcpf(d1));
     // r in FP64: one Newton step on the FP32 reciprocal square root
     const double y = (double)rinv;
     const double y1 = y * (1.5 - 0.5 * r2d * y * y);
-    const double dist = (r2d * y1 - (double)sigma) + C.sigma0;
+    const double dist = (r2d * y1 - (double)sigma) + C.sigma0d;
     *cap = dist < 0.001;
-    const float rho = C.f_sigma0 * __builtin_amdgcn_rcpf((float)dist);
+    const float rho = C.sigma0 * __builtin_amdgcn_rcpf((float)dist);
     const float rho2 = rho * rho;
     const float rho6 = rho2 * rho2 * rho2;
-    return C.f_four_eps0 * eps * rho6 * (rho6 - 1.0f);
+    return C.four_eps0 * eps * rho6 * (rho6 - 1.0f);
 }
 
 // ------------------------------------------------------------------ GB pair, FP64
@@ -174,7 +196,7 @@ __device__ inline double g_omega64(double c, double ap2, double am2, double w) {
 
 // Term for term as pij_GB (MCnptColloid.c:2618-2681) with the inverse norms and cosine of GBij
 // (:1524-1535); returns the finite energy, *cap = the 1e9 cap.
-__device__ inline double gb_pair_f64(const RepConst& C, double uix, double uiy, double uiz,
+__device__ inline double gb_pair_f64(const GBd& C, double uix, double uiy, double uiz,
                                      double ujx, double ujy, double ujz, double dx, double dy,
                                      double dz, double r2, bool* cap) {
     double rij = sqrt(r2);
@@ -203,8 +225,9 @@ __device__ inline double gb_pair_f64(const RepConst& C, double uix, double uiy, 
 // ------------------------------------------------------------------ colloid-rod, FP64
 
 // GBCollRod (MCnptColloid.c:1758-1854); rel = rod centre - colloid centre.  Evaluated in FP64
-// in both precisions (two calls per move, only near the colloid).  *cap = the 1e8 cap.
-__device__ inline double colloid_f64(const RepConst& C, double rx, double ry, double rz,
+// in both precisions (two calls per move, only near the colloid; kept out of line so that the
+// sweep kernel does not carry its registers).  *cap = the 1e8 cap.
+__device__ __attribute__((noinline)) double colloid_f64(const RepConst& C, double rx, double ry, double rz,
                                      double ex, double ey, double ez, bool* cap) {
     *cap = false;
     double d = sqrt(rx * rx + ry * ry + rz * rz);
@@ -227,6 +250,23 @@ __device__ inline double colloid_f64(const RepConst& C, double rx, double ry, do
     double s3 = s0 * s0 * s0, s9 = s3 * s3 * s3;
     double x2 = x * x, x3 = x2 * x, x8 = x3 * x3 * x2, x9 = x8 * x;
     return eps * (s9 / (45.0 * x9) - s3 / (6.0 * x3) - s9 / (40.0 * d * x8) + s3 / (4.0 * d * x2));
+}
+
+__host__ __device__ inline GBf make_gbf(const RepConst& C) {
+    GBf g;
+    g.sigma0 = C.f_sigma0; g.four_eps0 = C.f_four_eps0; g.chi2 = C.f_chi2; g.chip2 = C.f_chip2;
+    g.mu = C.f_mu; g.half_nu = C.f_half_nu;
+    g.g1c0 = C.f_g1c0; g.g1c1 = C.f_g1c1; g.g1d0 = C.f_g1d0;
+    g.g2c0 = C.f_g2c0; g.g2c1 = C.f_g2c1; g.g2d0 = C.f_g2d0;
+    g.sigma0d = C.sigma0;
+    return g;
+}
+
+__host__ __device__ inline GBd make_gbd(const RepConst& C) {
+    GBd g;
+    g.sigma0 = C.sigma0; g.four_eps0 = C.four_eps0; g.chi = C.chi; g.chi2 = C.chi2;
+    g.chip = C.chip; g.mu = C.mu; g.nu = C.nu;
+    return g;
 }
 
 // ------------------------------------------------------------------ compensated sums

@@ -8,6 +8,7 @@
 #pragma once
 
 #include "mcgb_device.hpp"
+#include "mcgb_pass.hpp"
 #include "mcgb.h"
 
 namespace mcgb {
@@ -70,6 +71,10 @@ __global__ void k_begin(const RepConst* __restrict__ RC, RepDyn* __restrict__ RD
     D.k = (C.N + nc - 1) / nc;
     D.moved_abs = 0.0;
     D.resum = (resum_interval > 0 && ((t + 1) % (uint32_t)resum_interval) == 0) ? 1 : 0;
+}
+
+__global__ void k_zero8(int* __restrict__ p) {
+    if (threadIdx.x < 8) p[threadIdx.x] = 0;
 }
 
 // Host-requested cell list with zero shift (set_state, explicit recompute): grid from D.L.
@@ -221,397 +226,50 @@ __global__ __launch_bounds__(256) void k_sort_gather(
     }
 }
 
-// ------------------------------------------------------------------ sweep pass
+// ------------------------------------------------------------------ per-move draws
 
-struct PassArgs {
-    const RepConst* RC;
-    RepDyn* RD;
-    const int* blk_rep;
-    uint32_t *qx, *qy, *qz;
-    void *ux, *uy, *uz;
-    const int* pid;
-    const int* cell_start;
-    CellAcc* acc;
-    mcgb_move_trace* trace;
-    int pass;
-    int cap;          // LDS staging capacity (particles)
-};
-
-template <typename Real>
-struct PassLds {
-    // byte offsets inside the dynamic LDS block
-    static __host__ __device__ inline size_t bytes(int cap) {
-        return 256 + (size_t)cap * (12 + 3 * sizeof(Real) + 2) + 64 * (4 + 12 + 3 * sizeof(Real) + 8) + 64;
-    }
-};
-
-// Pair energy of the moved particle (orientation ui) and stencil particle j at fixed-point
-// separation (ox, oy, oz) = q_j - q_i.  Returns false when the pair is beyond the cutoff (the
-// decision is taken on the FP64 r^2 in both precisions).
-template <typename Real>
-__device__ inline bool pair_energy(const RepConst& C, const RepDyn& D, Real uix, Real uiy, Real uiz,
-                                   Real ujx, Real ujy, Real ujz, int ox, int oy, int oz, Real* u,
-                                   bool* cap);
-template <>
-__device__ inline bool pair_energy<float>(const RepConst& C, const RepDyn& D, float uix, float uiy,
-                                          float uiz, float ujx, float ujy, float ujz, int ox,
-                                          int oy, int oz, float* u, bool* cap) {
-    const double dx = (double)ox * D.s[0], dy = (double)oy * D.s[1], dz = (double)oz * D.s[2];
-    const double r2 = dx * dx + dy * dy + dz * dz;
-    if (!(r2 < C.rc2)) return false;
-    *u = gb_pair_f32(C, uix, uiy, uiz, ujx, ujy, ujz, (float)dx, (float)dy, (float)dz, r2, cap);
-    return true;
-}
-template <>
-__device__ inline bool pair_energy<double>(const RepConst& C, const RepDyn& D, double uix,
-                                           double uiy, double uiz, double ujx, double ujy,
-                                           double ujz, int ox, int oy, int oz, double* u, bool* cap) {
-    const double dx = (double)ox * D.s[0], dy = (double)oy * D.s[1], dz = (double)oz * D.s[2];
-    const double r2 = dx * dx + dy * dy + dz * dz;
-    if (!(r2 < C.rc2)) return false;
-    *u = gb_pair_f64(C, uix, uiy, uiz, ujx, ujy, ujz, dx, dy, dz, r2, cap);
-    return true;
-}
-
-// Per-lane accumulator of a finite energy difference: compensated in FP32, plain in FP64.
-template <typename Real> struct DeltaAcc;
-template <> struct DeltaAcc<float> {
-    float hi = 0.0f, lo = 0.0f;
-    __device__ void add(float v) { two_sum(hi, lo, v); }
-    __device__ double reduce() {
-#pragma unroll
-        for (int off = 32; off >= 1; off >>= 1) {
-            float oh = __shfl_xor(hi, off), ol = __shfl_xor(lo, off);
-            two_sum(hi, lo, oh);
-            lo += ol;
-        }
-        return bcast((double)hi + (double)lo);
-    }
-};
-template <> struct DeltaAcc<double> {
-    double s = 0.0;
-    __device__ void add(double v) { s += v; }
-    __device__ double reduce() { return wave_sum(s); }
-};
-
-// Flat stencil index e -> global particle slot (segments of the 27 neighbour cells).
-__device__ inline int stencil_slot(const int* s_off, const int* s_start, int part_off, int e) {
-    int lo = 0, hi = 27;
-    while (hi - lo > 1) {
-        int mid = (lo + hi) >> 1;
-        if (s_off[mid] <= e) lo = mid; else hi = mid;
-    }
-    return part_off + s_start[lo] + (e - s_off[lo]);
-}
-
-// One sublattice pass: a wavefront per active cell makes the cell's k trial moves in sequence
-// (the TODO body of MCnptColloid.c:683-700 filled with attempt_monte_carlo_move, :2445-2604).
-// The 3x3x3 stencil is staged in LDS as SoA; each move tests every stencil particle against the
-// old and trial positions, compacts those inside the cutoff of either, and evaluates both GB
-// energies per listed pair across the 64 lanes.  A stencil larger than the staging capacity is
-// read from global memory instead (same arithmetic, no staging).
-template <typename Real>
-__global__ __launch_bounds__(64) void k_pass(PassArgs P) {
-    extern __shared__ __align__(16) unsigned char smem[];
-    const int b = blockIdx.x;
-    const int r = P.blk_rep[b];
-    const RepConst& C = P.RC[r];
-    RepDyn& D = P.RD[r];
-    const int a = b - C.act_off;
-    const int nact = D.nact;
-    if (a >= nact) return;
-    const int lane = threadIdx.x;
-    const int cap = P.cap;
-    Real* __restrict__ Ux = (Real*)P.ux;
-    Real* __restrict__ Uy = (Real*)P.uy;
-    Real* __restrict__ Uz = (Real*)P.uz;
-
-    int* s_start = (int*)smem;            // 32
-    int* s_off = s_start + 32;            // 32 (28 used)
-    uint32_t* sqx = (uint32_t*)(smem + 256);
-    uint32_t* sqy = sqx + cap;
-    uint32_t* sqz = sqy + cap;
-    Real* sux = (Real*)(sqz + cap);
-    Real* suy = sux + cap;
-    Real* suz = suy + cap;
-    unsigned short* slist = (unsigned short*)(suz + cap);
-    unsigned char* dp = (unsigned char*)(slist + cap);
-    dp = (unsigned char*)(((uintptr_t)dp + 15) & ~(uintptr_t)15);
-    double* sthr = (double*)dp;                       // 64
-    Real* svx = (Real*)(sthr + 64);                   // 3 x 64
-    Real* svy = svx + 64;
-    Real* svz = svy + 64;
-    int* sdqx = (int*)(svz + 64);                     // 3 x 64
-    int* sdqy = sdqx + 64;
-    int* sdqz = sdqy + 64;
-    uint32_t* sj = (uint32_t*)(sdqz + 64);            // 64
-
-    const int nx = D.n[0], ny = D.n[1], nz = D.n[2];
+// Draws of every trial move of the sweep (all 8 passes): record p*nact*k + a*k + m of the
+// replica holds move m of active cell a in pass p: particle selector, symmetric fixed-point step
+// (MCnptColloid.c:2502-2509), rotation vector on the unit sphere (:2526-2550) and the
+// Metropolis threshold -log(U) T (:2566-2574), from A = philox({m, cell, t, p<<8}) and
+// B = philox({m, cell, t, (p<<8)|1}).
+__global__ __launch_bounds__(256) void k_draws(const RepConst* __restrict__ RC,
+                                               const RepDyn* __restrict__ RD,
+                                               MoveDraw* __restrict__ rec) {
+    const int r = blockIdx.y;
+    const RepConst& C = RC[r];
+    const RepDyn& D = RD[r];
+    const int nact = D.nact, k = D.k;
+    const int total = 8 * nact * k;
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= total) return;
+    const int p = idx / (nact * k);
+    const int rem = idx - p * nact * k;
+    const int a = rem / k;
+    const int m = rem - a * k;
+    const int nx = D.n[0], ny = D.n[1];
     const int hx = nx >> 1, hy = ny >> 1;
-    const uint32_t sig = D.order[P.pass];
-    const int ax = a % hx, ay = (a / hx) % hy, az = a / (hx * hy);
-    const int cx = 2 * ax + (int)(sig & 1u), cy = 2 * ay + (int)((sig >> 1) & 1u),
-              cz = 2 * az + (int)((sig >> 2) & 1u);
-    const int cell = cx + nx * (cy + ny * cz);
-    const int* cs = P.cell_start + C.cs_off;
-    const uint32_t gx = D.g[0], gy = D.g[1], gz = D.g[2];
-
-    // stencil table: 27 neighbour cells in (dz, dy, dx) order, centre = 13
-    int cnt = 0;
-    if (lane < 27) {
-        int ex = cx + (lane % 3) - 1, ey = cy + ((lane / 3) % 3) - 1, ez = cz + lane / 9 - 1;
-        ex += ex < 0 ? nx : 0; ex -= ex >= nx ? nx : 0;
-        ey += ey < 0 ? ny : 0; ey -= ey >= ny ? ny : 0;
-        ez += ez < 0 ? nz : 0; ez -= ez >= nz ? nz : 0;
-        int nb = ex + nx * (ey + ny * ez);
-        int st = cs[nb];
-        cnt = cs[nb + 1] - st;
-        s_start[lane] = st;
-    }
-    int incl = cnt;
-#pragma unroll
-    for (int off = 1; off < 32; off <<= 1) {
-        int v = __shfl_up(incl, off);
-        if (lane >= off) incl += v;
-    }
-    if (lane < 27) s_off[lane + 1] = incl;
-    if (lane == 0) s_off[0] = 0;
-    __syncthreads();
-    const int S = s_off[27];
-    const int coff = s_off[13];
-    const int nc = s_off[14] - coff;
-    const int cstart = C.part_off + s_start[13];
-    const bool staged = S <= cap;
-    if (!staged && lane == 0) atomicOr(&D.overflow, 1);
-    if (staged) {
-        for (int e = lane; e < S; e += 64) {
-            int gi = stencil_slot(s_off, s_start, C.part_off, e);
-            sqx[e] = P.qx[gi]; sqy[e] = P.qy[gi]; sqz[e] = P.qz[gi];
-            sux[e] = Ux[gi]; suy[e] = Uy[gi]; suz[e] = Uz[gi];
-        }
-    }
-
-    const int k = D.k;
-    const uint32_t t = D.t;
-    const size_t tbase = ((size_t)P.pass * (size_t)nact + (size_t)a) * (size_t)k;
-    int n_att = 0, n_acc = 0, n_left = 0, n_hard = 0, n_pairs = 0, n_coll = 0;
-    int dcap_sum = 0, dccap_sum = 0;
-    long long n_cand = 0;
-    double dfin_sum = 0.0, moved = 0.0;
-    const float sx = (float)D.s[0], sy = (float)D.s[1], sz = (float)D.s[2];
-    const float rc2 = C.f_rc2_pre;       // pre-filter; the cutoff itself is decided in FP64
-
-    if (nc == 0) {
-        if (P.trace)
-            for (int m = lane; m < k; m += 64) {
-                mcgb_move_trace rec = {-1, MCGB_MOVE_EMPTY, 0, 0, 0.0, 0.0};
-                P.trace[tbase + m] = rec;
-            }
-    } else {
-        for (int m0 = 0; m0 < k; m0 += 64) {
-            __syncthreads();
-            {   // lane-parallel draws of up to 64 moves
-                const int m = m0 + lane;
-                if (m < k) {
-                    uint32_t A[4], B[4];
-                    philox4x32_10((uint32_t)m, (uint32_t)cell, t, (uint32_t)P.pass << 8, C.key0, C.key1, A);
-                    philox4x32_10((uint32_t)m, (uint32_t)cell, t, ((uint32_t)P.pass << 8) | 1u, C.key0, C.key1, B);
-                    sj[lane] = A[0];
-                    sdqx[lane] = step_q(A[1], D.K[0]);
-                    sdqy[lane] = step_q(A[2], D.K[1]);
-                    sdqz[lane] = step_q(A[3], D.K[2]);
-                    double zz = 1.0 - 2.0 * u01(B[0]);
-                    double sxy = sqrt(fmax(0.0, 1.0 - zz * zz));
-                    double sn, cn;
-                    sincospi(2.0 * u01(B[1]), &sn, &cn);
-                    svx[lane] = (Real)(sxy * cn);
-                    svy[lane] = (Real)(sxy * sn);
-                    svz[lane] = (Real)zz;
-                    sthr[lane] = -log(u01(B[2])) * C.temperature;
-                }
-            }
-            __syncthreads();
-            const int mend = min(k, m0 + 64);
-            for (int m = m0; m < mend; ++m) {
-                const int ml = m - m0;
-                const int jsel = (int)(((uint64_t)sj[ml] * (uint32_t)nc) >> 32);
-                const int li = coff + jsel;          // flat stencil index of the moved particle
-                const int gsel = cstart + jsel;      // its global slot
-                uint32_t qix, qiy, qiz;
-                Real uix, uiy, uiz;
-                if (staged) {
-                    qix = sqx[li]; qiy = sqy[li]; qiz = sqz[li];
-                    uix = sux[li]; uiy = suy[li]; uiz = suz[li];
-                } else {
-                    qix = P.qx[gsel]; qiy = P.qy[gsel]; qiz = P.qz[gsel];
-                    uix = Ux[gsel]; uiy = Uy[gsel]; uiz = Uz[gsel];
-                }
-                const int dqx = sdqx[ml], dqy = sdqy[ml], dqz = sdqz[ml];
-                const uint32_t qnx = qix + (uint32_t)dqx, qny = qiy + (uint32_t)dqy,
-                               qnz = qiz + (uint32_t)dqz;
-                int outcome;
-                int capd = 0, ccd = 0;
-                double dfin = 0.0, scale = 0.0;
-                ++n_att;
-                if (cell_coord(qnx, gx, (uint32_t)nx) != (uint32_t)cx ||
-                    cell_coord(qny, gy, (uint32_t)ny) != (uint32_t)cy ||
-                    cell_coord(qnz, gz, (uint32_t)nz) != (uint32_t)cz) {
-                    outcome = MCGB_MOVE_LEFT;       // moves are confined to the active cell
-                    ++n_left;
-                } else {
-                    const double rnx = (double)(int32_t)(qnx - 0x80000000u) * D.s[0];
-                    const double rny = (double)(int32_t)(qny - 0x80000000u) * D.s[1];
-                    const double rnz = (double)(int32_t)(qnz - 0x80000000u) * D.s[2];
-                    if (C.colloid_on && rnx * rnx + rny * rny + rnz * rnz <= C.R2) {
-                        outcome = MCGB_MOVE_HARD;   // MCnptColloid.c:2516-2520
-                        ++n_hard;
-                    } else {
-                        // trial orientation normalize(u + d_rot v) (MCnptColloid.c:2526-2550)
-                        double tx = (double)uix + C.drot * (double)svx[ml];
-                        double ty = (double)uiy + C.drot * (double)svy[ml];
-                        double tz = (double)uiz + C.drot * (double)svz[ml];
-                        double n2 = tx * tx + ty * ty + tz * tz;
-                        Real unx, uny, unz;
-                        if (n2 <= 1e-12) {
-                            unx = uix; uny = uiy; unz = uiz;
-                        } else {
-                            double nr = sqrt(n2);
-                            unx = (Real)(tx / nr); uny = (Real)(ty / nr); unz = (Real)(tz / nr);
-                        }
-                        DeltaAcc<Real> acc;
-                        Real scl = 0;
-                        int pr = 0;
-                        if (staged) {
-                            // candidate test against both states, wave-level compaction
-                            int base = 0;
-                            for (int e0 = 0; e0 < S; e0 += 64) {
-                                const int e = e0 + lane;
-                                bool keep = false;
-                                if (e < S && e != li) {
-                                    const int ox = (int)(sqx[e] - qix), oy = (int)(sqy[e] - qiy),
-                                              oz = (int)(sqz[e] - qiz);
-                                    const float fx = (float)ox * sx, fy = (float)oy * sy, fz = (float)oz * sz;
-                                    const float hx2 = (float)(ox - dqx) * sx, hy2 = (float)(oy - dqy) * sy,
-                                                hz2 = (float)(oz - dqz) * sz;
-                                    const float r2o = fx * fx + fy * fy + fz * fz;
-                                    const float r2n = hx2 * hx2 + hy2 * hy2 + hz2 * hz2;
-                                    keep = (r2o < rc2) | (r2n < rc2);
-                                }
-                                const unsigned long long mask = __ballot(keep);
-                                if (keep) slist[base + __popcll(mask & __lanemask_lt())] = (unsigned short)e;
-                                base += __popcll(mask);
-                            }
-                            __syncthreads();
-                            const int items = 2 * base;
-                            for (int it = lane; it < items; it += 64) {
-                                const int e = slist[it >> 1];
-                                const bool nw = it & 1;
-                                const uint32_t px = nw ? qnx : qix, py = nw ? qny : qiy, pz = nw ? qnz : qiz;
-                                bool cp;
-                                Real u;
-                                if (pair_energy<Real>(C, D, nw ? unx : uix, nw ? uny : uiy, nw ? unz : uiz,
-                                                      sux[e], suy[e], suz[e], (int)(sqx[e] - px),
-                                                      (int)(sqy[e] - py), (int)(sqz[e] - pz), &u, &cp)) {
-                                    ++pr;
-                                    if (cp) capd += nw ? 1 : -1;
-                                    else {
-                                        acc.add(nw ? u : -u);
-                                        scl += u < 0 ? -u : u;
-                                    }
-                                }
-                            }
-                        } else {
-                            // global-memory path: both states per candidate, no compaction
-                            for (int e = lane; e < S; e += 64) {
-                                if (e == li) continue;
-                                const int gj = stencil_slot(s_off, s_start, C.part_off, e);
-                                const uint32_t jx = P.qx[gj], jy = P.qy[gj], jz = P.qz[gj];
-                                const Real ujx = Ux[gj], ujy = Uy[gj], ujz = Uz[gj];
-#pragma unroll
-                                for (int st = 0; st < 2; ++st) {
-                                    const bool nw = st == 1;
-                                    bool cp;
-                                    Real u;
-                                    if (pair_energy<Real>(C, D, nw ? unx : uix, nw ? uny : uiy, nw ? unz : uiz,
-                                                          ujx, ujy, ujz, (int)(jx - (nw ? qnx : qix)),
-                                                          (int)(jy - (nw ? qny : qiy)),
-                                                          (int)(jz - (nw ? qnz : qiz)), &u, &cp)) {
-                                        ++pr;
-                                        if (cp) capd += nw ? 1 : -1;
-                                        else {
-                                            acc.add(nw ? u : -u);
-                                            scl += u < 0 ? -u : u;
-                                        }
-                                    }
-                                }
-                            }
-                        }
-                        // colloid-rod term of both states (lanes 0 and 1), FP64
-                        double cacc = 0.0, cscl = 0.0;
-                        if (C.colloid_on && lane < 2) {
-                            const bool nw = lane == 1;
-                            const double rx = (double)(int32_t)((nw ? qnx : qix) - 0x80000000u) * D.s[0];
-                            const double ry = (double)(int32_t)((nw ? qny : qiy) - 0x80000000u) * D.s[1];
-                            const double rz = (double)(int32_t)((nw ? qnz : qiz) - 0x80000000u) * D.s[2];
-                            bool cp;
-                            const double u = colloid_f64(C, rx, ry, rz, (double)(nw ? unx : uix),
-                                                         (double)(nw ? uny : uiy), (double)(nw ? unz : uiz), &cp);
-                            if (cp) ccd = nw ? 1 : -1;
-                            else {
-                                cacc = nw ? u : -u;
-                                cscl = fabs(u);
-                            }
-                        }
-                        dfin = acc.reduce() + wave_sum(cacc);
-                        capd = wave_sum(capd);
-                        ccd = wave_sum(ccd);
-                        pr = wave_sum(pr);
-                        scale = wave_sum((double)scl + cscl);
-                        n_pairs += pr;
-                        n_cand += 2LL * (S - 1);
-                        if (C.colloid_on) n_coll += 2;
-                        const double dE = (double)capd * 1.0e9 + (double)ccd * 1.0e8 + dfin;
-                        if (dE <= sthr[ml]) {                 // log U <= -beta dE (MCnptColloid.c:2574)
-                            outcome = MCGB_MOVE_ACCEPT;
-                            ++n_acc;
-                            dcap_sum += capd;
-                            dccap_sum += ccd;
-                            dfin_sum += dfin;
-                            moved += fabs(dfin);
-                            if (lane == 0) {
-                                if (staged) {
-                                    sqx[li] = qnx; sqy[li] = qny; sqz[li] = qnz;
-                                    sux[li] = unx; suy[li] = uny; suz[li] = unz;
-                                }
-                                P.qx[gsel] = qnx; P.qy[gsel] = qny; P.qz[gsel] = qnz;
-                                Ux[gsel] = unx; Uy[gsel] = uny; Uz[gsel] = unz;
-                            }
-                        } else {
-                            outcome = MCGB_MOVE_REJECT;
-                        }
-                    }
-                }
-                if (P.trace && lane == 0) {
-                    mcgb_move_trace rec;
-                    rec.pid = P.pid[gsel];
-                    rec.outcome = outcome;
-                    rec.dcap = capd;
-                    rec.dccap = ccd;
-                    rec.dfin = dfin;
-                    rec.scale = scale;
-                    P.trace[tbase + m] = rec;
-                }
-                __syncthreads();
-            }
-        }
-    }
-    if (lane == 0) {
-        CellAcc ca;
-        ca.att = n_att; ca.acc = n_acc; ca.left = n_left; ca.hard = n_hard;
-        ca.dcap = dcap_sum; ca.dccap = dccap_sum; ca.pairs = n_pairs; ca.coll = n_coll;
-        ca.cand = n_cand; ca.dfin = dfin_sum; ca.moved_abs = moved; ca.pad = 0.0;
-        P.acc[C.cell_off + cell] = ca;
-    }
+    const uint32_t sig = D.order[p];
+    const int cx = 2 * (a % hx) + (int)(sig & 1u), cy = 2 * ((a / hx) % hy) + (int)((sig >> 1) & 1u),
+              cz = 2 * (a / (hx * hy)) + (int)((sig >> 2) & 1u);
+    const uint32_t cell = (uint32_t)(cx + nx * (cy + ny * cz));
+    uint32_t A[4], B[4];
+    philox4x32_10((uint32_t)m, cell, D.t, (uint32_t)p << 8, C.key0, C.key1, A);
+    philox4x32_10((uint32_t)m, cell, D.t, ((uint32_t)p << 8) | 1u, C.key0, C.key1, B);
+    MoveDraw d;
+    d.j = A[0];
+    d.dq[0] = step_q(A[1], D.K[0]);
+    d.dq[1] = step_q(A[2], D.K[1]);
+    d.dq[2] = step_q(A[3], D.K[2]);
+    const double zz = 1.0 - 2.0 * u01(B[0]);
+    const double sxy = sqrt(fmax(0.0, 1.0 - zz * zz));
+    double sn, cn;
+    sincospi(2.0 * u01(B[1]), &sn, &cn);
+    d.v[0] = sxy * cn;
+    d.v[1] = sxy * sn;
+    d.v[2] = zz;
+    d.thr = -log(u01(B[2])) * C.temperature;
+    rec[C.rec_off + idx] = d;
 }
 
 // ------------------------------------------------------------------ end of the passes
@@ -691,6 +349,7 @@ __global__ __launch_bounds__(128) void k_energy(
     const int cx = (int)cell_coord(xi, g0, n0), cy = (int)cell_coord(yi, g1, n1),
               cz = (int)cell_coord(zi, g2, n2);
     const int* cs = cell_start + C.cs_off;
+    const GBd gd = make_gbd(C);
     double f = 0.0;
     int cp = 0;
     for (int dz = -1; dz <= 1; ++dz)
@@ -711,7 +370,7 @@ __global__ __launch_bounds__(128) void k_energy(
                     const double r2 = ddx * ddx + ddy * ddy + ddz * ddz;
                     if (r2 < C.rc2) {
                         bool c;
-                        const double u = gb_pair_f64(C, uix, uiy, uiz, (double)ux[gj], (double)uy[gj],
+                        const double u = gb_pair_f64(gd, uix, uiy, uiz, (double)ux[gj], (double)uy[gj],
                                                      (double)uz[gj], ddx, ddy, ddz, r2, &c);
                         if (c) ++cp; else f += u;
                     }
@@ -774,6 +433,7 @@ __global__ __launch_bounds__(256) void k_energy_reduce(const RepConst* __restric
     if (mode == 3) return;                             // compute only
     if (mode == 0 || mode == 1) {
         if (mode == 1) {
+            D.resums += 1;
             double run = (double)D.cap_pairs * 1e9 + (double)D.cap_coll * 1e8 + D.e_fin;
             double now = sc[0][0] * 1e9 + sc[0][1] * 1e8 + sf[0];
             double dr = fabs(run - now);
@@ -889,14 +549,14 @@ __global__ void k_pair_batch(RepConst C, int n, int prec, const double* __restri
     double v;
     if (prec == 64) {
         double dx = dr[3 * k], dy = dr[3 * k + 1], dz = dr[3 * k + 2];
-        v = gb_pair_f64(C, ei[3 * k], ei[3 * k + 1], ei[3 * k + 2], ej[3 * k], ej[3 * k + 1],
+        v = gb_pair_f64(make_gbd(C), ei[3 * k], ei[3 * k + 1], ei[3 * k + 2], ej[3 * k], ej[3 * k + 1],
                         ej[3 * k + 2], dx, dy, dz, dx * dx + dy * dy + dz * dz, &cp);
     } else {
         // the FP32 kernel takes unit vectors (the sweep stores normalised orientations)
         double ni = 1.0 / sqrt(ei[3 * k] * ei[3 * k] + ei[3 * k + 1] * ei[3 * k + 1] + ei[3 * k + 2] * ei[3 * k + 2]);
         double nj = 1.0 / sqrt(ej[3 * k] * ej[3 * k] + ej[3 * k + 1] * ej[3 * k + 1] + ej[3 * k + 2] * ej[3 * k + 2]);
         const double ddx = dr[3 * k], ddy = dr[3 * k + 1], ddz = dr[3 * k + 2];
-        v = (double)gb_pair_f32(C, (float)(ei[3 * k] * ni), (float)(ei[3 * k + 1] * ni),
+        v = (double)gb_pair_f32(make_gbf(C), (float)(ei[3 * k] * ni), (float)(ei[3 * k + 1] * ni),
                                 (float)(ei[3 * k + 2] * ni), (float)(ej[3 * k] * nj),
                                 (float)(ej[3 * k + 1] * nj), (float)(ej[3 * k + 2] * nj), (float)ddx,
                                 (float)ddy, (float)ddz, ddx * ddx + ddy * ddy + ddz * ddz, &cp);

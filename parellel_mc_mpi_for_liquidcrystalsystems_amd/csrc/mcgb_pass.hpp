@@ -1,0 +1,548 @@
+// mcgb_pass.hpp — the sublattice-pass kernel: W wavefronts per active cell make the cell's k
+// trial moves in sequence (the TODO body of MCnptColloid.c:683-700 filled with the move of
+// attempt_monte_carlo_move, :2445-2604).
+//
+// Per activation: the 3x3x3 stencil (27 cell segments of the device cell list) is staged in LDS
+// as SoA; the draws of the cell's moves (written by k_draws) are read in chunks of 64.  Per move: every
+// thread forms the trial state; each wavefront tests its share of the stencil against the old
+// and trial positions (FP32 pre-filter), compacts the hits into a wavefront-private LDS list,
+// evaluates both GB energies of every listed pair (cutoff decided on the FP64 r^2), and reduces
+// (DPP) a compensated energy difference; the wavefront partials meet in LDS, every thread takes
+// the same Metropolis decision, thread 0 commits an accepted move to LDS and HBM.
+// Two workgroup barriers per move.  Stencils beyond the staging capacity read HBM directly.
+#pragma once
+
+#include "mcgb_device.hpp"
+#include "mcgb.h"
+
+namespace mcgb {
+
+// ------------------------------------------------------------------ DPP wave reductions
+
+// Sum over the 64 lanes of a wavefront; the total lands in lane 63 (gfx9 DPP row/bank modes).
+template <int CTRL, int ROWMASK>
+__device__ inline int dpp_i(int v) {
+    return __builtin_amdgcn_update_dpp(0, v, CTRL, ROWMASK, 0xF, false);
+}
+template <int CTRL, int ROWMASK>
+__device__ inline float dpp_f(float v) {
+    return __builtin_bit_cast(float, dpp_i<CTRL, ROWMASK>(__builtin_bit_cast(int, v)));
+}
+template <int CTRL, int ROWMASK>
+__device__ inline double dpp_d(double v) {
+    unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+    int lo = dpp_i<CTRL, ROWMASK>((int)(u & 0xFFFFFFFFull));
+    int hi = dpp_i<CTRL, ROWMASK>((int)(u >> 32));
+    return __builtin_bit_cast(double, ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
+}
+
+#define MCGB_DPP_STEPS(X) X(0xB1, 0xF) X(0x4E, 0xF) X(0x141, 0xF) X(0x140, 0xF) X(0x142, 0xA) X(0x143, 0xC)
+
+__device__ inline int wave_sum_i63(int v) {
+#define S_(c, m) v += dpp_i<c, m>(v);
+    MCGB_DPP_STEPS(S_)
+#undef S_
+    return v;
+}
+__device__ inline double wave_sum_d63(double v) {
+#define S_(c, m) v += dpp_d<c, m>(v);
+    MCGB_DPP_STEPS(S_)
+#undef S_
+    return v;
+}
+// compensated (hi, lo) float pair: TwoSum of the hi parts, lo parts added
+__device__ inline void wave_sum_c63(float& hi, float& lo) {
+#define S_(c, m)                         \
+    {                                    \
+        float oh = dpp_f<c, m>(hi);      \
+        float ol = dpp_f<c, m>(lo);      \
+        two_sum(hi, lo, oh);             \
+        lo += ol;                        \
+    }
+    MCGB_DPP_STEPS(S_)
+#undef S_
+}
+
+// ------------------------------------------------------------------ arguments and LDS layout
+
+struct PassArgs {
+    const RepConst* RC;
+    RepDyn* RD;
+    const int* blk_rep;
+    uint32_t *qx, *qy, *qz;
+    void *ux, *uy, *uz;
+    const int* pid;
+    const int* cell_start;
+    CellAcc* acc;
+    mcgb_move_trace* trace;
+    const MoveDraw* draws;
+    int* ovf_count;   // [8] cells of each pass left to the HBM kernel
+    int* ovf_list;    // [8][nblk] their block indices
+    int nblk;
+    int pass;
+    int cap;          // LDS staging capacity (particles)
+};
+
+struct WavePartial {
+    float hi, lo;     // FP32 path: compensated finite dE
+    double d;         // FP64 path: finite dE
+    int capd;         // change of capped pairs
+    int pad;
+    double scale;     // sum |terms| (trace only)
+};
+
+template <typename Real, int W>
+struct PassLds {
+    static constexpr int kDraws = 32;
+    __host__ __device__ static inline int list_cap(int cap) {
+        return ((cap + 64 * W - 1) / (64 * W)) * 64;
+    }
+    __host__ __device__ static inline size_t bytes(int cap) {
+        size_t b = 256;                                   // stencil tables
+        b += (size_t)cap * (12 + 3 * sizeof(Real));       // staged SoA
+        b += (size_t)W * list_cap(cap) * 2;               // wavefront lists (ushort)
+        b = (b + 15) & ~(size_t)15;
+        b += kDraws * (8 + 3 * sizeof(Real) + 12 + 4);    // draws
+        b += W * sizeof(WavePartial) + 16;
+        return b + 16;
+    }
+};
+
+// Box scales and cutoff held in registers for the whole pass.
+struct PairGeom {
+    double s0, s1, s2, rc2;
+};
+
+template <typename Real> struct GBConst;
+template <> struct GBConst<float> { using type = GBf; };
+template <> struct GBConst<double> { using type = GBd; };
+
+__device__ inline GBf gb_const(const RepConst& C, float*) { return make_gbf(C); }
+__device__ inline GBd gb_const(const RepConst& C, double*) { return make_gbd(C); }
+
+// Pair energy of the moved particle (orientation ui) and stencil particle j at fixed-point
+// separation (ox, oy, oz) = q_j - q_i.  Returns false when the pair is beyond the cutoff (the
+// decision is taken on the FP64 r^2 in both precisions).
+__device__ inline bool pair_energy(const GBf& G, const PairGeom& X, float uix, float uiy, float uiz,
+                                   float ujx, float ujy, float ujz, int ox, int oy, int oz, float* u,
+                                   bool* cap) {
+    const double dx = (double)ox * X.s0, dy = (double)oy * X.s1, dz = (double)oz * X.s2;
+    const double r2 = dx * dx + dy * dy + dz * dz;
+    if (!(r2 < X.rc2)) return false;
+    *u = gb_pair_f32(G, uix, uiy, uiz, ujx, ujy, ujz, (float)dx, (float)dy, (float)dz, r2, cap);
+    return true;
+}
+__device__ inline bool pair_energy(const GBd& G, const PairGeom& X, double uix, double uiy,
+                                   double uiz, double ujx, double ujy, double ujz, int ox, int oy,
+                                   int oz, double* u, bool* cap) {
+    const double dx = (double)ox * X.s0, dy = (double)oy * X.s1, dz = (double)oz * X.s2;
+    const double r2 = dx * dx + dy * dy + dz * dz;
+    if (!(r2 < X.rc2)) return false;
+    *u = gb_pair_f64(G, uix, uiy, uiz, ujx, ujy, ujz, dx, dy, dz, r2, cap);
+    return true;
+}
+
+// Per-lane accumulator of a finite energy difference.  FP32: a lane sums only a few terms per
+// move, so it adds plainly and the cross-lane reduction is compensated (TwoSum per DPP step);
+// FP64: plain.
+template <typename Real> struct LaneAcc;
+template <> struct LaneAcc<float> {
+    float hi = 0.0f, lo = 0.0f;
+    __device__ void add(float v) { hi += v; }
+    __device__ void to_partial(WavePartial& p) {
+        wave_sum_c63(hi, lo);
+        p.hi = hi; p.lo = lo; p.d = 0.0;
+    }
+    __device__ static double total(const WavePartial* p, int w) {
+        // fixed order: deterministic
+        double s = 0.0;
+        for (int k = 0; k < w; ++k) s += (double)p[k].hi + (double)p[k].lo;
+        return s;
+    }
+};
+template <> struct LaneAcc<double> {
+    double s = 0.0;
+    __device__ void add(double v) { s += v; }
+    __device__ void to_partial(WavePartial& p) {
+        p.d = wave_sum_d63(s);
+        p.hi = 0.0f; p.lo = 0.0f;
+    }
+    __device__ static double total(const WavePartial* p, int w) {
+        double s = 0.0;
+        for (int k = 0; k < w; ++k) s += p[k].d;
+        return s;
+    }
+};
+
+// Trial orientation normalize(u + d_rot v) (MCnptColloid.c:2526-2550; kept when |.|^2 <= 1e-12).
+__device__ inline void rotate(double drot, float ux, float uy, float uz, float vx, float vy,
+                              float vz, float& ox, float& oy, float& oz) {
+    const float d = (float)drot;
+    const float tx = ux + d * vx, ty = uy + d * vy, tz = uz + d * vz;
+    const float n2 = tx * tx + ty * ty + tz * tz;
+    if (n2 <= 1e-12f) { ox = ux; oy = uy; oz = uz; return; }
+    float rn = __builtin_amdgcn_rsqf(n2);
+    rn = rn * (1.5f - 0.5f * n2 * rn * rn);          // one Newton step
+    ox = tx * rn; oy = ty * rn; oz = tz * rn;
+}
+__device__ inline void rotate(double drot, double ux, double uy, double uz, double vx, double vy,
+                              double vz, double& ox, double& oy, double& oz) {
+    const double tx = ux + drot * vx, ty = uy + drot * vy, tz = uz + drot * vz;
+    const double n2 = tx * tx + ty * ty + tz * tz;
+    if (n2 <= 1e-12) { ox = ux; oy = uy; oz = uz; return; }
+    const double nr = sqrt(n2);
+    ox = tx / nr; oy = ty / nr; oz = tz / nr;
+}
+
+// Flat stencil index e -> global particle slot (segments of the 27 neighbour cells).
+__device__ inline int stencil_slot(const int* s_off, const int* s_start, int part_off, int e) {
+    int lo = 0, hi = 27;
+    while (hi - lo > 1) {
+        int mid = (lo + hi) >> 1;
+        if (s_off[mid] <= e) lo = mid; else hi = mid;
+    }
+    return part_off + s_start[lo] + (e - s_off[lo]);
+}
+
+// GLOBAL = false: one workgroup per active cell, stencil staged in LDS; a cell whose stencil
+// exceeds the staging capacity is appended to the pass's overflow list and left untouched.
+// GLOBAL = true: the workgroups walk that overflow list and read the stencil from HBM.
+// TR = true: every trial move is recorded in P.trace (with the sum of |pair terms|).
+template <typename Real, int W, bool GLOBAL, bool TR>
+__global__ __launch_bounds__(64 * W) void k_pass(PassArgs P) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    int b;
+    if (GLOBAL) {
+        const int nov = P.ovf_count[P.pass];
+        if ((int)blockIdx.x >= nov) return;
+        b = P.ovf_list[(size_t)P.pass * P.nblk + blockIdx.x];
+    } else {
+        b = blockIdx.x;
+    }
+    const int r = P.blk_rep[b];
+    const RepConst& C = P.RC[r];
+    RepDyn& D = P.RD[r];
+    const int a = b - C.act_off;
+    const int nact = D.nact;
+    if (a >= nact) return;                    // uniform: whole block
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int w = tid >> 6;
+    const int cap = P.cap;
+    Real* __restrict__ Ux = (Real*)P.ux;
+    Real* __restrict__ Uy = (Real*)P.uy;
+    Real* __restrict__ Uz = (Real*)P.uz;
+    using L = PassLds<Real, W>;
+    const int lcap = L::list_cap(cap);
+
+    int* s_start = (int*)smem;                                 // 32
+    int* s_off = s_start + 32;                                 // 32 (28 used)
+    uint32_t* sqx = (uint32_t*)(smem + 256);
+    uint32_t* sqy = sqx + cap;
+    uint32_t* sqz = sqy + cap;
+    Real* sux = (Real*)(sqz + cap);
+    Real* suy = sux + cap;
+    Real* suz = suy + cap;
+    unsigned short* slist = (unsigned short*)(suz + cap) + (size_t)w * lcap;
+    unsigned char* dp = (unsigned char*)((unsigned short*)(suz + cap) + (size_t)W * lcap);
+    dp = (unsigned char*)(((uintptr_t)dp + 15) & ~(uintptr_t)15);
+    constexpr int KD = L::kDraws;                              // moves per draw chunk
+    double* sthr = (double*)dp;                                // KD
+    Real* svx = (Real*)(sthr + KD);                            // 3 x KD
+    Real* svy = svx + KD;
+    Real* svz = svy + KD;
+    int* sdqx = (int*)(svz + KD);                              // 3 x KD
+    int* sdqy = sdqx + KD;
+    int* sdqz = sdqy + KD;
+    uint32_t* sj = (uint32_t*)(sdqz + KD);                     // KD
+    WavePartial* sred = (WavePartial*)(((uintptr_t)(sj + KD) + 15) & ~(uintptr_t)15);
+
+    const typename GBConst<Real>::type G = gb_const(C, (Real*)nullptr);
+    const PairGeom X = {D.s[0], D.s[1], D.s[2], C.rc2};
+    const int part_off = C.part_off;
+    const int colloid_on = C.colloid_on;
+    const double R2 = C.R2;
+    const double drot = C.drot;
+    const int nx = D.n[0], ny = D.n[1], nz = D.n[2];
+    const int hx = nx >> 1, hy = ny >> 1;
+    const uint32_t sig = D.order[P.pass];
+    const int ax = a % hx, ay = (a / hx) % hy, az = a / (hx * hy);
+    const int cx = 2 * ax + (int)(sig & 1u), cy = 2 * ay + (int)((sig >> 1) & 1u),
+              cz = 2 * az + (int)((sig >> 2) & 1u);
+    const int cell = cx + nx * (cy + ny * cz);
+    const int* cs = P.cell_start + C.cs_off;
+    const uint32_t gx = D.g[0], gy = D.g[1], gz = D.g[2];
+
+    // stencil table: 27 neighbour cells in (dz, dy, dx) order, centre = 13
+    if (w == 0) {
+        int cnt = 0;
+        if (lane < 27) {
+            int ex = cx + (lane % 3) - 1, ey = cy + ((lane / 3) % 3) - 1, ez = cz + lane / 9 - 1;
+            ex += ex < 0 ? nx : 0; ex -= ex >= nx ? nx : 0;
+            ey += ey < 0 ? ny : 0; ey -= ey >= ny ? ny : 0;
+            ez += ez < 0 ? nz : 0; ez -= ez >= nz ? nz : 0;
+            int nb = ex + nx * (ey + ny * ez);
+            int st = cs[nb];
+            cnt = cs[nb + 1] - st;
+            s_start[lane] = st;
+        }
+        int incl = cnt;
+#pragma unroll
+        for (int off = 1; off < 32; off <<= 1) {
+            int v = __shfl_up(incl, off);
+            if (lane >= off) incl += v;
+        }
+        if (lane < 27) s_off[lane + 1] = incl;
+        if (lane == 0) s_off[0] = 0;
+    }
+    __syncthreads();
+    const int S = s_off[27];
+    const int coff = s_off[13];
+    const int nc = s_off[14] - coff;
+    const int cstart = part_off + s_start[13];
+    constexpr bool staged = !GLOBAL;
+    if (!GLOBAL && S > cap) {                  // uniform: leave this cell to the HBM kernel
+        if (tid == 0) {
+            atomicOr(&D.overflow, 1);
+            const int slot = atomicAdd(&P.ovf_count[P.pass], 1);
+            P.ovf_list[(size_t)P.pass * P.nblk + slot] = b;
+        }
+        return;
+    }
+    if (staged) {
+        for (int e = tid; e < S; e += 64 * W) {
+            int gi = stencil_slot(s_off, s_start, part_off, e);
+            sqx[e] = P.qx[gi]; sqy[e] = P.qy[gi]; sqz[e] = P.qz[gi];
+            sux[e] = Ux[gi]; suy[e] = Uy[gi]; suz[e] = Uz[gi];
+        }
+    }
+
+    const int k = D.k;
+    const size_t tbase = ((size_t)P.pass * (size_t)nact + (size_t)a) * (size_t)k;
+    const MoveDraw* __restrict__ drw = P.draws + C.rec_off + tbase;
+    constexpr bool tracing = TR;
+    int n_att = 0, n_acc = 0, n_left = 0, n_hard = 0, n_coll = 0;
+    int dcap_sum = 0, dccap_sum = 0;
+    int my_pairs = 0;
+    long long n_cand = 0;
+    double dfin_sum = 0.0, moved = 0.0;
+    const float sx = (float)X.s0, sy = (float)X.s1, sz = (float)X.s2;
+    const float rc2p = C.f_rc2_pre;     // pre-filter; the cutoff itself is decided in FP64
+    // colloid-rod terms only matter within R + 4 sigma_ee of the colloid centre
+    const double creach = C.R + C.colloid_cut + 0.5;
+    const double creach2 = creach * creach;
+
+    if (nc == 0) {
+        if (tracing)
+            for (int m = tid; m < k; m += 64 * W) {
+                mcgb_move_trace rec = {-1, MCGB_MOVE_EMPTY, 0, 0, 0.0, 0.0};
+                P.trace[tbase + m] = rec;
+            }
+    } else {
+        for (int m0 = 0; m0 < k; m0 += KD) {
+            __syncthreads();
+            if (tid < KD && m0 + tid < k) {   // the draws of up to KD moves (k_draws)
+                const MoveDraw& d = drw[m0 + tid];
+                sj[tid] = d.j;
+                sdqx[tid] = d.dq[0];
+                sdqy[tid] = d.dq[1];
+                sdqz[tid] = d.dq[2];
+                svx[tid] = (Real)d.v[0];
+                svy[tid] = (Real)d.v[1];
+                svz[tid] = (Real)d.v[2];
+                sthr[tid] = d.thr;
+            }
+            __syncthreads();
+            const int mend = min(k, m0 + KD);
+            for (int m = m0; m < mend; ++m) {
+                const int ml = m - m0;
+                const int jsel = (int)(((uint64_t)sj[ml] * (uint32_t)nc) >> 32);
+                const int li = coff + jsel;          // flat stencil index of the moved particle
+                const int gsel = cstart + jsel;      // its global slot
+                uint32_t qix, qiy, qiz;
+                Real uix, uiy, uiz;
+                if (staged) {
+                    qix = sqx[li]; qiy = sqy[li]; qiz = sqz[li];
+                    uix = sux[li]; uiy = suy[li]; uiz = suz[li];
+                } else {
+                    qix = P.qx[gsel]; qiy = P.qy[gsel]; qiz = P.qz[gsel];
+                    uix = Ux[gsel]; uiy = Uy[gsel]; uiz = Uz[gsel];
+                }
+                const int dqx = sdqx[ml], dqy = sdqy[ml], dqz = sdqz[ml];
+                const uint32_t qnx = qix + (uint32_t)dqx, qny = qiy + (uint32_t)dqy,
+                               qnz = qiz + (uint32_t)dqz;
+                int outcome;
+                int capd = 0, ccd = 0;
+                double dfin = 0.0, scale = 0.0;
+                ++n_att;
+                const double rnx = (double)(int32_t)(qnx - 0x80000000u) * X.s0;
+                const double rny = (double)(int32_t)(qny - 0x80000000u) * X.s1;
+                const double rnz = (double)(int32_t)(qnz - 0x80000000u) * X.s2;
+                if (cell_coord(qnx, gx, (uint32_t)nx) != (uint32_t)cx ||
+                    cell_coord(qny, gy, (uint32_t)ny) != (uint32_t)cy ||
+                    cell_coord(qnz, gz, (uint32_t)nz) != (uint32_t)cz) {
+                    outcome = MCGB_MOVE_LEFT;       // moves are confined to the active cell
+                    ++n_left;
+                } else if (colloid_on && rnx * rnx + rny * rny + rnz * rnz <= R2) {
+                    outcome = MCGB_MOVE_HARD;       // MCnptColloid.c:2516-2520
+                    ++n_hard;
+                } else {
+                    Real unx, uny, unz;
+                    rotate(drot, uix, uiy, uiz, svx[ml], svy[ml], svz[ml], unx, uny, unz);
+                    LaneAcc<Real> acc;
+                    Real scl = 0;
+                    int cpl = 0;
+                    if (staged) {
+                        // candidate test against both states, wavefront-private compaction
+                        int base = 0;
+                        for (int e0 = w * 64; e0 < S; e0 += 64 * W) {
+                            const int e = e0 + lane;
+                            bool keep = false;
+                            if (e < S && e != li) {
+                                const int ox = (int)(sqx[e] - qix), oy = (int)(sqy[e] - qiy),
+                                          oz = (int)(sqz[e] - qiz);
+                                const float fx = (float)ox * sx, fy = (float)oy * sy, fz = (float)oz * sz;
+                                const float hx2 = (float)(ox - dqx) * sx, hy2 = (float)(oy - dqy) * sy,
+                                            hz2 = (float)(oz - dqz) * sz;
+                                const float r2o = fx * fx + fy * fy + fz * fz;
+                                const float r2n = hx2 * hx2 + hy2 * hy2 + hz2 * hz2;
+                                keep = (r2o < rc2p) | (r2n < rc2p);
+                            }
+                            const unsigned long long mask = __ballot(keep);
+                            if (keep) slist[base + __popcll(mask & __lanemask_lt())] = (unsigned short)e;
+                            base += __popcll(mask);
+                        }
+                        // the list is private to this wavefront: a wavefront-scope fence suffices
+                        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                        __builtin_amdgcn_wave_barrier();
+                        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                        const int items = 2 * base;
+                        for (int it = lane; it < items; it += 64) {
+                            const int e = slist[it >> 1];
+                            const bool nw = it & 1;
+                            const uint32_t px = nw ? qnx : qix, py = nw ? qny : qiy, pz = nw ? qnz : qiz;
+                            bool cp;
+                            Real u;
+                            if (pair_energy(G, X, nw ? unx : uix, nw ? uny : uiy, nw ? unz : uiz,
+                                            sux[e], suy[e], suz[e], (int)(sqx[e] - px),
+                                            (int)(sqy[e] - py), (int)(sqz[e] - pz), &u, &cp)) {
+                                ++my_pairs;
+                                if (cp) cpl += nw ? 1 : -1;
+                                else {
+                                    acc.add(nw ? u : -u);
+                                    if (TR) scl += u < 0 ? -u : u;
+                                }
+                            }
+                        }
+                    } else {
+                        // HBM path (stencil beyond the staging capacity): both states per
+                        // candidate, no compaction
+                        for (int e = tid; e < S; e += 64 * W) {
+                            if (e == li) continue;
+                            const int gj = stencil_slot(s_off, s_start, part_off, e);
+                            const uint32_t jx = P.qx[gj], jy = P.qy[gj], jz = P.qz[gj];
+                            const Real ujx = Ux[gj], ujy = Uy[gj], ujz = Uz[gj];
+                            for (int st = 0; st < 2; ++st) {
+                                const bool nw = st == 1;
+                                bool cp;
+                                Real u;
+                                if (pair_energy(G, X, nw ? unx : uix, nw ? uny : uiy, nw ? unz : uiz,
+                                                ujx, ujy, ujz, (int)(jx - (nw ? qnx : qix)),
+                                                (int)(jy - (nw ? qny : qiy)), (int)(jz - (nw ? qnz : qiz)),
+                                                &u, &cp)) {
+                                    ++my_pairs;
+                                    if (cp) cpl += nw ? 1 : -1;
+                                    else {
+                                        acc.add(nw ? u : -u);
+                                        if (TR) scl += u < 0 ? -u : u;
+                                    }
+                                }
+                            }
+                        }
+                    }
+                    // wavefront partials -> LDS
+                    WavePartial part;
+                    acc.to_partial(part);
+                    part.capd = wave_sum_i63(cpl);
+                    part.scale = tracing ? wave_sum_d63((double)scl) : 0.0;
+                    if (lane == 63) sred[w] = part;
+                    // colloid-rod term of both states in FP64, only near the colloid
+                    double cacc = 0.0, cscl = 0.0;
+                    if (colloid_on) {
+                        const double rox = (double)(int32_t)(qix - 0x80000000u) * X.s0;
+                        const double roy = (double)(int32_t)(qiy - 0x80000000u) * X.s1;
+                        const double roz = (double)(int32_t)(qiz - 0x80000000u) * X.s2;
+                        const bool nearc = rox * rox + roy * roy + roz * roz <= creach2 ||
+                                           rnx * rnx + rny * rny + rnz * rnz <= creach2;
+                        n_coll += 2;
+                        if (nearc) {
+                            bool co, cn;
+                            const double eo = colloid_f64(C, rox, roy, roz, (double)uix, (double)uiy,
+                                                          (double)uiz, &co);
+                            const double en = colloid_f64(C, rnx, rny, rnz, (double)unx, (double)uny,
+                                                          (double)unz, &cn);
+                            ccd = (cn ? 1 : 0) - (co ? 1 : 0);
+                            cacc = (cn ? 0.0 : en) - (co ? 0.0 : eo);
+                            cscl = fabs(cn ? 0.0 : en) + fabs(co ? 0.0 : eo);
+                        }
+                    }
+                    __syncthreads();
+                    dfin = LaneAcc<Real>::total(sred, W) + cacc;
+                    capd = 0;
+                    scale = cscl;
+                    for (int q = 0; q < W; ++q) {
+                        capd += sred[q].capd;
+                        scale += sred[q].scale;
+                    }
+                    n_cand += 2LL * (S - 1);
+                    const double dE = (double)capd * 1.0e9 + (double)ccd * 1.0e8 + dfin;
+                    if (dE <= sthr[ml]) {                 // log U <= -beta dE (MCnptColloid.c:2574)
+                        outcome = MCGB_MOVE_ACCEPT;
+                        ++n_acc;
+                        dcap_sum += capd;
+                        dccap_sum += ccd;
+                        dfin_sum += dfin;
+                        moved += fabs(dfin);
+                        if (tid == 0) {
+                            if (staged) {
+                                sqx[li] = qnx; sqy[li] = qny; sqz[li] = qnz;
+                                sux[li] = unx; suy[li] = uny; suz[li] = unz;
+                            }
+                            P.qx[gsel] = qnx; P.qy[gsel] = qny; P.qz[gsel] = qnz;
+                            Ux[gsel] = unx; Uy[gsel] = uny; Uz[gsel] = unz;
+                        }
+                    } else {
+                        outcome = MCGB_MOVE_REJECT;
+                    }
+                }
+                if (tracing && tid == 0) {
+                    mcgb_move_trace rec;
+                    rec.pid = P.pid[gsel];
+                    rec.outcome = outcome;
+                    rec.dcap = capd;
+                    rec.dccap = ccd;
+                    rec.dfin = dfin;
+                    rec.scale = scale;
+                    P.trace[tbase + m] = rec;
+                }
+                __syncthreads();
+            }
+        }
+    }
+    // per-cell accumulators (pairs counted per lane)
+    const int wp = wave_sum_i63(my_pairs);
+    __syncthreads();
+    if (lane == 63) sred[w].capd = wp;
+    __syncthreads();
+    if (tid == 0) {
+        int np = 0;
+        for (int q = 0; q < W; ++q) np += sred[q].capd;
+        CellAcc ca;
+        ca.att = n_att; ca.acc = n_acc; ca.left = n_left; ca.hard = n_hard;
+        ca.dcap = dcap_sum; ca.dccap = dccap_sum; ca.pairs = np; ca.coll = n_coll;
+        ca.cand = n_cand; ca.dfin = dfin_sum; ca.moved_abs = moved; ca.pad = 0.0;
+        P.acc[C.cell_off + cell] = ca;
+    }
+}
+
+}  // namespace mcgb

@@ -45,7 +45,7 @@ class Params(C.Structure):
         ("seed", C.c_uint64), ("precision", C.c_int), ("npt_max_dlnv", C.c_double),
         ("energy_check_interval", C.c_int), ("lattice_jitter", C.c_double),
         ("colloid_exclusion_radius", C.c_double), ("stage_capacity", C.c_int),
-        ("keys_read", C.c_uint32)]
+        ("waves_per_cell", C.c_int), ("keys_read", C.c_uint32)]
 
 
 class MoveTrace(C.Structure):
@@ -63,7 +63,7 @@ class Stats(C.Structure):
         "candidates", "pairs", "colloid_evals", "volume_attempted", "volume_accepted")] + [
         ("cap_pairs", C.c_int64), ("cap_colloid", C.c_int64), ("energy_finite", C.c_double),
         ("energy", C.c_double), ("box", C.c_double * 3), ("volume", C.c_double),
-        ("density", C.c_double), ("max_drift", C.c_double)]
+        ("density", C.c_double), ("max_drift", C.c_double), ("resummations", C.c_uint64)]
 
     def as_dict(self):
         d = {n: getattr(self, n) for n, _ in self._fields_}
@@ -108,6 +108,7 @@ def lib():
         "mcgb_series": (C.c_int, [ctx, C.c_int, C.c_int, dp, dp, dp]),
         "mcgb_series_clear": (C.c_int, [ctx]),
         "mcgb_kernel_times": (C.c_int, [ctx, dp, dp, dp]),
+        "mcgb_autotune": (C.c_int, [ctx, C.c_int, ip, dp]),
         "mcgb_cell_list": (C.c_int, [ctx, C.c_int, C.c_uint32, ip, C.c_int, ip, C.c_int, up, up, ip]),
         "mcgb_pair_energy_batch": (C.c_int, [C.POINTER(Params), C.c_int, dp, dp, dp, C.c_int, dp, ip]),
         "mcgb_colloid_energy_batch": (C.c_int, [C.POINTER(Params), C.c_int, dp, dp, C.c_int, dp, ip]),
@@ -342,6 +343,12 @@ class Context:
 
     def series_clear(self):
         check(lib().mcgb_series_clear(self.h))
+
+    def autotune(self, sweeps=4):
+        best = C.c_int()
+        ms = np.empty(4)
+        check(lib().mcgb_autotune(self.h, sweeps, C.byref(best), _d(ms)))
+        return best.value, dict(zip((1, 2, 4, 8), ms.tolist()))
 
     def kernel_times(self):
         a, b, c = C.c_double(), C.c_double(), C.c_double()

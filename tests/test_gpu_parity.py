@@ -262,3 +262,57 @@ def test_stencil_beyond_lds_capacity_uses_global_path(precision):
             tr, info = ctx.sweep_traced()
             _, _, st = replay_sweep(c, q0, u0, L, s.seed, 0, 0, tr, info, check_tol=1e-5, fp32=True)
             assert st["checked"] > 500
+
+
+def test_waves_per_cell_do_not_change_fp64_results():
+    # the wavefront split only reorders FP64 sums: identical decisions and states
+    states = []
+    for w in (1, 2, 4, 8):
+        q, u, L = layered_system(1500, 0.25, colloid_radius=3.0, seed=41)
+        p = m.gb_params(1500, 0.25, 1.0, colloid_radius=3.0, precision=64, seed=6060,
+                        energy_check_interval=0, waves_per_cell=w)
+        with m.Context(p) as ctx:
+            ctx.set_state_fixed(q, u, L)
+            ctx.sweep(5)
+            states.append(ctx.get_state_fixed())
+    for qs, us, _ in states[1:]:
+        for d in range(3):
+            assert np.array_equal(qs[d], states[0][0][d])
+            np.testing.assert_allclose(us[d], states[0][1][d], rtol=0, atol=1e-13)
+
+
+def test_autotune_keeps_results_and_picks_a_supported_width():
+    ctx, s, c = make_pair(N=1300, precision=64, seed=17)
+    with ctx:
+        best, ms = ctx.autotune(2)
+        assert best in (1, 2, 4, 8) and all(v > 0 for v in ms.values())
+        for _ in range(8):
+            s.sweep()
+        q, u, _ = ctx.get_state_fixed()
+        for d in range(3):
+            assert np.array_equal(q[d], s.q[d])
+
+
+def test_graph_replay_matches_single_sweeps():
+    # mcgb_sweep(6) replays a captured two-sweep HIP graph; six mcgb_sweep(1) calls launch
+    # kernels one by one: identical FP64 trajectories
+    q, u, L = layered_system(1500, 0.25, colloid_radius=3.0, seed=44)
+    out = []
+    for mode in ("graph", "single"):
+        p = m.gb_params(1500, 0.25, 1.0, colloid_radius=3.0, precision=64, seed=123,
+                        energy_check_interval=4)
+        with m.Context(p) as ctx:
+            ctx.set_state_fixed(q, u, L)
+            if mode == "graph":
+                ctx.sweep(6)
+            else:
+                for _ in range(6):
+                    ctx.sweep(1)
+            out.append((ctx.get_state_fixed(), ctx.stats()))
+    (qa, ua, _), sa = out[0]
+    (qb, ub, _), sb = out[1]
+    for d in range(3):
+        assert np.array_equal(qa[d], qb[d])
+        assert np.array_equal(ua[d], ub[d])
+    assert sa["accepted"] == sb["accepted"] and sa["energy"] == sb["energy"]
+    assert sa["resummations"] == sb["resummations"] == 1

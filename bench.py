@@ -71,7 +71,7 @@ def main():
     ap.add_argument("--steps", type=int, default=40)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--cpu-sweeps", type=int, default=1)
+    ap.add_argument("--cpu-sweeps", type=int, default=10)
     ap.add_argument("--n", type=int, default=256000)
     ap.add_argument("--relax", type=int, default=200,
                     help="untimed sweeps from the lattice start before tuning and warm-up")

@@ -189,6 +189,14 @@ __device__ inline float gb_pair_f32(const GBf& C, float uix, float uiy, float ui
 
 // ------------------------------------------------------------------ GB pair, FP64
 
+// pow with the exponents 1 and 2 taken as x and x*x (the correctly rounded value, which pow
+// returns to within its 1-ulp bound); any other exponent goes through pow.
+__device__ inline double pow_int_fast(double x, double e) {
+    if (e == 1.0) return x;
+    if (e == 2.0) return x * x;
+    return pow(x, e);
+}
+
 __device__ inline double g_omega64(double c, double ap2, double am2, double w) {
     double oe = w * c;
     return 1.0 - 0.5 * w * (ap2 / (1.0 + oe) + am2 / (1.0 - oe));
@@ -209,8 +217,8 @@ __device__ inline double gb_pair_f64(const GBd& C, double uix, double uiy, doubl
     double ap = a + b, am = a - b;
     double ap2 = ap * ap, am2 = am * am;
     double eps1 = 1.0 / sqrt(1.0 - C.chi2 * c * c);
-    double ctep1 = pow(eps1, C.nu);
-    double ctep2 = pow(g_omega64(c, ap2, am2, C.chip), C.mu);
+    double ctep1 = pow_int_fast(eps1, C.nu);
+    double ctep2 = pow_int_fast(g_omega64(c, ap2, am2, C.chip), C.mu);
     double schi = g_omega64(c, ap2, am2, C.chi);
     double sigma = C.sigma0 / sqrt(schi);
     double dist = rij - sigma + C.sigma0;
@@ -244,7 +252,7 @@ __device__ __attribute__((noinline)) double colloid_f64(const RepConst& C, doubl
     double scr = s0 * sqrt(num / den);
     double den_e = 1.0 - C.chip * sin2;
     if (den_e <= 1e-12) den_e = 1e-12;
-    double eps = C.kappa_prime * pow(C.one_minus_chip / den_e, C.mu);
+    double eps = C.kappa_prime * pow_int_fast(C.one_minus_chip / den_e, C.mu);
     double x = dij - scr + s0;
     if (x <= 1e-10) { *cap = true; return 0.0; }
     double s3 = s0 * s0 * s0, s9 = s3 * s3 * s3;

@@ -47,7 +47,7 @@ void mcgb_params_default(mcgb_params *p)
     p->energy_check_interval = 100;
     p->lattice_jitter = 0.0;
     p->colloid_exclusion_radius = -1.0;      /* < 0: colloid_radius + 0.5 */
-    p->waves_per_cell = 4;
+    p->waves_per_cell = 2;
 }
 
 enum { T_INT, T_DOUBLE, T_FLOAT, T_U64 };

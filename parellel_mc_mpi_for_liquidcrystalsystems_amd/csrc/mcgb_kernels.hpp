@@ -319,10 +319,16 @@ __global__ __launch_bounds__(256) void k_end_moves(const RepConst* __restrict__ 
 
 // ------------------------------------------------------------------ FP64 full energy
 
-// Per-particle FP64 energy over a cell list (each unordered pair once: the partner must come
-// later in the list), colloid term and colloid-core overlap flag.  mode: 0 always; 1 only when
-// the replica flagged a re-summation; 2 the NPT trial box (indirect list, zero shift).
-// tot_ene, MCnptColloid.c:1598-1622.
+// FP64 energy over a cell list (each unordered pair once: the partner must come later in the
+// list), colloid term and colloid-core overlap count; one output record per wavefront of 64
+// consecutive list positions.  mode: 0 always; 1 only when the replica flagged a
+// re-summation; 2 the NPT trial box (indirect list, zero shift).  tot_ene,
+// MCnptColloid.c:1598-1622.
+//
+// Each lane owns one particle and walks its 27 neighbour cells; in-cutoff partners are
+// compacted (ballot) into a wavefront-private ring in LDS and evaluated 64 at a time with every
+// lane busy, so the FP64 pair body never runs under divergence.  Every lane's running sums and
+// the final shuffle tree are in a fixed order: the result is deterministic.
 template <typename Real>
 __global__ __launch_bounds__(128) void k_energy(
     const RepConst* __restrict__ RC, const RepDyn* __restrict__ RD, int mode,
@@ -330,54 +336,116 @@ __global__ __launch_bounds__(128) void k_energy(
     const Real* __restrict__ ux, const Real* __restrict__ uy, const Real* __restrict__ uz,
     const int* __restrict__ cell_start, const int* __restrict__ list,
     double* __restrict__ efin, int* __restrict__ ecap, int* __restrict__ eccap, int* __restrict__ ehard) {
+    constexpr int QN = 128;                       // ring capacity (>= 63 pending + 64 appended)
+    __shared__ int s_j[2][QN];
+    __shared__ unsigned char s_l[2][QN];
+    __shared__ uint32_t s_q[2][3][64];
+    __shared__ double s_u[2][3][64];
     const int r = blockIdx.y;
     const RepConst& C = RC[r];
     const RepDyn& D = RD[r];
     if (mode == 1 && !D.resum) return;
     if (mode == 2 && !D.trial_ok) return;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= C.N) return;
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int nw = (C.N + 63) >> 6;
+    const int wi = blockIdx.x * 2 + w;
+    if (wi >= nw) return;                         // whole wavefront (no workgroup barriers below)
+    const int i = wi * 64 + lane;
+    const bool act = i < C.N;
     const bool ind = mode == 2;
     const int n0 = ind ? D.nt[0] : D.n[0], n1 = ind ? D.nt[1] : D.n[1], n2 = ind ? D.nt[2] : D.n[2];
     const double L0 = ind ? D.Lt[0] : D.L[0], L1 = ind ? D.Lt[1] : D.L[1], L2 = ind ? D.Lt[2] : D.L[2];
     const double s0 = L0 / kTwo32, s1 = L1 / kTwo32, s2 = L2 / kTwo32;
     const uint32_t g0 = ind ? 0u : D.g[0], g1 = ind ? 0u : D.g[1], g2 = ind ? 0u : D.g[2];
     const int po = C.part_off;
-    const int gi = po + (ind ? list[po + i] : i);
-    const uint32_t xi = qx[gi], yi = qy[gi], zi = qz[gi];
-    const double uix = ux[gi], uiy = uy[gi], uiz = uz[gi];
-    const int cx = (int)cell_coord(xi, g0, n0), cy = (int)cell_coord(yi, g1, n1),
-              cz = (int)cell_coord(zi, g2, n2);
+    const double rc2 = C.rc2;
+    uint32_t xi = 0u, yi = 0u, zi = 0u;
+    double uix = 0.0, uiy = 0.0, uiz = 1.0;
+    int cx = 0, cy = 0, cz = 0;
+    if (act) {
+        const int gi = po + (ind ? list[po + i] : i);
+        xi = qx[gi]; yi = qy[gi]; zi = qz[gi];
+        uix = ux[gi]; uiy = uy[gi]; uiz = uz[gi];
+        cx = (int)cell_coord(xi, g0, n0); cy = (int)cell_coord(yi, g1, n1);
+        cz = (int)cell_coord(zi, g2, n2);
+    }
+    s_q[w][0][lane] = xi; s_q[w][1][lane] = yi; s_q[w][2][lane] = zi;
+    s_u[w][0][lane] = uix; s_u[w][1][lane] = uiy; s_u[w][2][lane] = uiz;
     const int* cs = cell_start + C.cs_off;
     const GBd gd = make_gbd(C);
     double f = 0.0;
     int cp = 0;
-    for (int dz = -1; dz <= 1; ++dz)
-        for (int dy = -1; dy <= 1; ++dy)
-            for (int dx = -1; dx <= 1; ++dx) {
-                int ex = cx + dx, ey = cy + dy, ez = cz + dz;
-                ex += ex < 0 ? n0 : 0; ex -= ex >= n0 ? n0 : 0;
-                ey += ey < 0 ? n1 : 0; ey -= ey >= n1 ? n1 : 0;
-                ez += ez < 0 ? n2 : 0; ez -= ez >= n2 ? n2 : 0;
-                const int nb = ex + n0 * (ey + n1 * ez);
-                const int b0 = cs[nb], b1 = cs[nb + 1];
-                for (int kk = b0; kk < b1; ++kk) {
-                    if (kk <= i) continue;
-                    const int gj = po + (ind ? list[po + kk] : kk);
-                    const double ddx = (double)(int32_t)(qx[gj] - xi) * s0;
-                    const double ddy = (double)(int32_t)(qy[gj] - yi) * s1;
-                    const double ddz = (double)(int32_t)(qz[gj] - zi) * s2;
-                    const double r2 = ddx * ddx + ddy * ddy + ddz * ddz;
-                    if (r2 < C.rc2) {
-                        bool c;
-                        const double u = gb_pair_f64(gd, uix, uiy, uiz, (double)ux[gj], (double)uy[gj],
-                                                     (double)uz[gj], ddx, ddy, ddz, r2, &c);
-                        if (c) ++cp; else f += u;
-                    }
-                }
+    int head = 0, tail = 0;                       // wavefront-uniform ring cursors
+
+    // evaluate the n oldest queued pairs (n <= 64), pair (owner lane l, partner gj) on lane k
+    auto flush = [&](int n) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        if (lane < n) {
+            const int slot = (head + lane) & (QN - 1);
+            const int gj = s_j[w][slot];
+            const int l = s_l[w][slot];
+            const double ddx = (double)(int32_t)(qx[gj] - s_q[w][0][l]) * s0;
+            const double ddy = (double)(int32_t)(qy[gj] - s_q[w][1][l]) * s1;
+            const double ddz = (double)(int32_t)(qz[gj] - s_q[w][2][l]) * s2;
+            const double r2 = ddx * ddx + ddy * ddy + ddz * ddz;
+            bool c;
+            const double u = gb_pair_f64(gd, s_u[w][0][l], s_u[w][1][l], s_u[w][2][l], (double)ux[gj],
+                                         (double)uy[gj], (double)uz[gj], ddx, ddy, ddz, r2, &c);
+            if (c) ++cp; else f += u;
+        }
+        head += n;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    };
+
+    for (int c = 0; c < 27; ++c) {
+        int st = 0, len = 0;
+        if (act) {
+            int ex = cx + (c % 3) - 1, ey = cy + ((c / 3) % 3) - 1, ez = cz + c / 9 - 1;
+            ex += ex < 0 ? n0 : 0; ex -= ex >= n0 ? n0 : 0;
+            ey += ey < 0 ? n1 : 0; ey -= ey >= n1 ? n1 : 0;
+            ez += ez < 0 ? n2 : 0; ez -= ez >= n2 ? n2 : 0;
+            const int nb = ex + n0 * (ey + n1 * ez);
+            const int b0 = cs[nb], b1 = cs[nb + 1];
+            st = b0 > i + 1 ? b0 : i + 1;
+            len = b1 - st;
+            len = len > 0 ? len : 0;
+        }
+        int mx = len;
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) {
+            const int o = __shfl_xor(mx, off);
+            mx = o > mx ? o : mx;
+        }
+        mx = bcast(mx);
+        for (int t = 0; t < mx; ++t) {
+            bool keep = false;
+            int gj = 0;
+            if (t < len) {
+                const int kk = st + t;
+                gj = po + (ind ? list[po + kk] : kk);
+                const double ddx = (double)(int32_t)(qx[gj] - xi) * s0;
+                const double ddy = (double)(int32_t)(qy[gj] - yi) * s1;
+                const double ddz = (double)(int32_t)(qz[gj] - zi) * s2;
+                keep = ddx * ddx + ddy * ddy + ddz * ddz < rc2;
             }
+            const unsigned long long m = __ballot(keep);
+            if (keep) {
+                const int slot = (tail + __popcll(m & __lanemask_lt())) & (QN - 1);
+                s_j[w][slot] = gj;
+                s_l[w][slot] = (unsigned char)lane;
+            }
+            tail += __popcll(m);
+            if (tail - head >= 64) flush(64);
+        }
+    }
+    while (tail > head) flush(tail - head < 64 ? tail - head : 64);
+
     int ccp = 0, hard = 0;
-    if (C.colloid_on) {
+    if (C.colloid_on && act) {
         const double rx = (double)(int32_t)(xi - 0x80000000u) * s0;
         const double ry = (double)(int32_t)(yi - 0x80000000u) * s1;
         const double rz = (double)(int32_t)(zi - 0x80000000u) * s2;
@@ -386,10 +454,16 @@ __global__ __launch_bounds__(128) void k_energy(
         if (c) ccp = 1; else f += u;
         hard = (rx * rx + ry * ry + rz * rz <= C.R2) ? 1 : 0;
     }
-    efin[po + i] = f;
-    ecap[po + i] = cp;
-    eccap[po + i] = ccp;
-    ehard[po + i] = hard;
+    f = wave_sum(f);
+    cp = wave_sum(cp);
+    ccp = wave_sum(ccp);
+    hard = wave_sum(hard);
+    if (lane == 0) {
+        efin[po + wi] = f;
+        ecap[po + wi] = cp;
+        eccap[po + wi] = ccp;
+        ehard[po + wi] = hard;
+    }
 }
 
 // Deterministic per-replica reduction of k_energy's output into D.e_res, then by mode:
@@ -412,7 +486,8 @@ __global__ __launch_bounds__(256) void k_energy_reduce(const RepConst* __restric
     const int tid = threadIdx.x;
     double f = 0.0;
     long long a = 0, b = 0, h = 0;
-    for (int i = tid; i < C.N; i += 256) {
+    const int nw = (C.N + 63) >> 6;                    // one record per k_energy wavefront
+    for (int i = tid; i < nw; i += 256) {
         f += efin[C.part_off + i];
         a += ecap[C.part_off + i];
         b += eccap[C.part_off + i];

@@ -27,6 +27,10 @@ METRIC = "trial moves/s and ESS/s of <E>,P2 at N=256k: plain vs surrogate-DA"
 PEAK_FP32_TFLOPS = 157.3
 # algorithmic flop counts: candidate test 17, GB pair 68, colloid-rod term 45
 F_CAND, F_PAIR, F_COLL = 17.0, 68.0, 45.0
+# HBM bytes per launch of the W=2 pass kernel on this workload: rocprofv3 FETCH_SIZE x 2.000 +
+# WRITE_SIZE x 1.000 (the factors a plain 256 MiB float32 copy reads), from the two counter
+# passes summarised in profiles/sweep_fp32_N256000_colloid_hbm_traffic.md
+TRAFFIC_PASS_BYTES = 27.99e6
 
 
 def workload(N=256000, rho=0.32, T=1.0, R=5.0, seed=20261203):
@@ -96,9 +100,11 @@ def main():
             ctx.sweep(args.relax)
         t_relax = time.perf_counter() - t_relax
         tune = None
+        tune_w = args.waves
         if not args.waves:
             best, ms = ctx.autotune(4)
             tune = {"waves_per_cell": best, "ms_per_pass": ms}
+            tune_w = best
         if args.warmup:
             ctx.sweep(args.warmup)
         ctx.reset_counters()
@@ -109,6 +115,9 @@ def main():
         st = ctx.stats()
         kt = ctx.kernel_times()
         e, p2, _ = ctx.series()
+        t_get = time.perf_counter()
+        ctx.get_state()                # the whole state device -> host (PCIe), outside the timed loop
+        t_get = time.perf_counter() - t_get
     moves = st["attempted"]
     value = moves / dt
     flops_sweep = (F_CAND * st["candidates"] + F_PAIR * st["pairs"] + F_COLL * st["colloid_evals"]) / args.steps
@@ -132,12 +141,14 @@ def main():
                    "cutoff": 4.0, "max_displacement": 0.1, "max_rotation": 0.1,
                    "lattice_jitter": 0.05, "seed": seed, "step": "one sweep (8 sublattice passes)"},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
+                     "frac": achieved / PEAK_FP32_TFLOPS,
+                     "traffic": TRAFFIC_PASS_BYTES if tune_w == 2 else None,
                      "kernel": "k_pass<float>", "ms_per_launch": kt["ms_pass"],
                      "flop_per_launch": flops_pass},
         "acceptance": st["accepted"] / max(1, moves),
         "ms_rebuild": kt["ms_rebuild"],
         "ms_sweep_device": kt["ms_sweep"],
+        "ms_get_state": 1e3 * t_get,
         "ess_per_s": {"E/N": ess_per_s(e, dt), "P2": ess_per_s(p2, dt), "samples": int(len(e))},
         "max_fp64_drift": st["max_drift"],
         "resummations_timed": st["resummations"],

@@ -1,14 +1,22 @@
-"""Benchmark of the plain FP32 checkerboard sweep on one MI355X.
+"""Benchmark of the checkerboard sweep on one MI355X: plain FP32 vs learned delayed acceptance.
 
-Workload: N = 256000 Gay-Berne ellipsoids GB(3, 5, 2, 1) at rho* = 0.32, T* = 1.0, one colloid
-of radius 5 sigma0 fixed at the box centre (GBCollRod form), seeded jittered 64^3 lattice
-(jitter 0.05, sites within 5.5 of the colloid removed), moves 0.1 sigma0 / 0.1 rad, FP32 pair
-math with FP64 totals.  A step is one sweep (8 sublattice passes, ~N trial moves).
+Plain workload (the `value` of the line): N = 256000 Gay-Berne ellipsoids GB(3, 5, 2, 1) at
+rho* = 0.32, T* = 1.0, one colloid of radius 5 sigma0 fixed at the box centre (GBCollRod form),
+seeded jittered 64^3 lattice (jitter 0.05, sites within 5.5 of the colloid removed), moves
+0.1 sigma0 / 0.1 rad, FP32 pair math with FP64 totals.  A step is one sweep (8 sublattice
+passes, ~N trial moves).
 
-Prints one JSON line: trial moves/s over the timed sweeps, the roofline of the dominant kernel
-(the sublattice-pass kernel, algorithmic FP32 flops per launch over its HIP-event-timed average
-duration, against the 157.3 TFLOP/s FP32 vector peak), ESS/s of E/N and P2 over the timed
-sweeps, and the FP64 CPU oracle timed on a bounded sample of the same workload.
+Delayed-acceptance workload (the `da` object): the same geometry at T* = 0.8.  The per-pair
+MLP 4-64-64-1 (SiLU, BF16 matrix-core inference) is trained on the GPU on ~10^7 exact pair
+energies emitted from GPU trajectories of both temperatures; then the plain sweep and the
+delayed-acceptance sweep run the same number of sweeps from the same state, and their
+trial moves/s and ESS/s of E/N and P2 are reported side by side.
+
+Prints one JSON line: trial moves/s of the plain sweep over the timed sweeps, the roofline of
+its dominant kernel (the sublattice-pass kernel: algorithmic FP32 flops per launch over its
+HIP-event-timed average duration, against the 157.3 TFLOP/s FP32 vector peak), ESS/s of E/N
+and P2, the delayed-acceptance comparison, and the FP64 CPU oracle timed on a bounded sample of
+the plain workload.
 """
 import argparse
 import json
@@ -69,17 +77,101 @@ def ess_per_s(x, seconds):
     return float(len(x) / tau / seconds)
 
 
+def timed(ctx, sweeps):
+    ctx.reset_counters()
+    ctx.series_clear()
+    t0 = time.perf_counter()
+    ctx.sweep(sweeps)                  # returns after hipStreamSynchronize
+    dt = time.perf_counter() - t0
+    st = ctx.stats()
+    e, p2, _ = ctx.series()
+    return dt, st, e, p2
+
+
+def da_leg(args, pairs_plain, waves):
+    """Learned delayed acceptance against the plain sweep on the T* = 0.8 state."""
+    N, rho, T, R = args.n, 0.32, 0.8, 5.0
+    p, pos, u, box = workload(N, rho, T, R)
+    p.waves_per_cell = waves
+    out = {"workload": "delayed acceptance vs plain, same geometry at T*=0.8",
+           "T": T, "sweeps": args.da_sweeps}
+    with m.Context(p) as ctx:
+        ctx.set_state(pos, u, box)
+        ctx.sweep(args.relax)
+        # training pairs: this trajectory plus the plain workload's (T* = 1.0)
+        t0 = time.perf_counter()
+        ctx.pairs_reserve(int(1.2 * args.pairs) + 4 * 10 ** 6)
+        if pairs_plain is not None:
+            ctx.pairs_put(pairs_plain)
+        k = 0
+        while ctx.pairs_count() < args.pairs and k < 16:
+            ctx.sweep(10)
+            ctx.emit_pairs(stride=8, offset=k % 8)
+            k += 1
+        out["train_pairs"] = ctx.pairs_count()
+        out["emit_s"] = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        ctx.surrogate_init(1)
+        losses = ctx.surrogate_train(args.train_steps, batch=4096, lr=3e-3, clip=10.0, seed=5,
+                                     wfloor=1.0)
+        out["train_s"] = time.perf_counter() - t0
+        out["train_steps"] = args.train_steps
+        out["loss_first_last"] = [float(losses[:10].mean()), float(losses[-200:].mean())]
+        q0, u0, L0 = ctx.get_state_fixed()
+        res = {}
+        for mode in ("plain", "da"):
+            ctx.set_state_fixed(q0, u0, L0)
+            ctx.da_enable(mode == "da")
+            ctx.sweep(2)                                   # graph capture outside the timing
+            dt, st, e, p2 = timed(ctx, args.da_sweeps)
+            evaluated = st["attempted"] - st["rejected_left_cell"] - st["rejected_hard_core"]
+            ee, ep = ess_per_s(e, dt), ess_per_s(p2, dt)
+            r = {"moves_per_s": st["attempted"] / dt, "ms_per_sweep": 1e3 * dt / args.da_sweeps,
+                 "acceptance": st["accepted"] / max(1, evaluated),
+                 "ess_per_s": {"E/N": ee, "P2": ep},
+                 "mean": {"E/N": float(np.mean(e)), "P2": float(np.mean(p2))}}
+            if mode == "da":
+                s1, s2 = st["da_stage1_rejected"], st["da_stage2_rejected"]
+                r["stage1_rejection"] = s1 / max(1, evaluated)
+                r["stage2_rejection"] = s2 / max(1, evaluated - s1)
+                r["exact_evaluations_skipped"] = s1 / max(1, evaluated)
+            res[mode] = r
+        # the plain kernel as judge: exact stage for every move, surrogate error histogram
+        ctx.set_state_fixed(q0, u0, L0)
+        ctx.da_enable(True, judge=True)
+        ctx.da_histogram(clear=True)
+        ctx.sweep(4)
+        h = ctx.da_histogram()
+        ctx.da_enable(False)
+    tot = max(1, int(h.sum()))
+    res["da"]["error_hist_kT"] = {"within_0.5": float(h[28:36].sum() / tot),
+                                  "within_1": float(h[24:40].sum() / tot),
+                                  "below_-4": float(h[64] / tot), "above_4": float(h[65] / tot)}
+    out.update(res)
+    mins = []
+    for mode in ("plain", "da"):
+        v = [x for x in res[mode]["ess_per_s"].values() if x is not None]
+        mins.append(min(v) if v else None)
+    out["ess_ratio_da_over_plain"] = (mins[1] / mins[0]) if (mins[0] and mins[1]) else None
+    out["moves_ratio_da_over_plain"] = res["da"]["moves_per_s"] / res["plain"]["moves_per_s"]
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=40)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-da", action="store_true", help="skip the delayed-acceptance leg")
     ap.add_argument("--cpu-sweeps", type=int, default=10)
     ap.add_argument("--n", type=int, default=256000)
     ap.add_argument("--relax", type=int, default=200,
                     help="untimed sweeps from the lattice start before tuning and warm-up")
     ap.add_argument("--waves", type=int, default=0, help="wavefronts per cell (0: autotune)")
+    ap.add_argument("--da-sweeps", type=int, default=100)
+    ap.add_argument("--pairs", type=int, default=10 ** 7, help="training pairs for the surrogate")
+    ap.add_argument("--train-steps", type=int, default=20000)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the sweep is not "
@@ -93,6 +185,7 @@ def main():
     seed = int(p.seed)
     if args.waves:
         p.waves_per_cell = args.waves
+    pairs_plain = None
     with m.Context(p) as ctx:
         ctx.set_state(pos, u, box)
         t_relax = time.perf_counter()
@@ -107,17 +200,19 @@ def main():
             tune_w = best
         if args.warmup:
             ctx.sweep(args.warmup)
-        ctx.reset_counters()
-        ctx.series_clear()
-        t0 = time.perf_counter()
-        ctx.sweep(args.steps)          # returns after hipStreamSynchronize
-        dt = time.perf_counter() - t0
-        st = ctx.stats()
+        dt, st, e, p2 = timed(ctx, args.steps)
         kt = ctx.kernel_times()
-        e, p2, _ = ctx.series()
         t_get = time.perf_counter()
         ctx.get_state()                # the whole state device -> host (PCIe), outside the timed loop
         t_get = time.perf_counter() - t_get
+        if not args.no_da:
+            ctx.pairs_reserve(args.pairs // 2 + 2 * 10 ** 6)
+            k = 0
+            while ctx.pairs_count() < args.pairs // 2 and k < 8:
+                ctx.sweep(10)
+                ctx.emit_pairs(stride=8, offset=k % 8)
+                k += 1
+            pairs_plain = ctx.pairs_get()
     moves = st["attempted"]
     value = moves / dt
     flops_sweep = (F_CAND * st["candidates"] + F_PAIR * st["pairs"] + F_COLL * st["colloid_evals"]) / args.steps
@@ -156,6 +251,8 @@ def main():
         "relax_s": t_relax,
         "autotune": tune,
     }
+    if not args.no_da:
+        out["da"] = da_leg(args, pairs_plain, tune_w or 2)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(pos, u, box, N, rho, T, R, seed, sweeps=args.cpu_sweeps)
     print(json.dumps(out))

@@ -116,6 +116,8 @@ typedef struct mcgb_stats {
     double   density;              /* N / (V - V_colloid), MCnptColloid.c:2799-2804          */
     double   max_drift;            /* largest |running - recomputed| seen at a re-summation  */
     uint64_t resummations;         /* FP64 re-summations of the running energy              */
+    uint64_t da_stage1_rejected;   /* delayed acceptance: moves rejected by the surrogate    */
+    uint64_t da_stage2_rejected;   /* delayed acceptance: rejected by the exact second stage */
 } mcgb_stats;
 
 typedef struct mcgb_ctx mcgb_ctx;
@@ -184,7 +186,8 @@ int mcgb_get_state_fixed(mcgb_ctx *ctx, int replica, int n,
  * (:5048-5264) / clean_neighbor_cells() (:5035-5046) by the device cell list. */
 int mcgb_sweep(mcgb_ctx *ctx, int n_sweeps);
 
-/* Runs ONE sweep on replica 0 only (other replicas untouched) and records every trial move.
+/* Runs ONE plain Metropolis sweep of a single-replica context and records every trial move
+ * (MCGB_ERR_ARG with more than one replica or with delayed acceptance enabled).
  * Record index = pass*(n_cells/8)*k + active_cell*k + move; capacity must be >= n_cells*k.
  * info receives {shift_x, shift_y, shift_z, order[0..7], k, n_cells}. */
 int mcgb_sweep_traced(mcgb_ctx *ctx, mcgb_move_trace *trace, int capacity, uint32_t info[13]);
@@ -275,6 +278,51 @@ int mcgb_write_fort1(const char *path, int n, const double *x, const double *y, 
                      const double box[3], double colloid_radius);
 int mcgb_read_fort1(const char *path, int n, double *x, double *y, double *z,
                     double *ux, double *uy, double *uz, double box[3], double *colloid_radius);
+
+/* ---------------------------------------------------------------- learned first stage
+ *
+ * No counterpart in the reference (its MC has no surrogate).  A per-pair MLP 4 -> 64 -> 64 -> 1
+ * (SiLU) on the features (r, u_i.r^, u_j.r^, u_i.u_j) of GB pairs inside the cutoff, fitted to
+ * the exact pair energy clipped at `clip`, trained on the GPU by the library's own forward,
+ * backward and Adam kernels, and evaluated inside the sweep with BF16 matrix-core tiles as the
+ * first stage of delayed acceptance (Christen & Fox 2005); the second stage is the exact GB
+ * energy, so the chain keeps the exact Boltzmann distribution. */
+#define MCGB_SUR_NPARAM 4545   /* W1[64][4] b1[64] W2[64][64] b2[64] W3[64] b3, row-major [out][in] */
+#define MCGB_DA_BINS    66     /* histogram of (dE_surrogate - dE_exact)/kT: 64 bins over [-4, 4], under, over */
+
+/* FP32 master parameters (host copies in and out). */
+int mcgb_surrogate_set(mcgb_ctx *ctx, const float *params);
+int mcgb_surrogate_get(mcgb_ctx *ctx, float *params);
+/* Uniform(+-1/sqrt(fan_in)) initialisation from Philox(seed). */
+int mcgb_surrogate_init(mcgb_ctx *ctx, uint32_t seed);
+/* y[i] for n feature rows x[i] = (r, a, b, c): use_mfma = 1 the BF16 matrix-core path of the
+ * sweep, 0 the FP32 path. */
+int mcgb_surrogate_eval(mcgb_ctx *ctx, int n, const float *x, int use_mfma, float *y);
+/* Gradient of the mean loss (1/n) sum (y - min(t, clip))^2 with respect to every parameter,
+ * by the training kernels; params == NULL uses the context's parameters. */
+int mcgb_surrogate_grad(mcgb_ctx *ctx, const float *params, int n, const float *x, const float *t,
+                        float clip, float *grad, double *loss);
+/* steps Adam steps (beta 0.9/0.999, eps 1e-8; learning rate cosine-decayed from lr to lr/20)
+ * on batches of `batch` pairs drawn with replacement (Philox(seed)) from the emitted pairs, each
+ * mapped by a random symmetry of the pair energy (head/tail flips, i <-> j).  wfloor > 0
+ * weights each squared error by (wfloor + 1)/(wfloor + |min(t, clip)|) (0: unweighted);
+ * losses[k] = mean (weighted) loss of batch k (or NULL). */
+int mcgb_surrogate_train(mcgb_ctx *ctx, int steps, int batch, float lr, float clip, float wfloor,
+                         uint32_t seed, float *losses);
+/* Device storage of exact pair records (r, a, b, c, U; U = 1e30 for a capped pair). */
+int mcgb_pairs_reserve(mcgb_ctx *ctx, long long capacity);
+int mcgb_pairs_clear(mcgb_ctx *ctx);
+int mcgb_pairs_count(mcgb_ctx *ctx, long long *n);
+int mcgb_pairs_get(mcgb_ctx *ctx, long long first, int n, float *out);
+/* Appends n host records (e.g. pairs emitted by another context). */
+int mcgb_pairs_put(mcgb_ctx *ctx, int n, const float *rec);
+/* Appends every in-cutoff ordered pair (i, j) of the list positions i = offset + m*stride of
+ * replica's current state, with its exact FP64 GB energy (pij_GB, MCnptColloid.c:2618-2681). */
+int mcgb_emit_pairs(mcgb_ctx *ctx, int replica, int stride, int offset, long long *added);
+/* Delayed acceptance on/off for the following sweeps; judge = 1 also evaluates the exact
+ * stage for moves the surrogate rejected (decisions unchanged; fills the histogram). */
+int mcgb_da_enable(mcgb_ctx *ctx, int on, int judge);
+int mcgb_da_histogram(mcgb_ctx *ctx, int replica, uint64_t counts[MCGB_DA_BINS], int clear);
 
 #ifdef __cplusplus
 }

@@ -86,6 +86,18 @@ struct mcgb_ctx {
     Events events;
     hipEvent_t tev[22] = {};
     hipGraphExec_t graph[2] = {nullptr, nullptr};
+    // learned first stage + delayed acceptance
+    float* sur_P = nullptr;            // FP32 master parameters (kNParam)
+    MlpPacked* sur_M = nullptr;        // packed BF16 matrix-core operands
+    bool sur_set = false;
+    int da_on = 0, da_judge = 0;
+    unsigned long long* da_hist = nullptr;   // [nrep][kDaBins + 2]
+    float* pairs = nullptr;            // emitted exact pair records (kPairRec floats each)
+    long long pairs_cap = 0, pairs_n = 0;
+    float *tr_grad = nullptr, *tr_m = nullptr, *tr_v = nullptr, *tr_partial = nullptr;
+    float* tr_scratch = nullptr;
+    double *tr_lpart = nullptr, *tr_loss = nullptr;
+    int tr_nblk = 0, adam_t = 0;
     bool use_graph = true;
     bool timing = true;
     double ms_pass = 0, ms_rebuild = 0, ms_sweep = 0;
@@ -244,11 +256,20 @@ int enqueue_observe(mcgb_ctx* c, int advance, int record) {
 
 template <typename Real, bool TR>
 void launch_pass_t(mcgb_ctx* c, const PassArgs& P) {
+    if (!TR && c->da_on) {
+        switch (c->waves) {
+        case 1: k_pass<Real, 1, false, false, true><<<c->nblk, 64, c->lds, c->st>>>(P); break;
+        case 2: k_pass<Real, 2, false, false, true><<<c->nblk, 128, c->lds, c->st>>>(P); break;
+        case 8: k_pass<Real, 8, false, false, true><<<c->nblk, 512, c->lds, c->st>>>(P); break;
+        default: k_pass<Real, 4, false, false, true><<<c->nblk, 256, c->lds, c->st>>>(P); break;
+        }
+    } else {
     switch (c->waves) {
     case 1: k_pass<Real, 1, false, TR><<<c->nblk, 64, c->lds, c->st>>>(P); break;
     case 2: k_pass<Real, 2, false, TR><<<c->nblk, 128, c->lds, c->st>>>(P); break;
     case 8: k_pass<Real, 8, false, TR><<<c->nblk, 512, c->lds, c->st>>>(P); break;
     default: k_pass<Real, 4, false, TR><<<c->nblk, 256, c->lds, c->st>>>(P); break;
+    }
     }
     // cells whose stencil exceeded the staging capacity: one single-wavefront workgroup per
     // overflow-list entry reads the stencil from HBM (with an empty list every workgroup exits
@@ -299,6 +320,9 @@ int enqueue_one(mcgb_ctx* c, mcgb_move_trace* trace, const hipEvent_t* ev) {
     P.ovf_list = c->ovf_list;
     P.nblk = c->nblk;
     P.cap = c->cap;
+    P.mlp = c->sur_M;
+    P.da_hist = c->da_hist;
+    P.da_judge = c->da_judge;
     for (int p = 0; p < 8; ++p) {
         P.pass = p;
         launch_pass<Real>(c, P);
@@ -829,6 +853,10 @@ int mcgb_sweep_traced(mcgb_ctx* c, mcgb_move_trace* trace, int capacity, uint32_
         mcgb_set_error("mcgb_sweep_traced: the context must hold exactly one replica");
         return MCGB_ERR_ARG;
     }
+    if (c->da_on) {
+        mcgb_set_error("mcgb_sweep_traced: traces record plain Metropolis moves; disable delayed acceptance first");
+        return MCGB_ERR_ARG;
+    }
     const RepConst& C = c->hC[0];
     const int ncap = C.ncap[0] * C.ncap[1] * C.ncap[2];
     const size_t need = (size_t)C.N + (size_t)ncap;     // n_cells * ceil(N / n_cells) <= N + n_cells
@@ -886,6 +914,8 @@ int mcgb_get_stats(mcgb_ctx* c, int r, mcgb_stats* out) {
     out->density = c->hC[r].N / (out->volume - c->hC[r].vcoll);
     out->max_drift = d.max_drift;
     out->resummations = d.resums;
+    out->da_stage1_rejected = d.da_s1;
+    out->da_stage2_rejected = d.da_s2;
     return MCGB_OK;
 }
 
@@ -899,6 +929,7 @@ int mcgb_reset_counters(mcgb_ctx* c, int r) {
     d.max_drift = 0.0;
     d.overflow = 0;
     d.resums = 0;
+    d.da_s1 = d.da_s2 = 0;
     return write_dyn(c, D);
 }
 
@@ -1139,6 +1170,392 @@ int mcgb_philox_batch(int n, const uint32_t* ctr, const uint32_t* key, uint32_t*
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpy(out, d_o, n * 16, hipMemcpyDeviceToHost));
     HIPCHK(hipFree(d_c)); HIPCHK(hipFree(d_k)); HIPCHK(hipFree(d_o));
+    return MCGB_OK;
+}
+
+
+// ---------------------------------------------------------------- learned first stage
+
+}  // extern "C"
+
+namespace {
+
+uint16_t f32_to_bf16(float f) {      // round to nearest even
+    uint32_t u;
+    memcpy(&u, &f, 4);
+    if ((u & 0x7FFFFFFFu) > 0x7F800000u) return (uint16_t)((u >> 16) | 0x40u);
+    return (uint16_t)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
+}
+
+// FP32 parameters -> the per-lane matrix-core operands of mlp_tile32.
+void pack_mlp(const float* P, MlpPacked* M) {
+    memset(M, 0, sizeof(*M));
+    for (int lane = 0; lane < 64; ++lane) {
+        const int o = lane & 31, g = lane >> 5;
+        for (int t = 0; t < 2; ++t) {
+            const int h = 32 * t + o;
+            for (int e = 0; e < 8; ++e) {   // group 0: (W1, W1) against (x_hi, x_lo); group 1: b1
+                const float v = g == 0 ? P[kOffW1 + 4 * h + (e & 3)] : (e == 0 ? P[kOffB1 + h] : 0.0f);
+                M->w1[t][e][lane] = f32_to_bf16(v);
+            }
+            for (int m = 0; m < 4; ++m)
+                for (int e = 0; e < 8; ++e) {
+                    const int hin = 32 * (m >> 1) + mfma_row(8 * (m & 1) + e, g);
+                    M->w2[t][m][e][lane] = f32_to_bf16(P[kOffW2 + kHid * h + hin]);
+                }
+            for (int r = 0; r < 16; ++r) {
+                const int row = 32 * t + mfma_row(r, g);
+                M->b2[t][r][lane] = P[kOffB2 + row];
+                M->w3[t][r][lane] = P[kOffW3 + row];
+            }
+        }
+    }
+    M->b3 = P[kOffB3];
+}
+
+int sur_alloc(mcgb_ctx* c) {
+    int rc = MCGB_OK;
+    if (!c->sur_P) {
+        if ((rc = dalloc_t(c, &c->sur_P, kNParam))) return rc;
+        if ((rc = dalloc_t(c, &c->sur_M, 1))) return rc;
+        if ((rc = dalloc_t(c, &c->tr_grad, kNParam))) return rc;
+        if ((rc = dalloc_t(c, &c->tr_m, kNParam))) return rc;
+        if ((rc = dalloc_t(c, &c->tr_v, kNParam))) return rc;
+        if ((rc = dalloc_t(c, &c->tr_loss, 1))) return rc;
+        if ((rc = dalloc_t(c, &c->da_hist, (size_t)c->nrep * (kDaBins + 2)))) return rc;
+        HIPCHK(hipMemset(c->da_hist, 0, sizeof(unsigned long long) * c->nrep * (kDaBins + 2)));
+        HIPCHK(hipMemset(c->sur_P, 0, sizeof(float) * kNParam));
+        MlpPacked z;
+        std::vector<float> zp(kNParam, 0.0f);
+        pack_mlp(zp.data(), &z);
+        HIPCHK(hipMemcpy(c->sur_M, &z, sizeof(z), hipMemcpyHostToDevice));
+    }
+    return rc;
+}
+
+int train_alloc(mcgb_ctx* c, int nblk) {
+    if (nblk <= c->tr_nblk) return MCGB_OK;
+    int rc;
+    if ((rc = dalloc_t(c, &c->tr_partial, (size_t)nblk * kNParam))) return rc;
+    if ((rc = dalloc_t(c, &c->tr_scratch, (size_t)nblk * 2 * 4096))) return rc;
+    if ((rc = dalloc_t(c, &c->tr_lpart, (size_t)nblk))) return rc;
+    c->tr_nblk = nblk;
+    return MCGB_OK;
+}
+
+void drop_graphs(mcgb_ctx* c) {
+    for (int k = 0; k < 2; ++k)
+        if (c->graph[k]) { (void)hipGraphExecDestroy(c->graph[k]); c->graph[k] = nullptr; }
+}
+
+// partial gradients of nblk workgroups -> c->tr_grad, loss sum -> c->tr_loss
+int enqueue_grad(mcgb_ctx* c, const float* P, const float* data, long long ndata, const float* xs,
+                 const float* ts, int nexp, uint32_t step, uint32_t seed, float clip, float wfloor,
+                 int nblk, float inv_n) {
+    k_mlp_grad<<<nblk, 64, 0, c->st>>>(P, data, ndata, xs, ts, nexp, step, seed, 0x6D6C7000u, clip,
+                                       wfloor, inv_n, c->tr_scratch, c->tr_partial, c->tr_lpart);
+    k_mlp_reduce<<<(kNParam + 255) / 256, 256, 0, c->st>>>(c->tr_partial, nblk, c->tr_lpart,
+                                                          c->tr_grad, c->tr_loss);
+    HIPCHK(hipGetLastError());
+    return MCGB_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mcgb_surrogate_set(mcgb_ctx* c, const float* params) {
+    int rc = check_ctx(c, 0);
+    if (rc) return rc;
+    if (!params) { mcgb_set_error("mcgb_surrogate_set: null parameters"); return MCGB_ERR_ARG; }
+    for (int k = 0; k < kNParam; ++k)
+        if (!std::isfinite(params[k])) {
+            mcgb_set_error("mcgb_surrogate_set: parameter %d is not finite", k);
+            return MCGB_ERR_ARG;
+        }
+    if ((rc = sur_alloc(c))) return rc;
+    MlpPacked M;
+    pack_mlp(params, &M);
+    HIPCHK(hipMemcpy(c->sur_P, params, sizeof(float) * kNParam, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->sur_M, &M, sizeof(M), hipMemcpyHostToDevice));
+    HIPCHK(hipMemset(c->tr_m, 0, sizeof(float) * kNParam));
+    HIPCHK(hipMemset(c->tr_v, 0, sizeof(float) * kNParam));
+    c->adam_t = 0;
+    c->sur_set = true;
+    return MCGB_OK;
+}
+
+int mcgb_surrogate_get(mcgb_ctx* c, float* params) {
+    int rc = check_ctx(c, 0);
+    if (rc) return rc;
+    if (!params) { mcgb_set_error("mcgb_surrogate_get: null output"); return MCGB_ERR_ARG; }
+    if ((rc = sur_alloc(c))) return rc;
+    HIPCHK(hipMemcpy(params, c->sur_P, sizeof(float) * kNParam, hipMemcpyDeviceToHost));
+    return MCGB_OK;
+}
+
+int mcgb_surrogate_init(mcgb_ctx* c, uint32_t seed) {
+    int rc = check_ctx(c, 0);
+    if (rc) return rc;
+    // U(-1/sqrt(fan_in), 1/sqrt(fan_in)) for every weight and bias of a layer (Philox draws)
+    std::vector<float> P(kNParam);
+    const uint32_t key[2] = {seed, 0x5EEDu};
+    for (int k = 0; k < kNParam; ++k) {
+        const uint32_t ctr[4] = {(uint32_t)k, 0u, 0u, 0x1A1A0000u};
+        uint32_t o[4];
+        mcgb_philox_host(ctr, key, o);
+        const double u = (o[0] + 0.5) / 4294967296.0 * 2.0 - 1.0;
+        const double fan = k < kOffW2 ? 4.0 : (k < kOffW3 ? 64.0 : 64.0);
+        P[k] = (float)(u / sqrt(fan));
+    }
+    return mcgb_surrogate_set(c, P.data());
+}
+
+int mcgb_surrogate_eval(mcgb_ctx* c, int n, const float* x, int use_mfma, float* y) {
+    int rc = check_ctx(c, 0);
+    if (rc) return rc;
+    if (n <= 0 || !x || !y) { mcgb_set_error("mcgb_surrogate_eval: bad arguments"); return MCGB_ERR_ARG; }
+    if ((rc = sur_alloc(c))) return rc;
+    float *dx, *dy;
+    HIPCHK(hipMalloc(&dx, sizeof(float) * 4 * (size_t)n));
+    HIPCHK(hipMalloc(&dy, sizeof(float) * (size_t)n));
+    HIPCHK(hipMemcpy(dx, x, sizeof(float) * 4 * (size_t)n, hipMemcpyHostToDevice));
+    const int per = use_mfma ? 32 : 64;
+    k_mlp_eval<<<(n + per - 1) / per, 64, 0, c->st>>>(c->sur_P, c->sur_M, n, dx, use_mfma ? 1 : 0, dy);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(c->st));
+    HIPCHK(hipMemcpy(y, dy, sizeof(float) * (size_t)n, hipMemcpyDeviceToHost));
+    HIPCHK(hipFree(dx));
+    HIPCHK(hipFree(dy));
+    return MCGB_OK;
+}
+
+int mcgb_surrogate_grad(mcgb_ctx* c, const float* params, int n, const float* x, const float* t,
+                        float clip, float* grad, double* loss) {
+    int rc = check_ctx(c, 0);
+    if (rc) return rc;
+    if (n <= 0 || !x || !t || !grad) { mcgb_set_error("mcgb_surrogate_grad: bad arguments"); return MCGB_ERR_ARG; }
+    if ((rc = sur_alloc(c))) return rc;
+    const int nblk = (n + 63) / 64;
+    if ((rc = train_alloc(c, nblk))) return rc;
+    float *dP = c->sur_P, *dx, *dt;
+    if (params) {
+        HIPCHK(hipMalloc(&dP, sizeof(float) * kNParam));
+        HIPCHK(hipMemcpy(dP, params, sizeof(float) * kNParam, hipMemcpyHostToDevice));
+    }
+    HIPCHK(hipMalloc(&dx, sizeof(float) * 4 * (size_t)n));
+    HIPCHK(hipMalloc(&dt, sizeof(float) * (size_t)n));
+    HIPCHK(hipMemcpy(dx, x, sizeof(float) * 4 * (size_t)n, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(dt, t, sizeof(float) * (size_t)n, hipMemcpyHostToDevice));
+    if ((rc = enqueue_grad(c, dP, nullptr, 0, dx, dt, n, 0, 0, clip, 0.0f, nblk, 1.0f / (float)n))) return rc;
+    HIPCHK(hipStreamSynchronize(c->st));
+    HIPCHK(hipMemcpy(grad, c->tr_grad, sizeof(float) * kNParam, hipMemcpyDeviceToHost));
+    double l;
+    HIPCHK(hipMemcpy(&l, c->tr_loss, sizeof(double), hipMemcpyDeviceToHost));
+    if (loss) *loss = l / n;
+    if (params) HIPCHK(hipFree(dP));
+    HIPCHK(hipFree(dx));
+    HIPCHK(hipFree(dt));
+    return MCGB_OK;
+}
+
+int mcgb_surrogate_train(mcgb_ctx* c, int steps, int batch, float lr, float clip, float wfloor,
+                         uint32_t seed, float* losses) {
+    int rc = check_ctx(c, 0);
+    if (rc) return rc;
+    if (steps <= 0 || batch < 64 || !(lr > 0.0f) || !(clip > 0.0f)) {
+        mcgb_set_error("mcgb_surrogate_train: need steps > 0, batch >= 64, lr > 0, clip > 0");
+        return MCGB_ERR_ARG;
+    }
+    if (c->pairs_n <= 0) {
+        mcgb_set_error("mcgb_surrogate_train: no training pairs (mcgb_emit_pairs)");
+        return MCGB_ERR_STATE;
+    }
+    if ((rc = sur_alloc(c))) return rc;
+    const int nblk = (batch + 63) / 64;
+    if ((rc = train_alloc(c, nblk))) return rc;
+    double* dl = nullptr;
+    if (losses) HIPCHK(hipMalloc(&dl, sizeof(double) * (size_t)steps));
+    const float inv = 1.0f / (float)(nblk * 64);
+    for (int k = 0; k < steps; ++k) {
+        if ((rc = enqueue_grad(c, c->sur_P, c->pairs, c->pairs_n, nullptr, nullptr, 0, (uint32_t)k,
+                               seed, clip, wfloor, nblk, inv)))
+            return rc;
+        ++c->adam_t;
+        // cosine decay from lr to lr/20 over the call
+        const float lr_k = (float)(lr / 20.0 + 0.5 * (lr - lr / 20.0) * (1.0 + cos(kPi * k / steps)));
+        k_mlp_adam<<<(kNParam + 255) / 256, 256, 0, c->st>>>(c->sur_P, c->tr_grad, c->tr_m, c->tr_v,
+                                                             c->adam_t, lr_k, 0.9f, 0.999f, 1e-8f);
+        if (dl) HIPCHK(hipMemcpyAsync(dl + k, c->tr_loss, sizeof(double), hipMemcpyDeviceToDevice, c->st));
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(c->st));
+    std::vector<float> P(kNParam);
+    HIPCHK(hipMemcpy(P.data(), c->sur_P, sizeof(float) * kNParam, hipMemcpyDeviceToHost));
+    MlpPacked M;
+    pack_mlp(P.data(), &M);
+    HIPCHK(hipMemcpy(c->sur_M, &M, sizeof(M), hipMemcpyHostToDevice));
+    c->sur_set = true;
+    if (dl) {
+        std::vector<double> h(steps);
+        HIPCHK(hipMemcpy(h.data(), dl, sizeof(double) * (size_t)steps, hipMemcpyDeviceToHost));
+        for (int k = 0; k < steps; ++k) losses[k] = (float)(h[k] / (nblk * 64));
+        HIPCHK(hipFree(dl));
+    }
+    return MCGB_OK;
+}
+
+int mcgb_pairs_reserve(mcgb_ctx* c, long long capacity) {
+    int rc = check_ctx(c, 0);
+    if (rc) return rc;
+    if (capacity <= 0) { mcgb_set_error("mcgb_pairs_reserve: capacity must be positive"); return MCGB_ERR_ARG; }
+    if (capacity <= c->pairs_cap) return MCGB_OK;
+    float* np = nullptr;
+    if ((rc = dalloc_t(c, &np, (size_t)capacity * kPairRec))) return rc;
+    if (c->pairs) {
+        HIPCHK(hipMemcpy(np, c->pairs, sizeof(float) * kPairRec * (size_t)c->pairs_n,
+                         hipMemcpyDeviceToDevice));
+        auto it = std::find(c->allocs.begin(), c->allocs.end(), (void*)c->pairs);
+        if (it != c->allocs.end()) c->allocs.erase(it);
+        HIPCHK(hipFree(c->pairs));
+    }
+    c->pairs = np;
+    c->pairs_cap = capacity;
+    return MCGB_OK;
+}
+
+int mcgb_pairs_clear(mcgb_ctx* c) {
+    int rc = check_ctx(c, 0);
+    if (rc) return rc;
+    c->pairs_n = 0;
+    return MCGB_OK;
+}
+
+int mcgb_pairs_count(mcgb_ctx* c, long long* n) {
+    int rc = check_ctx(c, 0);
+    if (rc) return rc;
+    if (!n) { mcgb_set_error("mcgb_pairs_count: null output"); return MCGB_ERR_ARG; }
+    *n = c->pairs_n;
+    return MCGB_OK;
+}
+
+int mcgb_pairs_get(mcgb_ctx* c, long long first, int n, float* out) {
+    int rc = check_ctx(c, 0);
+    if (rc) return rc;
+    if (!out || n < 0 || first < 0 || first + n > c->pairs_n) {
+        mcgb_set_error("mcgb_pairs_get: range [%lld, %lld) outside the %lld stored pairs", first,
+                       first + n, c->pairs_n);
+        return MCGB_ERR_ARG;
+    }
+    if (n == 0) return MCGB_OK;
+    HIPCHK(hipMemcpy(out, c->pairs + (size_t)first * kPairRec, sizeof(float) * kPairRec * (size_t)n,
+                     hipMemcpyDeviceToHost));
+    return MCGB_OK;
+}
+
+int mcgb_pairs_put(mcgb_ctx* c, int n, const float* rec) {
+    int rc = check_ctx(c, 0);
+    if (rc) return rc;
+    if (n < 0 || (n > 0 && !rec)) { mcgb_set_error("mcgb_pairs_put: bad arguments"); return MCGB_ERR_ARG; }
+    if (c->pairs_n + n > c->pairs_cap) {
+        mcgb_set_error("mcgb_pairs_put: %d more pairs exceed the reserved %lld (%lld stored)", n,
+                       c->pairs_cap, c->pairs_n);
+        return MCGB_ERR_CAPACITY;
+    }
+    if (n == 0) return MCGB_OK;
+    HIPCHK(hipMemcpy(c->pairs + (size_t)c->pairs_n * kPairRec, rec, sizeof(float) * kPairRec * (size_t)n,
+                     hipMemcpyHostToDevice));
+    c->pairs_n += n;
+    return MCGB_OK;
+}
+
+int mcgb_emit_pairs(mcgb_ctx* c, int r, int stride, int offset, long long* added) {
+    int rc = check_ctx(c, r);
+    if (rc) return rc;
+    if (stride <= 0 || offset < 0) { mcgb_set_error("mcgb_emit_pairs: need stride > 0, offset >= 0"); return MCGB_ERR_ARG; }
+    if (!c->pairs) { mcgb_set_error("mcgb_emit_pairs: reserve storage first (mcgb_pairs_reserve)"); return MCGB_ERR_STATE; }
+    const int N = c->hC[r].N;
+    const int M = offset < N ? (N - offset + stride - 1) / stride : 0;
+    if (M == 0) { if (added) *added = 0; return MCGB_OK; }
+    // a cell list of the current box (shift 0)
+    k_grid_g0<<<(c->nrep + 63) / 64, 64, 0, c->st>>>(c->dC, c->dD, c->nrep);
+    rc = c->prec == 64 ? enqueue_rebuild<double>(c) : enqueue_rebuild<float>(c);
+    if (rc) return rc;
+    int* dcnt;
+    long long* doff;
+    HIPCHK(hipMalloc(&dcnt, sizeof(int) * (size_t)M));
+    HIPCHK(hipMalloc(&doff, sizeof(long long) * (size_t)M));
+    const int b = c->cur;
+    const int g = (M + 127) / 128;
+    if (c->prec == 64)
+        k_emit<double><<<g, 128, 0, c->st>>>(c->dC, c->dD, r, 0, c->q[b][0], c->q[b][1], c->q[b][2],
+                                             (const double*)c->u[b][0], (const double*)c->u[b][1],
+                                             (const double*)c->u[b][2], c->cell_start, stride, offset,
+                                             M, dcnt, doff, nullptr);
+    else
+        k_emit<float><<<g, 128, 0, c->st>>>(c->dC, c->dD, r, 0, c->q[b][0], c->q[b][1], c->q[b][2],
+                                            (const float*)c->u[b][0], (const float*)c->u[b][1],
+                                            (const float*)c->u[b][2], c->cell_start, stride, offset,
+                                            M, dcnt, doff, nullptr);
+    HIPCHK(hipGetLastError());
+    std::vector<int> cnt(M);
+    HIPCHK(hipMemcpyAsync(cnt.data(), dcnt, sizeof(int) * (size_t)M, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    std::vector<long long> off(M);
+    long long tot = 0;
+    for (int m = 0; m < M; ++m) { off[m] = c->pairs_n + tot; tot += cnt[m]; }
+    if (c->pairs_n + tot > c->pairs_cap) {
+        HIPCHK(hipFree(dcnt));
+        HIPCHK(hipFree(doff));
+        mcgb_set_error("mcgb_emit_pairs: %lld more pairs exceed the reserved %lld (%lld stored)", tot,
+                       c->pairs_cap, c->pairs_n);
+        return MCGB_ERR_CAPACITY;
+    }
+    HIPCHK(hipMemcpy(doff, off.data(), sizeof(long long) * (size_t)M, hipMemcpyHostToDevice));
+    if (c->prec == 64)
+        k_emit<double><<<g, 128, 0, c->st>>>(c->dC, c->dD, r, 1, c->q[b][0], c->q[b][1], c->q[b][2],
+                                             (const double*)c->u[b][0], (const double*)c->u[b][1],
+                                             (const double*)c->u[b][2], c->cell_start, stride, offset,
+                                             M, dcnt, doff, c->pairs);
+    else
+        k_emit<float><<<g, 128, 0, c->st>>>(c->dC, c->dD, r, 1, c->q[b][0], c->q[b][1], c->q[b][2],
+                                            (const float*)c->u[b][0], (const float*)c->u[b][1],
+                                            (const float*)c->u[b][2], c->cell_start, stride, offset,
+                                            M, dcnt, doff, c->pairs);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(c->st));
+    HIPCHK(hipFree(dcnt));
+    HIPCHK(hipFree(doff));
+    c->pairs_n += tot;
+    if (added) *added = tot;
+    return MCGB_OK;
+}
+
+int mcgb_da_enable(mcgb_ctx* c, int on, int judge) {
+    int rc = check_ctx(c, 0);
+    if (rc) return rc;
+    if (on && !c->sur_set) {
+        mcgb_set_error("mcgb_da_enable: no surrogate parameters (mcgb_surrogate_set/_init/_train)");
+        return MCGB_ERR_STATE;
+    }
+    if ((rc = sur_alloc(c))) return rc;
+    drop_graphs(c);
+    c->da_on = on ? 1 : 0;
+    c->da_judge = judge ? 1 : 0;
+    return MCGB_OK;
+}
+
+int mcgb_da_histogram(mcgb_ctx* c, int r, uint64_t* counts, int clear) {
+    int rc = check_ctx(c, r);
+    if (rc) return rc;
+    if ((rc = sur_alloc(c))) return rc;
+    HIPCHK(hipStreamSynchronize(c->st));
+    if (counts)
+        HIPCHK(hipMemcpy(counts, c->da_hist + (size_t)r * (kDaBins + 2),
+                         sizeof(unsigned long long) * (kDaBins + 2), hipMemcpyDeviceToHost));
+    if (clear)
+        HIPCHK(hipMemset(c->da_hist + (size_t)r * (kDaBins + 2), 0,
+                         sizeof(unsigned long long) * (kDaBins + 2)));
     return MCGB_OK;
 }
 

@@ -70,15 +70,18 @@ struct RepDyn {
     int pad1;
     double e_res[4];             // last full recompute {cap_pairs, cap_coll, fin, hard}
     unsigned long long resums;   // re-summations of the running energy
+    unsigned long long da_s1, da_s2;   // delayed-acceptance rejections, surrogate / exact stage
 };
 
 // Draws of one trial move (k_draws): particle selector, fixed-point step, rotation vector,
-// Metropolis threshold -log(U) T.
+// Metropolis threshold -log(U) T, and the threshold of the surrogate stage of delayed
+// acceptance -log(U') T (U' an independent uniform).
 struct MoveDraw {
     uint32_t j;
     int32_t dq[3];
     double thr;
     double v[3];
+    double thr1;
 };
 
 // FP32 constants of the pair function, held in registers by the pass kernel.
@@ -99,7 +102,7 @@ struct CellAcc {
     int dcap, dccap, pairs, coll;
     long long cand;
     double dfin, moved_abs;
-    double pad;
+    int s1rej, s2rej;          // delayed acceptance: rejections by the surrogate / exact stage
 };
 
 // ------------------------------------------------------------------ Philox4x32-10

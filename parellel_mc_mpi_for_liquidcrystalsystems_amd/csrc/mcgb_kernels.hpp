@@ -269,6 +269,7 @@ __global__ __launch_bounds__(256) void k_draws(const RepConst* __restrict__ RC,
     d.v[1] = sxy * sn;
     d.v[2] = zz;
     d.thr = -log(u01(B[2])) * C.temperature;
+    d.thr1 = -log(u01(B[3])) * C.temperature;
     rec[C.rec_off + idx] = d;
 }
 
@@ -279,28 +280,29 @@ __global__ __launch_bounds__(256) void k_draws(const RepConst* __restrict__ RC,
 __global__ __launch_bounds__(256) void k_end_moves(const RepConst* __restrict__ RC,
                                                    RepDyn* __restrict__ RD,
                                                    const CellAcc* __restrict__ acc) {
-    __shared__ long long si[256][9];
+    __shared__ long long si[256][11];
     __shared__ double sd[256][2];
     const int r = blockIdx.x;
     const RepConst& C = RC[r];
     RepDyn& D = RD[r];
     const int tid = threadIdx.x;
-    long long v[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    long long v[11] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     double f = 0.0, mv = 0.0;
     for (int c = tid; c < D.ncell; c += 256) {
         const CellAcc& A = acc[C.cell_off + c];
         v[0] += A.att; v[1] += A.acc; v[2] += A.left; v[3] += A.hard;
         v[4] += A.cand; v[5] += A.pairs; v[6] += A.coll; v[7] += A.dcap; v[8] += A.dccap;
+        v[9] += A.s1rej; v[10] += A.s2rej;
         f += A.dfin;
         mv += A.moved_abs;
     }
-    for (int q = 0; q < 9; ++q) si[tid][q] = v[q];
+    for (int q = 0; q < 11; ++q) si[tid][q] = v[q];
     sd[tid][0] = f;
     sd[tid][1] = mv;
     __syncthreads();
     for (int off = 128; off >= 1; off >>= 1) {
         if (tid < off) {
-            for (int q = 0; q < 9; ++q) si[tid][q] += si[tid + off][q];
+            for (int q = 0; q < 11; ++q) si[tid][q] += si[tid + off][q];
             sd[tid][0] += sd[tid + off][0];
             sd[tid][1] += sd[tid + off][1];
         }
@@ -311,6 +313,8 @@ __global__ __launch_bounds__(256) void k_end_moves(const RepConst* __restrict__ 
         D.cand += si[0][4]; D.pairs += si[0][5]; D.coll += si[0][6];
         D.cap_pairs += si[0][7];
         D.cap_coll += si[0][8];
+        D.da_s1 += si[0][9];
+        D.da_s2 += si[0][10];
         D.e_fin += sd[0][0];
         D.moved_abs = sd[0][1];
         if (sd[0][1] > 1.0e10) D.resum = 1;

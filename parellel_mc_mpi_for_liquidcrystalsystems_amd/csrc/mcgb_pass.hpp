@@ -13,6 +13,7 @@
 #pragma once
 
 #include "mcgb_device.hpp"
+#include "mcgb_surrogate.hpp"
 #include "mcgb.h"
 
 namespace mcgb {
@@ -40,6 +41,12 @@ __device__ inline double dpp_d(double v) {
 
 __device__ inline int wave_sum_i63(int v) {
 #define S_(c, m) v += dpp_i<c, m>(v);
+    MCGB_DPP_STEPS(S_)
+#undef S_
+    return v;
+}
+__device__ inline float wave_sum_f63(float v) {
+#define S_(c, m) v += dpp_f<c, m>(v);
     MCGB_DPP_STEPS(S_)
 #undef S_
     return v;
@@ -81,6 +88,10 @@ struct PassArgs {
     int nblk;
     int pass;
     int cap;          // LDS staging capacity (particles)
+    // delayed acceptance
+    const MlpPacked* mlp;             // packed BF16 operands of the surrogate
+    unsigned long long* da_hist;      // [replica][kDaBins + 2] surrogate error histogram
+    int da_judge;                     // 1: exact stage evaluated for every move (diagnostics)
 };
 
 struct WavePartial {
@@ -102,8 +113,9 @@ struct PassLds {
         b += (size_t)cap * (12 + 3 * sizeof(Real));       // staged SoA
         b += (size_t)W * list_cap(cap) * 2;               // wavefront lists (ushort)
         b = (b + 15) & ~(size_t)15;
-        b += kDraws * (8 + 3 * sizeof(Real) + 12 + 4);    // draws
+        b += kDraws * (8 + 8 + 3 * sizeof(Real) + 12 + 4);    // draws
         b += W * sizeof(WavePartial) + 16;
+        b += W * sizeof(float) + 16;                          // surrogate partials
         return b + 16;
     }
 };
@@ -208,8 +220,14 @@ __device__ inline int stencil_slot(const int* s_off, const int* s_start, int par
 // exceeds the staging capacity is appended to the pass's overflow list and left untouched.
 // GLOBAL = true: the workgroups walk that overflow list and read the stencil from HBM.
 // TR = true: every trial move is recorded in P.trace (with the sum of |pair terms|).
-template <typename Real, int W, bool GLOBAL, bool TR>
+// DA = true: delayed acceptance (Christen & Fox 2005).  Stage 1 accepts with
+// min(1, exp(-beta dE~)), dE~ = the surrogate's GB pair terms plus the exact colloid term; only
+// then is the exact GB energy evaluated, and stage 2 accepts with min(1, exp(-beta (dE - dE~))).
+// The surrogate orders each pair by stencil slot (lower slot first), a fixed function of the
+// configuration for the whole move, so the composite kernel keeps detailed balance exactly.
+template <typename Real, int W, bool GLOBAL, bool TR, bool DA = false>
 __global__ __launch_bounds__(64 * W) void k_pass(PassArgs P) {
+    static_assert(!(DA && (GLOBAL || TR)), "delayed acceptance runs in the staged, untraced pass");
     extern __shared__ __align__(16) unsigned char smem[];
     int b;
     if (GLOBAL) {
@@ -248,7 +266,8 @@ __global__ __launch_bounds__(64 * W) void k_pass(PassArgs P) {
     dp = (unsigned char*)(((uintptr_t)dp + 15) & ~(uintptr_t)15);
     constexpr int KD = L::kDraws;                              // moves per draw chunk
     double* sthr = (double*)dp;                                // KD
-    Real* svx = (Real*)(sthr + KD);                            // 3 x KD
+    double* sthr1 = sthr + KD;                                 // KD
+    Real* svx = (Real*)(sthr1 + KD);                           // 3 x KD
     Real* svy = svx + KD;
     Real* svz = svy + KD;
     int* sdqx = (int*)(svz + KD);                              // 3 x KD
@@ -256,6 +275,9 @@ __global__ __launch_bounds__(64 * W) void k_pass(PassArgs P) {
     int* sdqz = sdqy + KD;
     uint32_t* sj = (uint32_t*)(sdqz + KD);                     // KD
     WavePartial* sred = (WavePartial*)(((uintptr_t)(sj + KD) + 15) & ~(uintptr_t)15);
+    float* ssur = (float*)(((uintptr_t)(sred + W) + 15) & ~(uintptr_t)15);
+    MlpRegs MR;
+    if (DA) mlp_load(P.mlp, lane, MR);
 
     const typename GBConst<Real>::type G = gb_const(C, (Real*)nullptr);
     const PairGeom X = {D.s[0], D.s[1], D.s[2], C.rc2};
@@ -321,7 +343,7 @@ __global__ __launch_bounds__(64 * W) void k_pass(PassArgs P) {
     const size_t tbase = ((size_t)P.pass * (size_t)nact + (size_t)a) * (size_t)k;
     const MoveDraw* __restrict__ drw = P.draws + C.rec_off + tbase;
     constexpr bool tracing = TR;
-    int n_att = 0, n_acc = 0, n_left = 0, n_hard = 0, n_coll = 0;
+    int n_att = 0, n_acc = 0, n_left = 0, n_hard = 0, n_coll = 0, n_s1 = 0, n_s2 = 0;
     int dcap_sum = 0, dccap_sum = 0;
     int my_pairs = 0;
     long long n_cand = 0;
@@ -351,6 +373,7 @@ __global__ __launch_bounds__(64 * W) void k_pass(PassArgs P) {
                 svy[tid] = (Real)d.v[1];
                 svz[tid] = (Real)d.v[2];
                 sthr[tid] = d.thr;
+                sthr1[tid] = d.thr1;
             }
             __syncthreads();
             const int mend = min(k, m0 + KD);
@@ -392,9 +415,9 @@ __global__ __launch_bounds__(64 * W) void k_pass(PassArgs P) {
                     LaneAcc<Real> acc;
                     Real scl = 0;
                     int cpl = 0;
+                    int base = 0;                        // this wavefront's listed candidates
                     if (staged) {
                         // candidate test against both states, wavefront-private compaction
-                        int base = 0;
                         for (int e0 = w * 64; e0 < S; e0 += 64 * W) {
                             const int e = e0 + lane;
                             bool keep = false;
@@ -416,6 +439,76 @@ __global__ __launch_bounds__(64 * W) void k_pass(PassArgs P) {
                         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
                         __builtin_amdgcn_wave_barrier();
                         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                    }
+                    // colloid-rod term of both states in FP64, only near the colloid
+                    double cacc = 0.0, cscl = 0.0;
+                    if (colloid_on) {
+                        const double rox = (double)(int32_t)(qix - 0x80000000u) * X.s0;
+                        const double roy = (double)(int32_t)(qiy - 0x80000000u) * X.s1;
+                        const double roz = (double)(int32_t)(qiz - 0x80000000u) * X.s2;
+                        const bool nearc = rox * rox + roy * roy + roz * roz <= creach2 ||
+                                           rnx * rnx + rny * rny + rnz * rnz <= creach2;
+                        n_coll += 2;
+                        if (nearc) {
+                            bool co, cn;
+                            const double eo = colloid_f64(C, rox, roy, roz, (double)uix, (double)uiy,
+                                                          (double)uiz, &co);
+                            const double en = colloid_f64(C, rnx, rny, rnz, (double)unx, (double)uny,
+                                                          (double)unz, &cn);
+                            ccd = (cn ? 1 : 0) - (co ? 1 : 0);
+                            cacc = (cn ? 0.0 : en) - (co ? 0.0 : eo);
+                            cscl = fabs(cn ? 0.0 : en) + fabs(co ? 0.0 : eo);
+                        }
+                    }
+                    // delayed acceptance, stage 1: the surrogate over this wavefront's list,
+                    // 32 (slot, state) items per matrix-core tile
+                    bool exact = true, stage1 = true;
+                    double sur = 0.0;
+                    if (DA) {
+                        float sacc = 0.0f;
+                        const int items = 2 * base;
+                        for (int t0 = 0; t0 < items; t0 += 32) {
+                            const int it = t0 + (lane & 31);
+                            bool on = false;
+                            float fr = 0.f, fa = 0.f, fb = 0.f, fc = 0.f;
+                            if (lane < 32 && it < items) {
+                                const int e = slist[it >> 1];
+                                const bool nw = it & 1;
+                                const double dx = (double)(int)(sqx[e] - (nw ? qnx : qix)) * X.s0;
+                                const double dy = (double)(int)(sqy[e] - (nw ? qny : qiy)) * X.s1;
+                                const double dz = (double)(int)(sqz[e] - (nw ? qnz : qiz)) * X.s2;
+                                const double r2 = dx * dx + dy * dy + dz * dz;
+                                if (r2 < X.rc2) {
+                                    on = true;
+                                    const float ex = (float)(nw ? unx : uix), ey = (float)(nw ? uny : uiy),
+                                                ez = (float)(nw ? unz : uiz);
+                                    const float jx = (float)sux[e], jy = (float)suy[e], jz = (float)suz[e];
+                                    const float fx = (float)dx, fy = (float)dy, fz = (float)dz;
+                                    fr = sqrtf((float)r2);
+                                    const float ir = 1.0f / fr;
+                                    float a = (fx * ex + fy * ey + fz * ez) * ir;
+                                    float bb = (fx * jx + fy * jy + fz * jz) * ir;
+                                    if (li > e) {             // pair seen from the lower slot
+                                        const float ta = a;
+                                        a = -bb;
+                                        bb = -ta;
+                                    }
+                                    fa = a; fb = bb;
+                                    fc = ex * jx + ey * jy + ez * jz;
+                                }
+                            }
+                            const float y = mlp_tile32(MR, mlp_input(fr, fa, fb, fc, on, lane));
+                            if (on) sacc += (it & 1) ? y : -y;
+                        }
+                        sacc = wave_sum_f63(sacc);
+                        if (lane == 63) ssur[w] = sacc;
+                        __syncthreads();
+                        for (int q = 0; q < W; ++q) sur += (double)ssur[q];
+                        const double dEs = (double)ccd * 1.0e8 + cacc + sur;
+                        stage1 = dEs <= sthr1[ml];
+                        exact = stage1 || P.da_judge;
+                    }
+                    if (staged && exact) {
                         const int items = 2 * base;
                         for (int it = lane; it < items; it += 64) {
                             const int e = slist[it >> 1];
@@ -434,7 +527,7 @@ __global__ __launch_bounds__(64 * W) void k_pass(PassArgs P) {
                                 }
                             }
                         }
-                    } else {
+                    } else if (!staged) {
                         // HBM path (stencil beyond the staging capacity): both states per
                         // candidate, no compaction
                         for (int e = tid; e < S; e += 64 * W) {
@@ -460,43 +553,42 @@ __global__ __launch_bounds__(64 * W) void k_pass(PassArgs P) {
                             }
                         }
                     }
-                    // wavefront partials -> LDS
-                    WavePartial part;
-                    acc.to_partial(part);
-                    part.capd = wave_sum_i63(cpl);
-                    part.scale = tracing ? wave_sum_d63((double)scl) : 0.0;
-                    if (lane == 63) sred[w] = part;
-                    // colloid-rod term of both states in FP64, only near the colloid
-                    double cacc = 0.0, cscl = 0.0;
-                    if (colloid_on) {
-                        const double rox = (double)(int32_t)(qix - 0x80000000u) * X.s0;
-                        const double roy = (double)(int32_t)(qiy - 0x80000000u) * X.s1;
-                        const double roz = (double)(int32_t)(qiz - 0x80000000u) * X.s2;
-                        const bool nearc = rox * rox + roy * roy + roz * roz <= creach2 ||
-                                           rnx * rnx + rny * rny + rnz * rnz <= creach2;
-                        n_coll += 2;
-                        if (nearc) {
-                            bool co, cn;
-                            const double eo = colloid_f64(C, rox, roy, roz, (double)uix, (double)uiy,
-                                                          (double)uiz, &co);
-                            const double en = colloid_f64(C, rnx, rny, rnz, (double)unx, (double)uny,
-                                                          (double)unz, &cn);
-                            ccd = (cn ? 1 : 0) - (co ? 1 : 0);
-                            cacc = (cn ? 0.0 : en) - (co ? 0.0 : eo);
-                            cscl = fabs(cn ? 0.0 : en) + fabs(co ? 0.0 : eo);
+                    double dpair = 0.0;
+                    n_cand += 2LL * (S - 1);
+                    if (exact) {
+                        // wavefront partials -> LDS
+                        WavePartial part;
+                        acc.to_partial(part);
+                        part.capd = wave_sum_i63(cpl);
+                        part.scale = tracing ? wave_sum_d63((double)scl) : 0.0;
+                        if (lane == 63) sred[w] = part;
+                        __syncthreads();
+                        dpair = LaneAcc<Real>::total(sred, W);
+                        capd = 0;
+                        scale = cscl;
+                        for (int q = 0; q < W; ++q) {
+                            capd += sred[q].capd;
+                            scale += sred[q].scale;
                         }
                     }
-                    __syncthreads();
-                    dfin = LaneAcc<Real>::total(sred, W) + cacc;
-                    capd = 0;
-                    scale = cscl;
-                    for (int q = 0; q < W; ++q) {
-                        capd += sred[q].capd;
-                        scale += sred[q].scale;
+                    dfin = dpair + cacc;
+                    bool accept;
+                    if (DA) {
+                        // stage 2: the exact GB pair terms against the surrogate's
+                        accept = stage1 && (double)capd * 1.0e9 + (dpair - sur) <= sthr[ml];
+                        if (exact && capd == 0 && tid == 0) {
+                            const float x = (float)((sur - dpair) / C.temperature) / kDaRange;
+                            int bin = (int)floorf((x + 1.0f) * (0.5f * kDaBins));
+                            bin = bin < 0 ? kDaBins : (bin >= kDaBins ? kDaBins + 1 : bin);
+                            atomicAdd(&P.da_hist[(size_t)r * (kDaBins + 2) + bin], 1ull);
+                        }
+                        if (!stage1) ++n_s1;
+                        else if (!accept) ++n_s2;
+                    } else {
+                        // log U <= -beta dE (MCnptColloid.c:2574)
+                        accept = (double)capd * 1.0e9 + (double)ccd * 1.0e8 + dfin <= sthr[ml];
                     }
-                    n_cand += 2LL * (S - 1);
-                    const double dE = (double)capd * 1.0e9 + (double)ccd * 1.0e8 + dfin;
-                    if (dE <= sthr[ml]) {                 // log U <= -beta dE (MCnptColloid.c:2574)
+                    if (accept) {
                         outcome = MCGB_MOVE_ACCEPT;
                         ++n_acc;
                         dcap_sum += capd;
@@ -540,7 +632,8 @@ __global__ __launch_bounds__(64 * W) void k_pass(PassArgs P) {
         CellAcc ca;
         ca.att = n_att; ca.acc = n_acc; ca.left = n_left; ca.hard = n_hard;
         ca.dcap = dcap_sum; ca.dccap = dccap_sum; ca.pairs = np; ca.coll = n_coll;
-        ca.cand = n_cand; ca.dfin = dfin_sum; ca.moved_abs = moved; ca.pad = 0.0;
+        ca.cand = n_cand; ca.dfin = dfin_sum; ca.moved_abs = moved;
+        ca.s1rej = n_s1; ca.s2rej = n_s2;
         P.acc[C.cell_off + cell] = ca;
     }
 }

@@ -1,0 +1,442 @@
+// mcgb_surrogate.hpp — the learned first stage of the delayed-acceptance sweep.
+//
+// A per-pair MLP 4 -> 64 -> 64 -> 1 with SiLU activations on the features
+// (r, u_i.r^, u_j.r^, u_i.u_j) of a GB pair inside the cutoff (r^ from i to j), fitted to the
+// exact pair energy clipped at `clip`.  The reference has no learned part.
+//
+// Parameters: one flat FP32 vector (kNParam floats), W[out][in] row-major:
+//   W1[64][4] b1[64] W2[64][64] b2[64] W3[64] b3.
+// Input normalisation: x0 = r/2 - 1 (r in [0, 4) -> [-1, 1)), x1..x3 = a, b, c.
+//
+// Inference in the sweep runs the two dense layers on the matrix cores
+// (v_mfma_f32_32x32x16_bf16: BF16 operands, FP32 accumulation), 32 pairs per wavefront tile.
+// Operand layouts (one wavefront, lane l, g = l / 32):
+//   A (32 x 16): lane l holds row l % 32, k-slots of group g (8 values);
+//   B (16 x 32): lane l holds column l % 32, k-slots of group g (8 values);
+//   C (32 x 32, 16 values): lane l holds column l % 32, rows 8 (r / 4) + 4 g + r % 4.
+// Pairs are the columns.  Layer 1 contracts the inputs split into BF16 high and low parts
+// (xh0..xh3, xl0..xl3: ~16 significant bits) against (W1, W1) in group 0 and the constant 1
+// against b1 in group 1; its C tile, after SiLU and BF16 rounding, is layer 2's B operand directly:
+// the k-slot (g, idx) of K step m carries hidden unit hidx(m / 2, 8 (m % 2) + idx, g), and the
+// host packs W2 into the matching A operand.  Only the k order changes, never the sum.
+//
+// Training (FP32, VALU): one workgroup of 64 samples computes forward and backward, the
+// per-sample intermediates meet in LDS and a per-workgroup scratch, and each thread reduces
+// one output row of every gradient; a second kernel sums the workgroup partials in a fixed
+// order; a third applies Adam.
+#pragma once
+
+#include "mcgb_device.hpp"
+
+namespace mcgb {
+
+constexpr int kHid = 64;
+constexpr int kOffW1 = 0;
+constexpr int kOffB1 = 256;
+constexpr int kOffW2 = 320;
+constexpr int kOffB2 = 4416;
+constexpr int kOffW3 = 4480;
+constexpr int kOffB3 = 4544;
+constexpr int kNParam = 4545;
+constexpr int kPairRec = 5;          // emitted record: r, a, b, c, U
+constexpr int kDaBins = 64;          // DA error histogram: 64 bins over [-kDaRange, kDaRange] kT
+constexpr float kDaRange = 4.0f;     //  + underflow + overflow
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float floatx16 __attribute__((ext_vector_type(16)));
+
+__host__ __device__ inline int mfma_row(int r, int g) { return 8 * (r >> 2) + 4 * g + (r & 3); }
+
+// Packed MFMA operands, lane-minor so that a wavefront loads them coalesced.
+struct MlpPacked {
+    uint16_t w1[2][8][64];       // [hidden tile][elem][lane]   (BF16 bits)
+    uint16_t w2[2][4][8][64];    // [out tile][K step][elem][lane]
+    float b2[2][16][64];         // [out tile][C row slot][lane]
+    float w3[2][16][64];
+    float b3;
+    float pad[15];
+};
+
+__device__ inline float silu_f(float z) {
+    const float e = __builtin_amdgcn_exp2f(-1.4426950408889634f * z);
+    return z * __builtin_amdgcn_rcpf(1.0f + e);
+}
+// d silu / dz = s (1 + z (1 - s)), s = sigmoid(z)
+__device__ inline float silu_grad(float z) {
+    const float s = __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(-1.4426950408889634f * z));
+    return s * (1.0f + z * (1.0f - s));
+}
+
+// Per-lane MFMA operands of the network, held in registers by the kernels that use them.
+struct MlpRegs {
+    bf16x8 w1[2];
+    bf16x8 w2[2][4];
+    floatx16 b2[2];
+    floatx16 w3[2];
+    float b3;
+};
+
+__device__ inline bf16x8 load_bf16x8(const uint16_t* p, int lane) {
+    bf16x8 v;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) v[e] = __builtin_bit_cast(__bf16, p[e * 64 + lane]);
+    return v;
+}
+
+__device__ inline void mlp_load(const MlpPacked* __restrict__ M, int lane, MlpRegs& R) {
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+        R.w1[t] = load_bf16x8(&M->w1[t][0][0], lane);
+#pragma unroll
+        for (int m = 0; m < 4; ++m) R.w2[t][m] = load_bf16x8(&M->w2[t][m][0][0], lane);
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            R.b2[t][r] = M->b2[t][r][lane];
+            R.w3[t][r] = M->w3[t][r][lane];
+        }
+    }
+    R.b3 = M->b3;
+}
+
+// Network input of one pair: lane group 0 supplies the inputs as BF16 high + low parts (zeros
+// when !on), lane group 1 the constant 1 of the bias slot.
+__device__ inline bf16x8 mlp_input(float r, float a, float b, float c, bool on, int lane) {
+    bf16x8 x;
+    if (lane < 32) {
+        const float v[4] = {0.5f * r - 1.0f, a, b, c};
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const float f = on ? v[e] : 0.0f;
+            const __bf16 h = (__bf16)f;
+            x[e] = h;
+            x[4 + e] = (__bf16)(f - (float)h);
+        }
+    } else {
+        x[0] = (__bf16)1.0f;
+#pragma unroll
+        for (int e = 1; e < 8; ++e) x[e] = (__bf16)0.0f;
+    }
+    return x;
+}
+
+// Outputs of the 32 pairs whose inputs lanes 0..31 supplied: lanes j and j + 32 both return
+// pair j's value.  Must be called by the whole wavefront (wavefront-uniform control flow).
+__device__ inline float mlp_tile32(const MlpRegs& R, bf16x8 X) {
+    floatx16 z1a = {}, z1b = {};
+    z1a = __builtin_amdgcn_mfma_f32_32x32x16_bf16(R.w1[0], X, z1a, 0, 0, 0);
+    z1b = __builtin_amdgcn_mfma_f32_32x32x16_bf16(R.w1[1], X, z1b, 0, 0, 0);
+    bf16x8 h[4];
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+        h[0][e] = (__bf16)silu_f(z1a[e]);
+        h[1][e] = (__bf16)silu_f(z1a[8 + e]);
+        h[2][e] = (__bf16)silu_f(z1b[e]);
+        h[3][e] = (__bf16)silu_f(z1b[8 + e]);
+    }
+    floatx16 z2a = R.b2[0], z2b = R.b2[1];
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+        z2a = __builtin_amdgcn_mfma_f32_32x32x16_bf16(R.w2[0][m], h[m], z2a, 0, 0, 0);
+        z2b = __builtin_amdgcn_mfma_f32_32x32x16_bf16(R.w2[1][m], h[m], z2b, 0, 0, 0);
+    }
+    float y = 0.0f;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) y += R.w3[0][r] * silu_f(z2a[r]) + R.w3[1][r] * silu_f(z2b[r]);
+    y += __shfl_xor(y, 32);
+    return y + R.b3;
+}
+
+// FP32 reference forward of one pair (VALU), P = flat parameter vector.
+__device__ inline float mlp_fwd_f32(const float* __restrict__ P, const float x[4]) {
+    float h1[kHid];
+#pragma unroll
+    for (int h = 0; h < kHid; ++h) {
+        float z = P[kOffB1 + h];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) z += P[kOffW1 + 4 * h + k] * x[k];
+        h1[h] = silu_f(z);
+    }
+    float y = P[kOffB3];
+    for (int o = 0; o < kHid; ++o) {
+        float z = P[kOffB2 + o];
+#pragma unroll
+        for (int h = 0; h < kHid; ++h) z += P[kOffW2 + kHid * o + h] * h1[h];
+        y += P[kOffW3 + o] * silu_f(z);
+    }
+    return y;
+}
+
+// ------------------------------------------------------------------ inference kernels (tests)
+
+// y[i] of n feature rows x[i] = (r, a, b, c); mfma = 1: BF16 matrix-core tiles, else FP32.
+__global__ __launch_bounds__(64) void k_mlp_eval(const float* __restrict__ P,
+                                                 const MlpPacked* __restrict__ M, int n,
+                                                 const float* __restrict__ x, int mfma,
+                                                 float* __restrict__ y) {
+    const int lane = threadIdx.x;
+    if (mfma) {
+        MlpRegs R;
+        mlp_load(M, lane, R);
+        const int j = blockIdx.x * 32 + (lane & 31);
+        const bool on = lane < 32 && j < n;
+        float f[4] = {0.f, 0.f, 0.f, 0.f};
+        if (on)
+            for (int k = 0; k < 4; ++k) f[k] = x[4 * (size_t)j + k];
+        const float v = mlp_tile32(R, mlp_input(f[0], f[1], f[2], f[3], on, lane));
+        if (on) y[j] = v;
+    } else {
+        const int j = blockIdx.x * 64 + lane;
+        if (j >= n) return;
+        float f[4];
+        for (int k = 0; k < 4; ++k) f[k] = x[4 * (size_t)j + k];
+        f[0] = 0.5f * f[0] - 1.0f;
+        y[j] = mlp_fwd_f32(P, f);
+    }
+}
+
+// ------------------------------------------------------------------ training kernels
+
+// Gradient of (1/n) sum w (y - min(t, clip))^2 over one workgroup's 64 samples, with
+// w = 1 (wfloor <= 0) or w = (wfloor + 1) / (wfloor + |min(t, clip)|) (relative weighting).
+//   data != nullptr: sample s of step `step` is record floor(u * ndata), u a Philox draw
+//   (key seed, counter {sample, step, 0, 0x5A5A0000}), mapped by a random element of the
+//   energy's symmetry group (u_i -> -u_i: (a, c) -> (-a, -c); u_j -> -u_j: (b, c) -> (-b, -c);
+//   i <-> j: (a, b) -> (-b, -a)); else samples are xs/ts[0, nexp) as given.
+// Writes kNParam partial sums to partial[blockIdx.x] and the loss sum to lpart[blockIdx.x].
+// scratch: per-workgroup 2 x 64 x 64 floats (h2 and dz1 rows).
+__global__ __launch_bounds__(64) void k_mlp_grad(const float* __restrict__ Pg,
+                                                 const float* __restrict__ data, long long ndata,
+                                                 const float* __restrict__ xs,
+                                                 const float* __restrict__ ts, int nexp,
+                                                 uint32_t step, uint32_t key0, uint32_t key1,
+                                                 float clip, float wfloor, float inv_n,
+                                                 float* __restrict__ scratch,
+                                                 float* __restrict__ partial,
+                                                 double* __restrict__ lpart) {
+    __shared__ float P[kNParam + 3];
+    __shared__ float A[64][65];       // h1 rows
+    __shared__ float B[64][65];       // dz2 rows
+    __shared__ float X[64][4];
+    __shared__ float DY[64];
+    const int t = threadIdx.x;
+    for (int p = t; p < kNParam; p += 64) P[p] = Pg[p];
+    const size_t s = (size_t)blockIdx.x * 64 + t;
+    float x[4] = {0.f, 0.f, 0.f, 0.f}, tg = 0.f;
+    bool valid;
+    if (data) {
+        uint32_t u[4];
+        philox4x32_10((uint32_t)s, step, 0u, 0x5A5A0000u, key0, key1, u);
+        const long long idx = (long long)(((uint64_t)u[0] * (uint64_t)ndata) >> 32);
+        const float* rec = data + (size_t)idx * kPairRec;
+        x[0] = rec[0]; x[1] = rec[1]; x[2] = rec[2]; x[3] = rec[3]; tg = rec[4];
+        if (u[1] & 1u) { x[1] = -x[1]; x[3] = -x[3]; }
+        if (u[1] & 2u) { x[2] = -x[2]; x[3] = -x[3]; }
+        if (u[1] & 4u) { const float ta = x[1]; x[1] = -x[2]; x[2] = -ta; }
+        valid = true;
+    } else {
+        valid = (long long)s < (long long)nexp;
+        if (valid) {
+            for (int k = 0; k < 4; ++k) x[k] = xs[4 * s + k];
+            tg = ts[s];
+        }
+    }
+    tg = tg < clip ? tg : clip;
+    x[0] = 0.5f * x[0] - 1.0f;
+    for (int k = 0; k < 4; ++k) X[t][k] = x[k];
+    __syncthreads();
+
+    float* H2 = scratch + ((size_t)blockIdx.x * 2 + 0) * 4096;    // h2 of sample q at q*64 + o
+    float* DZ1 = scratch + ((size_t)blockIdx.x * 2 + 1) * 4096;   // dz1 of sample q at q*64 + h
+
+    // forward
+    float h1[kHid];
+#pragma unroll
+    for (int h = 0; h < kHid; ++h) {
+        float z = P[kOffB1 + h];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) z += P[kOffW1 + 4 * h + k] * x[k];
+        h1[h] = silu_f(z);
+        A[t][h] = h1[h];
+    }
+    float z2[kHid];
+#pragma unroll
+    for (int o = 0; o < kHid; ++o) {
+        float z = P[kOffB2 + o];
+#pragma unroll
+        for (int h = 0; h < kHid; ++h) z += P[kOffW2 + kHid * o + h] * h1[h];
+        z2[o] = z;
+    }
+    float y = P[kOffB3];
+#pragma unroll
+    for (int o = 0; o < kHid; ++o) y += P[kOffW3 + o] * silu_f(z2[o]);
+    const float diff = y - tg;
+    const float wt = wfloor > 0.0f ? (wfloor + 1.0f) / (wfloor + fabsf(tg)) : 1.0f;
+    const float dy = valid ? 2.0f * wt * diff * inv_n : 0.0f;
+    DY[t] = dy;
+    const double loss = valid ? (double)wt * (double)diff * (double)diff : 0.0;
+    // backward to layer 2
+#pragma unroll
+    for (int o = 0; o < kHid; ++o) {
+        H2[t * 64 + o] = silu_f(z2[o]);
+        B[t][o] = dy * P[kOffW3 + o] * silu_grad(z2[o]);
+    }
+    // backward to layer 1 (dz2 from this thread's own LDS row)
+    float dh1[kHid];
+#pragma unroll
+    for (int h = 0; h < kHid; ++h) dh1[h] = 0.0f;
+    for (int o = 0; o < kHid; ++o) {
+        const float d = B[t][o];
+#pragma unroll
+        for (int h = 0; h < kHid; ++h) dh1[h] += P[kOffW2 + kHid * o + h] * d;
+    }
+#pragma unroll
+    for (int h = 0; h < kHid; ++h) {
+        float z = P[kOffB1 + h];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) z += P[kOffW1 + 4 * h + k] * x[k];
+        DZ1[t * 64 + h] = dh1[h] * silu_grad(z);
+    }
+    __threadfence_block();
+    __syncthreads();
+
+    // reductions over the 64 samples: thread t owns output row t of every matrix
+    float gw2[kHid];
+#pragma unroll
+    for (int h = 0; h < kHid; ++h) gw2[h] = 0.0f;
+    float gb2 = 0.f, gw3 = 0.f, gb1 = 0.f, gw1[4] = {0.f, 0.f, 0.f, 0.f};
+    for (int q = 0; q < 64; ++q) {
+        const float d2 = B[q][t];
+#pragma unroll
+        for (int h = 0; h < kHid; ++h) gw2[h] += d2 * A[q][h];
+        gb2 += d2;
+        gw3 += DY[q] * H2[q * 64 + t];
+        const float d1 = DZ1[q * 64 + t];
+        gb1 += d1;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) gw1[k] += d1 * X[q][k];
+    }
+    float* out = partial + (size_t)blockIdx.x * kNParam;
+#pragma unroll
+    for (int h = 0; h < kHid; ++h) out[kOffW2 + kHid * t + h] = gw2[h];
+    out[kOffB2 + t] = gb2;
+    out[kOffW3 + t] = gw3;
+    out[kOffB1 + t] = gb1;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) out[kOffW1 + 4 * t + k] = gw1[k];
+    // db3: fixed-order sum by thread 0
+    if (t == 0) {
+        float gb3 = 0.f;
+        for (int q = 0; q < 64; ++q) gb3 += DY[q];
+        out[kOffB3] = gb3;
+    }
+    // loss: wavefront tree (fixed order)
+    double l = loss;
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) l += __shfl_xor(l, off);
+    if (t == 0) lpart[blockIdx.x] = l;
+}
+
+// grad[p] = sum over workgroups of partial[b][p], in workgroup order; loss likewise.
+__global__ __launch_bounds__(256) void k_mlp_reduce(const float* __restrict__ partial, int nblk,
+                                                    const double* __restrict__ lpart,
+                                                    float* __restrict__ grad,
+                                                    double* __restrict__ loss) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < kNParam) {
+        float g = 0.f;
+        for (int b = 0; b < nblk; ++b) g += partial[(size_t)b * kNParam + p];
+        grad[p] = g;
+    }
+    if (p == 0 && loss) {
+        double l = 0.0;
+        for (int b = 0; b < nblk; ++b) l += lpart[b];
+        *loss = l;
+    }
+}
+
+// Adam (Kingma & Ba 2015) on the flat parameter vector; step = 1, 2, ...
+__global__ __launch_bounds__(256) void k_mlp_adam(float* __restrict__ P, const float* __restrict__ g,
+                                                  float* __restrict__ m, float* __restrict__ v,
+                                                  int step, float lr, float b1, float b2, float eps) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= kNParam) return;
+    const float gp = g[p];
+    const float mp = b1 * m[p] + (1.0f - b1) * gp;
+    const float vp = b2 * v[p] + (1.0f - b2) * gp * gp;
+    m[p] = mp;
+    v[p] = vp;
+    const float mh = mp / (1.0f - powf(b1, (float)step));
+    const float vh = vp / (1.0f - powf(b2, (float)step));
+    P[p] -= lr * mh / (sqrtf(vh) + eps);
+}
+
+// ------------------------------------------------------------------ exact pair emitter
+
+// Partners within the cutoff of the sampled list positions i = offset + m * stride (m < M):
+// pass 0 counts them into cnt[m]; pass 1 writes records (r, a, b, c, U) at out + 5*(off[m] + .),
+// U the FP64 exact pair energy (1e30 for a capped pair).  Every ordered pair (i, j), j != i.
+template <typename Real>
+__global__ __launch_bounds__(128) void k_emit(const RepConst* __restrict__ RC,
+                                              const RepDyn* __restrict__ RD, int r, int pass,
+                                              const uint32_t* __restrict__ qx,
+                                              const uint32_t* __restrict__ qy,
+                                              const uint32_t* __restrict__ qz,
+                                              const Real* __restrict__ ux, const Real* __restrict__ uy,
+                                              const Real* __restrict__ uz,
+                                              const int* __restrict__ cell_start, int stride,
+                                              int offset, int M, int* __restrict__ cnt,
+                                              const long long* __restrict__ off,
+                                              float* __restrict__ out) {
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= M) return;
+    const RepConst& C = RC[r];
+    const RepDyn& D = RD[r];
+    const int i = offset + m * stride;
+    if (i >= C.N) { if (pass == 0) cnt[m] = 0; return; }
+    const int n0 = D.n[0], n1 = D.n[1], n2 = D.n[2];
+    const double s0 = D.s[0], s1 = D.s[1], s2 = D.s[2];
+    const int po = C.part_off;
+    const int gi = po + i;
+    const uint32_t xi = qx[gi], yi = qy[gi], zi = qz[gi];
+    double uix = ux[gi], uiy = uy[gi], uiz = uz[gi];
+    const double ni = 1.0 / sqrt(uix * uix + uiy * uiy + uiz * uiz);
+    const int cx = (int)cell_coord(xi, D.g[0], n0), cy = (int)cell_coord(yi, D.g[1], n1),
+              cz = (int)cell_coord(zi, D.g[2], n2);
+    const int* cs = cell_start + C.cs_off;
+    const GBd gd = make_gbd(C);
+    int c = 0;
+    long long base = pass ? off[m] : 0;
+    for (int e = 0; e < 27; ++e) {
+        int ex = cx + (e % 3) - 1, ey = cy + ((e / 3) % 3) - 1, ez = cz + e / 9 - 1;
+        ex += ex < 0 ? n0 : 0; ex -= ex >= n0 ? n0 : 0;
+        ey += ey < 0 ? n1 : 0; ey -= ey >= n1 ? n1 : 0;
+        ez += ez < 0 ? n2 : 0; ez -= ez >= n2 ? n2 : 0;
+        const int nb = ex + n0 * (ey + n1 * ez);
+        for (int kk = cs[nb]; kk < cs[nb + 1]; ++kk) {
+            if (kk == i) continue;
+            const int gj = po + kk;
+            const double dx = (double)(int32_t)(qx[gj] - xi) * s0;
+            const double dy = (double)(int32_t)(qy[gj] - yi) * s1;
+            const double dz = (double)(int32_t)(qz[gj] - zi) * s2;
+            const double r2 = dx * dx + dy * dy + dz * dz;
+            if (!(r2 < C.rc2)) continue;
+            if (pass == 1) {
+                const double ujx = ux[gj], ujy = uy[gj], ujz = uz[gj];
+                bool cap;
+                const double u = gb_pair_f64(gd, uix, uiy, uiz, ujx, ujy, ujz, dx, dy, dz, r2, &cap);
+                const double rr = sqrt(r2), ir = 1.0 / rr;
+                const double nj = 1.0 / sqrt(ujx * ujx + ujy * ujy + ujz * ujz);
+                float* o = out + (size_t)(base + c) * kPairRec;
+                o[0] = (float)rr;
+                o[1] = (float)((dx * uix + dy * uiy + dz * uiz) * ir * ni);
+                o[2] = (float)((dx * ujx + dy * ujy + dz * ujz) * ir * nj);
+                o[3] = (float)((uix * ujx + uiy * ujy + uiz * ujz) * ni * nj);
+                o[4] = cap ? 1.0e30f : (float)u;
+            }
+            ++c;
+        }
+    }
+    if (pass == 0) cnt[m] = c;
+}
+
+}  // namespace mcgb

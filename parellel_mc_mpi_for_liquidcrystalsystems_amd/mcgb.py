@@ -63,7 +63,8 @@ class Stats(C.Structure):
         "candidates", "pairs", "colloid_evals", "volume_attempted", "volume_accepted")] + [
         ("cap_pairs", C.c_int64), ("cap_colloid", C.c_int64), ("energy_finite", C.c_double),
         ("energy", C.c_double), ("box", C.c_double * 3), ("volume", C.c_double),
-        ("density", C.c_double), ("max_drift", C.c_double), ("resummations", C.c_uint64)]
+        ("density", C.c_double), ("max_drift", C.c_double), ("resummations", C.c_uint64),
+        ("da_stage1_rejected", C.c_uint64), ("da_stage2_rejected", C.c_uint64)]
 
     def as_dict(self):
         d = {n: getattr(self, n) for n, _ in self._fields_}
@@ -75,6 +76,11 @@ _lib = None
 dp = C.POINTER(C.c_double)
 up = C.POINTER(C.c_uint32)
 ip = C.POINTER(C.c_int)
+fp = C.POINTER(C.c_float)
+u64p = C.POINTER(C.c_uint64)
+llp = C.POINTER(C.c_longlong)
+SUR_NPARAM = 4545
+DA_BINS = 66
 
 
 def lib():
@@ -119,6 +125,21 @@ def lib():
         "mcgb_eigen_sym3": (C.c_int, [dp, dp, dp]),
         "mcgb_write_fort1": (C.c_int, [C.c_char_p, C.c_int, dp, dp, dp, dp, dp, dp, dp, C.c_double]),
         "mcgb_read_fort1": (C.c_int, [C.c_char_p, C.c_int, dp, dp, dp, dp, dp, dp, dp, dp]),
+        "mcgb_surrogate_set": (C.c_int, [ctx, fp]),
+        "mcgb_surrogate_get": (C.c_int, [ctx, fp]),
+        "mcgb_surrogate_init": (C.c_int, [ctx, C.c_uint32]),
+        "mcgb_surrogate_eval": (C.c_int, [ctx, C.c_int, fp, C.c_int, fp]),
+        "mcgb_surrogate_grad": (C.c_int, [ctx, fp, C.c_int, fp, fp, C.c_float, fp, dp]),
+        "mcgb_surrogate_train": (C.c_int, [ctx, C.c_int, C.c_int, C.c_float, C.c_float, C.c_float,
+                                           C.c_uint32, fp]),
+        "mcgb_pairs_reserve": (C.c_int, [ctx, C.c_longlong]),
+        "mcgb_pairs_clear": (C.c_int, [ctx]),
+        "mcgb_pairs_count": (C.c_int, [ctx, llp]),
+        "mcgb_pairs_get": (C.c_int, [ctx, C.c_longlong, C.c_int, fp]),
+        "mcgb_pairs_put": (C.c_int, [ctx, C.c_int, fp]),
+        "mcgb_emit_pairs": (C.c_int, [ctx, C.c_int, C.c_int, C.c_int, llp]),
+        "mcgb_da_enable": (C.c_int, [ctx, C.c_int, C.c_int]),
+        "mcgb_da_histogram": (C.c_int, [ctx, C.c_int, u64p, C.c_int]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -144,6 +165,10 @@ def _u(a):
 
 def _i(a):
     return a.ctypes.data_as(ip)
+
+
+def _f(a):
+    return a.ctypes.data_as(fp)
 
 
 def default_params(**kw):
@@ -367,3 +392,74 @@ class Context:
                                    _u(order), _i(dims)))
         nc = int(np.prod(dims))
         return cs[:nc + 1].copy(), ids, shift, order, dims
+
+    # ------------------------------------------------------------ learned first stage
+
+    def surrogate_set(self, params):
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        assert p.shape == (SUR_NPARAM,)
+        check(lib().mcgb_surrogate_set(self.h, _f(p)))
+
+    def surrogate_get(self):
+        p = np.empty(SUR_NPARAM, dtype=np.float32)
+        check(lib().mcgb_surrogate_get(self.h, _f(p)))
+        return p
+
+    def surrogate_init(self, seed):
+        check(lib().mcgb_surrogate_init(self.h, seed))
+
+    def surrogate_eval(self, x, mfma=True):
+        x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, 4)
+        y = np.empty(x.shape[0], dtype=np.float32)
+        check(lib().mcgb_surrogate_eval(self.h, x.shape[0], _f(x), 1 if mfma else 0, _f(y)))
+        return y
+
+    def surrogate_grad(self, x, t, clip=10.0, params=None):
+        x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, 4)
+        t = np.ascontiguousarray(t, dtype=np.float32)
+        g = np.empty(SUR_NPARAM, dtype=np.float32)
+        loss = C.c_double()
+        pp = None if params is None else _f(np.ascontiguousarray(params, dtype=np.float32))
+        check(lib().mcgb_surrogate_grad(self.h, pp, x.shape[0], _f(x), _f(t), clip, _f(g),
+                                        C.byref(loss)))
+        return g, loss.value
+
+    def surrogate_train(self, steps, batch=8192, lr=1e-3, clip=10.0, seed=1, wfloor=0.0):
+        losses = np.empty(steps, dtype=np.float32)
+        check(lib().mcgb_surrogate_train(self.h, steps, batch, lr, clip, wfloor, seed, _f(losses)))
+        return losses
+
+    def pairs_reserve(self, capacity):
+        check(lib().mcgb_pairs_reserve(self.h, capacity))
+
+    def pairs_clear(self):
+        check(lib().mcgb_pairs_clear(self.h))
+
+    def pairs_count(self):
+        n = C.c_longlong()
+        check(lib().mcgb_pairs_count(self.h, C.byref(n)))
+        return n.value
+
+    def pairs_get(self, first=0, n=None):
+        if n is None:
+            n = self.pairs_count() - first
+        out = np.empty((n, 5), dtype=np.float32)
+        check(lib().mcgb_pairs_get(self.h, first, n, _f(out)))
+        return out
+
+    def pairs_put(self, rec):
+        rec = np.ascontiguousarray(rec, dtype=np.float32).reshape(-1, 5)
+        check(lib().mcgb_pairs_put(self.h, rec.shape[0], _f(rec)))
+
+    def emit_pairs(self, stride=1, offset=0, replica=0):
+        n = C.c_longlong()
+        check(lib().mcgb_emit_pairs(self.h, replica, stride, offset, C.byref(n)))
+        return n.value
+
+    def da_enable(self, on=True, judge=False):
+        check(lib().mcgb_da_enable(self.h, 1 if on else 0, 1 if judge else 0))
+
+    def da_histogram(self, replica=0, clear=False):
+        h = np.zeros(DA_BINS, dtype=np.uint64)
+        check(lib().mcgb_da_histogram(self.h, replica, h.ctypes.data_as(u64p), 1 if clear else 0))
+        return h

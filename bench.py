@@ -157,6 +157,72 @@ def da_leg(args, pairs_plain, waves):
     return out
 
 
+def npt_leg(args, waves):
+    """NPT: N = 64000 GB(3,5,2,1), no colloid, P* = 2, T* = 1, start rho* = 0.28 (jittered
+    lattice), one isotropic ln V move (max 0.01) per sweep with a full FP64 recompute."""
+    N, rho0, T, P = 64000, 0.28, 1.0, 2.0
+    seed = 20261204
+    p = m.gb_params(N, rho0, T, precision=32, seed=seed, npt=True, pressure=P,
+                    npt_max_dlnv=0.01, lattice_jitter=0.05)
+    p.waves_per_cell = waves
+    pos, u, box = m.init_lattice(N, rho0, jitter=0.05, seed=seed)
+    with m.Context(p) as ctx:
+        ctx.set_state(pos, u, box)
+        ctx.sweep(2)
+        ctx.reset_counters()
+        ctx.series_clear()
+        t0 = time.perf_counter()
+        ctx.sweep(args.npt_sweeps)
+        dt = time.perf_counter() - t0
+        st = ctx.stats()
+        e, p2, rho = ctx.series(max_n=max(1 << 16, args.npt_sweeps + 16))
+    h = len(rho) // 2
+    return {"workload": "NPT N=64000 P*=2 T*=1 from rho*=0.28, one ln V move per sweep",
+            "sweeps": args.npt_sweeps, "moves_per_s": st["attempted"] / dt,
+            "ms_per_sweep": 1e3 * dt / args.npt_sweeps,
+            "volume_acceptance": st["volume_accepted"] / max(1, st["volume_attempted"]),
+            "density_start_end": [float(rho[0]), float(rho[-1])],
+            "density_mean_second_half": float(np.mean(rho[h:])),
+            "P2_mean_second_half": float(np.mean(p2[h:])),
+            "ess_per_s": {"density": ess_per_s(rho[h:], dt / 2), "P2": ess_per_s(p2[h:], dt / 2)}}
+
+
+def replicas_leg(args):
+    """32 NVT replicas of GB(3,5,2,1) at rho* = 0.30: N log-uniform in [2000, 32000] (seeded),
+    T* = 0.8 .. 1.6 evenly, replicas with index = 0 mod 4 carry an R = 3 colloid.  Every
+    replica advances in the same kernel launches (one launch per sublattice pass covers the
+    active cells of all replicas), the sweep sequence replayed as a HIP graph."""
+    seed = 20261205
+    rng = np.random.default_rng(seed)
+    Ns = np.round(np.exp(rng.uniform(np.log(2000), np.log(32000), 32))).astype(int)
+    params, states = [], []
+    for i in range(32):
+        R = 3.0 if i % 4 == 0 else 0.0
+        T = 0.8 + i * 0.8 / 31
+        params.append(m.gb_params(int(Ns[i]), 0.30, T, colloid_radius=R, precision=32, seed=seed,
+                                  lattice_jitter=0.05,
+                                  colloid_exclusion_radius=R + 0.5 if R > 0 else 0.0))
+        states.append(m.init_lattice(int(Ns[i]), 0.30, colloid_radius=R,
+                                     exclusion_radius=R + 0.5 if R > 0 else -1.0, jitter=0.05,
+                                     seed=seed + i))
+    with m.Context(params) as ctx:
+        for i, (pos, u, box) in enumerate(states):
+            ctx.set_state(pos, u, box, replica=i)
+        ctx.sweep(args.relax // 2)
+        best, ms = ctx.autotune(4)
+        ctx.sweep(5)
+        a0 = sum(ctx.stats(replica=i)["attempted"] for i in range(32))
+        t0 = time.perf_counter()
+        ctx.sweep(args.steps)
+        dt = time.perf_counter() - t0
+        a1 = sum(ctx.stats(replica=i)["attempted"] for i in range(32))
+    return {"workload": "32 NVT replicas, N log-uniform in [2000, 32000], T* 0.8..1.6, rho*=0.30",
+            "N_total": int(Ns.sum()), "N_min_max": [int(Ns.min()), int(Ns.max())],
+            "sweeps": args.steps, "moves_per_s": (a1 - a0) / dt,
+            "ms_per_sweep": 1e3 * dt / args.steps, "waves_per_cell": best,
+            "autotune_ms_per_pass": ms}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -172,6 +238,9 @@ def main():
     ap.add_argument("--da-sweeps", type=int, default=100)
     ap.add_argument("--pairs", type=int, default=10 ** 7, help="training pairs for the surrogate")
     ap.add_argument("--train-steps", type=int, default=20000)
+    ap.add_argument("--no-npt", action="store_true", help="skip the NPT leg")
+    ap.add_argument("--npt-sweeps", type=int, default=5000)
+    ap.add_argument("--no-replicas", action="store_true", help="skip the 32-replica leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the sweep is not "
@@ -253,6 +322,10 @@ def main():
     }
     if not args.no_da:
         out["da"] = da_leg(args, pairs_plain, tune_w or 2)
+    if not args.no_npt:
+        out["npt"] = npt_leg(args, tune_w or 2)
+    if not args.no_replicas:
+        out["replicas"] = replicas_leg(args)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(pos, u, box, N, rho, T, R, seed, sweeps=args.cpu_sweeps)
     print(json.dumps(out))

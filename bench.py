@@ -88,11 +88,12 @@ def timed(ctx, sweeps):
     return dt, st, e, p2
 
 
-def da_leg(args, pairs_plain, waves):
+def da_leg(args, pairs_plain, waves, schedule):
     """Learned delayed acceptance against the plain sweep on the T* = 0.8 state."""
     N, rho, T, R = args.n, 0.32, 0.8, 5.0
     p, pos, u, box = workload(N, rho, T, R)
     p.waves_per_cell = waves
+    p.pass_schedule = schedule
     out = {"workload": "delayed acceptance vs plain, same geometry at T*=0.8",
            "T": T, "sweeps": args.da_sweeps}
     with m.Context(p) as ctx:
@@ -157,7 +158,7 @@ def da_leg(args, pairs_plain, waves):
     return out
 
 
-def npt_leg(args, waves):
+def npt_leg(args, waves, schedule):
     """NPT: N = 64000 GB(3,5,2,1), no colloid, P* = 2, T* = 1, start rho* = 0.28 (jittered
     lattice), one isotropic ln V move (max 0.01) per sweep with a full FP64 recompute."""
     N, rho0, T, P = 64000, 0.28, 1.0, 2.0
@@ -165,6 +166,7 @@ def npt_leg(args, waves):
     p = m.gb_params(N, rho0, T, precision=32, seed=seed, npt=True, pressure=P,
                     npt_max_dlnv=0.01, lattice_jitter=0.05)
     p.waves_per_cell = waves
+    p.pass_schedule = schedule
     pos, u, box = m.init_lattice(N, rho0, jitter=0.05, seed=seed)
     with m.Context(p) as ctx:
         ctx.set_state(pos, u, box)
@@ -209,7 +211,7 @@ def replicas_leg(args):
         for i, (pos, u, box) in enumerate(states):
             ctx.set_state(pos, u, box, replica=i)
         ctx.sweep(args.relax // 2)
-        best, ms = ctx.autotune(4)
+        best, sched, ms = ctx.autotune(4)
         ctx.sweep(5)
         a0 = sum(ctx.stats(replica=i)["attempted"] for i in range(32))
         t0 = time.perf_counter()
@@ -220,7 +222,7 @@ def replicas_leg(args):
             "N_total": int(Ns.sum()), "N_min_max": [int(Ns.min()), int(Ns.max())],
             "sweeps": args.steps, "moves_per_s": (a1 - a0) / dt,
             "ms_per_sweep": 1e3 * dt / args.steps, "waves_per_cell": best,
-            "autotune_ms_per_pass": ms}
+            "schedule": "speculative" if sched else "sequential", "autotune_ms_per_pass": ms}
 
 
 def main():
@@ -235,6 +237,8 @@ def main():
     ap.add_argument("--relax", type=int, default=200,
                     help="untimed sweeps from the lattice start before tuning and warm-up")
     ap.add_argument("--waves", type=int, default=0, help="wavefronts per cell (0: autotune)")
+    ap.add_argument("--schedule", type=int, default=1,
+                    help="pass schedule with --waves: 1 speculative, 0 sequential")
     ap.add_argument("--da-sweeps", type=int, default=100)
     ap.add_argument("--pairs", type=int, default=10 ** 7, help="training pairs for the surrogate")
     ap.add_argument("--train-steps", type=int, default=20000)
@@ -254,6 +258,7 @@ def main():
     seed = int(p.seed)
     if args.waves:
         p.waves_per_cell = args.waves
+        p.pass_schedule = args.schedule
     pairs_plain = None
     with m.Context(p) as ctx:
         ctx.set_state(pos, u, box)
@@ -263,10 +268,12 @@ def main():
         t_relax = time.perf_counter() - t_relax
         tune = None
         tune_w = args.waves
+        tune_s = args.schedule
         if not args.waves:
-            best, ms = ctx.autotune(4)
-            tune = {"waves_per_cell": best, "ms_per_pass": ms}
-            tune_w = best
+            best, sched, ms = ctx.autotune(4)
+            tune = {"waves_per_cell": best, "schedule": "speculative" if sched else "sequential",
+                    "ms_per_pass": ms}
+            tune_w, tune_s = best, sched
         if args.warmup:
             ctx.sweep(args.warmup)
         dt, st, e, p2 = timed(ctx, args.steps)
@@ -306,7 +313,7 @@ def main():
                    "lattice_jitter": 0.05, "seed": seed, "step": "one sweep (8 sublattice passes)"},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP32_TFLOPS,
-                     "traffic": TRAFFIC_PASS_BYTES if tune_w == 2 else None,
+                     "traffic": TRAFFIC_PASS_BYTES if (tune_w == 2 and tune_s == 0) else None,
                      "kernel": "k_pass<float>", "ms_per_launch": kt["ms_pass"],
                      "flop_per_launch": flops_pass},
         "acceptance": st["accepted"] / max(1, moves),
@@ -321,9 +328,9 @@ def main():
         "autotune": tune,
     }
     if not args.no_da:
-        out["da"] = da_leg(args, pairs_plain, tune_w or 2)
+        out["da"] = da_leg(args, pairs_plain, tune_w or 2, tune_s)
     if not args.no_npt:
-        out["npt"] = npt_leg(args, tune_w or 2)
+        out["npt"] = npt_leg(args, tune_w or 2, tune_s)
     if not args.no_replicas:
         out["replicas"] = replicas_leg(args)
     if not args.no_cpu:

@@ -74,6 +74,8 @@ typedef struct mcgb_params {
     double   colloid_exclusion_radius;      /* init: sites closer than this are removed    */
     int      stage_capacity;                /* particles staged in LDS per stencil, 0: auto */
     int      waves_per_cell;                /* wavefronts per active cell: 1, 2, 4 or 8     */
+    int      pass_schedule;                 /* 1: speculative (moves of a cell evaluated in
+                                               parallel, decided in order), 0: sequential  */
 
     uint32_t keys_read;                     /* bit i set: required key i was read          */
 } mcgb_params;
@@ -217,11 +219,13 @@ int mcgb_series(mcgb_ctx *ctx, int replica, int max,
                 double *energy_per_particle, double *p2, double *density);
 int mcgb_series_clear(mcgb_ctx *ctx);
 
-/* Autotuner: times `sweeps` sweeps with each supported number of wavefronts per active cell
- * (1, 2, 4, 8; the LDS staging capacity follows) and keeps the fastest.  The timed sweeps are
- * ordinary sweeps of the simulation (they advance it).  No counterpart in the reference.
- * ms_per_pass (may be NULL) receives the measured pass times in that order. */
-int mcgb_autotune(mcgb_ctx *ctx, int sweeps, int *best_waves, double ms_per_pass[4]);
+/* Autotuner: times `sweeps` sweeps with each pass schedule (sequential, speculative) and each
+ * supported number of wavefronts per active cell (1, 2, 4, 8; the LDS staging capacity
+ * follows) and keeps the fastest.  The timed sweeps are ordinary sweeps of the simulation (they
+ * advance it).  No counterpart in the reference.  ms_per_pass (may be NULL) receives the
+ * measured pass times, index 4*schedule + {0,1,2,3} for 1, 2, 4, 8 wavefronts. */
+int mcgb_autotune(mcgb_ctx *ctx, int sweeps, int *best_waves, int *best_schedule,
+                  double ms_per_pass[8]);
 
 /* Average device time per launch of the sweep-pass kernel, the cell-list rebuild and the whole
  * sweep, measured with HIP events on the context's stream over the last mcgb_sweep() call. */

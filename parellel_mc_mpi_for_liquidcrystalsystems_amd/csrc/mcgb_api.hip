@@ -57,6 +57,8 @@ struct mcgb_ctx {
     int cap_request = 0;
     int waves = 4;
     size_t lds = 0;
+    size_t lds_spec = 0;
+    bool spec = true;          // speculative pass schedule for untraced plain sweeps
     int resum_interval = 100;
     int series_cap = kSeriesCapDefault;
     bool any_npt = false;
@@ -263,6 +265,13 @@ void launch_pass_t(mcgb_ctx* c, const PassArgs& P) {
         case 8: k_pass<Real, 8, false, false, true><<<c->nblk, 512, c->lds, c->st>>>(P); break;
         default: k_pass<Real, 4, false, false, true><<<c->nblk, 256, c->lds, c->st>>>(P); break;
         }
+    } else if (!TR && c->spec) {
+        switch (c->waves) {
+        case 1: k_pass_spec<Real, 1><<<c->nblk, 64, c->lds_spec, c->st>>>(P); break;
+        case 2: k_pass_spec<Real, 2><<<c->nblk, 128, c->lds_spec, c->st>>>(P); break;
+        case 8: k_pass_spec<Real, 8><<<c->nblk, 512, c->lds_spec, c->st>>>(P); break;
+        default: k_pass_spec<Real, 4><<<c->nblk, 256, c->lds_spec, c->st>>>(P); break;
+        }
     } else {
     switch (c->waves) {
     case 1: k_pass<Real, 1, false, TR><<<c->nblk, 64, c->lds, c->st>>>(P); break;
@@ -286,13 +295,27 @@ void launch_pass(mcgb_ctx* c, const PassArgs& P) {
 }
 
 template <typename Real>
-size_t pass_lds(int waves, int cap) {
+size_t pass_lds_seq(int waves, int cap) {
     switch (waves) {
     case 1: return PassLds<Real, 1>::bytes(cap);
     case 2: return PassLds<Real, 2>::bytes(cap);
     case 8: return PassLds<Real, 8>::bytes(cap);
     default: return PassLds<Real, 4>::bytes(cap);
     }
+}
+template <typename Real>
+size_t pass_lds_spec(int waves, int cap) {
+    switch (waves) {
+    case 1: return SpecLds<Real, 1>::bytes(cap);
+    case 2: return SpecLds<Real, 2>::bytes(cap);
+    case 8: return SpecLds<Real, 8>::bytes(cap);
+    default: return SpecLds<Real, 4>::bytes(cap);
+    }
+}
+// LDS of the larger of the two schedules' staging layouts (both use the same capacity)
+template <typename Real>
+size_t pass_lds(int waves, int cap) {
+    return std::max(pass_lds_seq<Real>(waves, cap), pass_lds_spec<Real>(waves, cap));
 }
 
 // One sweep of every replica.  ev (may be null): 11 events recorded at the sweep start, after
@@ -574,7 +597,9 @@ int mcgb_create(const mcgb_params* p, int n_replicas, mcgb_ctx** out) {
     while (cap > 32 && (c->rsize == 8 ? pass_lds<double>(c->waves, cap) : pass_lds<float>(c->waves, cap)) > 65536)
         cap -= 32;
     c->cap = cap;
-    c->lds = c->rsize == 8 ? pass_lds<double>(c->waves, cap) : pass_lds<float>(c->waves, cap);
+    c->lds = c->rsize == 8 ? pass_lds_seq<double>(c->waves, cap) : pass_lds_seq<float>(c->waves, cap);
+    c->lds_spec = c->rsize == 8 ? pass_lds_spec<double>(c->waves, cap) : pass_lds_spec<float>(c->waves, cap);
+    c->spec = p[0].pass_schedule != 0;
 
     // ---- device memory
     int rc = 0;
@@ -1026,11 +1051,13 @@ static int set_waves(mcgb_ctx* c, int waves) {
     while (cap > 32 && (c->rsize == 8 ? pass_lds<double>(waves, cap) : pass_lds<float>(waves, cap)) > 65536)
         cap -= 32;
     c->cap = cap;
-    c->lds = c->rsize == 8 ? pass_lds<double>(waves, cap) : pass_lds<float>(waves, cap);
+    c->lds = c->rsize == 8 ? pass_lds_seq<double>(waves, cap) : pass_lds_seq<float>(waves, cap);
+    c->lds_spec = c->rsize == 8 ? pass_lds_spec<double>(waves, cap) : pass_lds_spec<float>(waves, cap);
     return MCGB_OK;
 }
 
-int mcgb_autotune(mcgb_ctx* c, int sweeps, int* best_waves, double ms_per_pass[4]) {
+int mcgb_autotune(mcgb_ctx* c, int sweeps, int* best_waves, int* best_schedule,
+                  double ms_per_pass[8]) {
     int rc = check_ctx(c, 0);
     if (rc) return rc;
     if (sweeps <= 0) { mcgb_set_error("mcgb_autotune: sweeps must be > 0"); return MCGB_ERR_ARG; }
@@ -1039,15 +1066,21 @@ int mcgb_autotune(mcgb_ctx* c, int sweeps, int* best_waves, double ms_per_pass[4
     c->timing = true;
     double best = 1e300;
     int bw = c->waves;
-    for (int k = 0; k < 4; ++k) {
-        set_waves(c, cand[k]);
-        if ((rc = sweep_impl(c, sweeps, nullptr))) { c->timing = tm; return rc; }
-        if (ms_per_pass) ms_per_pass[k] = c->ms_pass;
-        if (c->ms_pass < best) { best = c->ms_pass; bw = cand[k]; }
+    bool bs = c->spec;
+    for (int sched = 0; sched < 2; ++sched) {
+        c->spec = sched == 1;
+        for (int k = 0; k < 4; ++k) {
+            set_waves(c, cand[k]);
+            if ((rc = sweep_impl(c, sweeps, nullptr))) { c->timing = tm; return rc; }
+            if (ms_per_pass) ms_per_pass[4 * sched + k] = c->ms_pass;
+            if (c->ms_pass < best) { best = c->ms_pass; bw = cand[k]; bs = c->spec; }
+        }
     }
+    c->spec = bs;
     set_waves(c, bw);
     c->timing = tm;
     if (best_waves) *best_waves = bw;
+    if (best_schedule) *best_schedule = bs ? 1 : 0;
     return MCGB_OK;
 }
 

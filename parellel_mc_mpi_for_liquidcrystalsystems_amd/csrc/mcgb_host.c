@@ -48,6 +48,7 @@ void mcgb_params_default(mcgb_params *p)
     p->lattice_jitter = 0.0;
     p->colloid_exclusion_radius = -1.0;      /* < 0: colloid_radius + 0.5 */
     p->waves_per_cell = 2;
+    p->pass_schedule = 1;
 }
 
 enum { T_INT, T_DOUBLE, T_FLOAT, T_U64 };
@@ -110,6 +111,7 @@ int mcgb_params_load(const char *path, mcgb_params *out)
         {"colloid_exclusion_radius", T_DOUBLE, &p.colloid_exclusion_radius},
         {"stage_capacity", T_INT, &p.stage_capacity},
         {"waves_per_cell", T_INT, &p.waves_per_cell},
+        {"pass_schedule", T_INT, &p.pass_schedule},
     };
     const int nkeys = (int)(sizeof(keys) / sizeof(keys[0]));
     char line[256], name[256], value[256];

@@ -9,6 +9,7 @@
 
 #include "mcgb_device.hpp"
 #include "mcgb_pass.hpp"
+#include "mcgb_spec.hpp"
 #include "mcgb.h"
 
 namespace mcgb {

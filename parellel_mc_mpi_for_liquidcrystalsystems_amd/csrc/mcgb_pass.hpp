@@ -57,19 +57,6 @@ __device__ inline double wave_sum_d63(double v) {
 #undef S_
     return v;
 }
-// compensated (hi, lo) float pair: TwoSum of the hi parts, lo parts added
-__device__ inline void wave_sum_c63(float& hi, float& lo) {
-#define S_(c, m)                         \
-    {                                    \
-        float oh = dpp_f<c, m>(hi);      \
-        float ol = dpp_f<c, m>(lo);      \
-        two_sum(hi, lo, oh);             \
-        lo += ol;                        \
-    }
-    MCGB_DPP_STEPS(S_)
-#undef S_
-}
-
 // ------------------------------------------------------------------ arguments and LDS layout
 
 struct PassArgs {
@@ -154,21 +141,20 @@ __device__ inline bool pair_energy(const GBd& G, const PairGeom& X, double uix, 
     return true;
 }
 
-// Per-lane accumulator of a finite energy difference.  FP32: a lane sums only a few terms per
-// move, so it adds plainly and the cross-lane reduction is compensated (TwoSum per DPP step);
-// FP64: plain.
+// Per-lane accumulator of a finite energy difference: the FP32 pair terms are summed in FP64
+// (each lane adds only a few terms per move; FP64 adds and DPP moves are full rate), reduced
+// across lanes in a fixed DPP order and across wavefronts in a fixed order: deterministic.
 template <typename Real> struct LaneAcc;
 template <> struct LaneAcc<float> {
-    float hi = 0.0f, lo = 0.0f;
-    __device__ void add(float v) { hi += v; }
+    double s = 0.0;
+    __device__ void add(float v) { s += (double)v; }
     __device__ void to_partial(WavePartial& p) {
-        wave_sum_c63(hi, lo);
-        p.hi = hi; p.lo = lo; p.d = 0.0;
+        p.d = wave_sum_d63(s);
+        p.hi = 0.0f; p.lo = 0.0f;
     }
     __device__ static double total(const WavePartial* p, int w) {
-        // fixed order: deterministic
         double s = 0.0;
-        for (int k = 0; k < w; ++k) s += (double)p[k].hi + (double)p[k].lo;
+        for (int k = 0; k < w; ++k) s += p[k].d;
         return s;
     }
 };
@@ -204,6 +190,31 @@ __device__ inline void rotate(double drot, double ux, double uy, double uz, doub
     if (n2 <= 1e-12) { ox = ux; oy = uy; oz = uz; return; }
     const double nr = sqrt(n2);
     ox = tx / nr; oy = ty / nr; oz = tz / nr;
+}
+
+// Candidate pre-filter of a move: sphere around the midpoint of the old and trial positions
+// with radius sqrt(rc2p) + |dq|/2 (slightly enlarged for FP32 rounding), FP32 distances.
+struct MidFilter {
+    uint32_t mx, my, mz;
+    float sx, sy, sz, r2;
+    __device__ inline bool near(uint32_t qx, uint32_t qy, uint32_t qz) const {
+        const float fx = (float)(int)(qx - mx) * sx, fy = (float)(int)(qy - my) * sy,
+                    fz = (float)(int)(qz - mz) * sz;
+        return fx * fx + fy * fy + fz * fz < r2;
+    }
+};
+__device__ inline MidFilter mid_filter(uint32_t qix, uint32_t qiy, uint32_t qiz, int dqx, int dqy,
+                                       int dqz, float sx, float sy, float sz, float rc2p) {
+    MidFilter M;
+    M.mx = qix + (uint32_t)(dqx >> 1);
+    M.my = qiy + (uint32_t)(dqy >> 1);
+    M.mz = qiz + (uint32_t)(dqz >> 1);
+    M.sx = sx; M.sy = sy; M.sz = sz;
+    const float hx = (float)dqx * sx, hy = (float)dqy * sy, hz = (float)dqz * sz;
+    const float rm = (sqrtf(rc2p) + 0.5f * sqrtf(hx * hx + hy * hy + hz * hz)) * (1.0f + 1e-6f) +
+                     2.0f * (sx + sy + sz);
+    M.r2 = rm * rm;
+    return M;
 }
 
 // Flat stencil index e -> global particle slot (segments of the 27 neighbour cells).
@@ -417,20 +428,14 @@ __global__ __launch_bounds__(64 * W) void k_pass(PassArgs P) {
                     int cpl = 0;
                     int base = 0;                        // this wavefront's listed candidates
                     if (staged) {
-                        // candidate test against both states, wavefront-private compaction
+                        // candidates of both states, wavefront-private compaction: a partner
+                        // within the cutoff of either state lies within rc + |dq|/2 of their
+                        // midpoint (triangle inequality), so one FP32 distance per candidate
+                        const MidFilter M = mid_filter(qix, qiy, qiz, dqx, dqy, dqz, sx, sy, sz, rc2p);
                         for (int e0 = w * 64; e0 < S; e0 += 64 * W) {
                             const int e = e0 + lane;
                             bool keep = false;
-                            if (e < S && e != li) {
-                                const int ox = (int)(sqx[e] - qix), oy = (int)(sqy[e] - qiy),
-                                          oz = (int)(sqz[e] - qiz);
-                                const float fx = (float)ox * sx, fy = (float)oy * sy, fz = (float)oz * sz;
-                                const float hx2 = (float)(ox - dqx) * sx, hy2 = (float)(oy - dqy) * sy,
-                                            hz2 = (float)(oz - dqz) * sz;
-                                const float r2o = fx * fx + fy * fy + fz * fz;
-                                const float r2n = hx2 * hx2 + hy2 * hy2 + hz2 * hz2;
-                                keep = (r2o < rc2p) | (r2n < rc2p);
-                            }
+                            if (e < S && e != li) keep = M.near(sqx[e], sqy[e], sqz[e]);
                             const unsigned long long mask = __ballot(keep);
                             if (keep) slist[base + __popcll(mask & __lanemask_lt())] = (unsigned short)e;
                             base += __popcll(mask);

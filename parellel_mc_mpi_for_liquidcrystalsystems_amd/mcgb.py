@@ -45,7 +45,7 @@ class Params(C.Structure):
         ("seed", C.c_uint64), ("precision", C.c_int), ("npt_max_dlnv", C.c_double),
         ("energy_check_interval", C.c_int), ("lattice_jitter", C.c_double),
         ("colloid_exclusion_radius", C.c_double), ("stage_capacity", C.c_int),
-        ("waves_per_cell", C.c_int), ("keys_read", C.c_uint32)]
+        ("waves_per_cell", C.c_int), ("pass_schedule", C.c_int), ("keys_read", C.c_uint32)]
 
 
 class MoveTrace(C.Structure):
@@ -114,7 +114,7 @@ def lib():
         "mcgb_series": (C.c_int, [ctx, C.c_int, C.c_int, dp, dp, dp]),
         "mcgb_series_clear": (C.c_int, [ctx]),
         "mcgb_kernel_times": (C.c_int, [ctx, dp, dp, dp]),
-        "mcgb_autotune": (C.c_int, [ctx, C.c_int, ip, dp]),
+        "mcgb_autotune": (C.c_int, [ctx, C.c_int, ip, ip, dp]),
         "mcgb_cell_list": (C.c_int, [ctx, C.c_int, C.c_uint32, ip, C.c_int, ip, C.c_int, up, up, ip]),
         "mcgb_pair_energy_batch": (C.c_int, [C.POINTER(Params), C.c_int, dp, dp, dp, C.c_int, dp, ip]),
         "mcgb_colloid_energy_batch": (C.c_int, [C.POINTER(Params), C.c_int, dp, dp, C.c_int, dp, ip]),
@@ -370,10 +370,12 @@ class Context:
         check(lib().mcgb_series_clear(self.h))
 
     def autotune(self, sweeps=4):
-        best = C.c_int()
-        ms = np.empty(4)
-        check(lib().mcgb_autotune(self.h, sweeps, C.byref(best), _d(ms)))
-        return best.value, dict(zip((1, 2, 4, 8), ms.tolist()))
+        """Returns (waves_per_cell, schedule, {"seq/W" or "spec/W": ms per pass})."""
+        best, sched = C.c_int(), C.c_int()
+        ms = np.empty(8)
+        check(lib().mcgb_autotune(self.h, sweeps, C.byref(best), C.byref(sched), _d(ms)))
+        keys = [f"{s}/{w}" for s in ("seq", "spec") for w in (1, 2, 4, 8)]
+        return best.value, sched.value, dict(zip(keys, ms.tolist()))
 
     def kernel_times(self):
         a, b, c = C.c_double(), C.c_double(), C.c_double()

@@ -281,12 +281,54 @@ def test_waves_per_cell_do_not_change_fp64_results():
             np.testing.assert_allclose(us[d], states[0][1][d], rtol=0, atol=1e-13)
 
 
+@pytest.mark.parametrize("R,start", [(0.0, "layered"), (3.0, "layered"), (3.0, "lattice")])
+def test_speculative_schedule_matches_oracle_fp64(R, start):
+    # untraced sweeps run the speculative pass (moves of a cell evaluated in parallel against
+    # the fixed part of the stencil, decided in order): same states as the sequential oracle
+    ctx, s, c = make_pair(R=R, precision=64, seed=23, start=start)
+    with ctx:
+        ctx.sweep(4)
+        for _ in range(4):
+            s.sweep()
+        q, u, _ = ctx.get_state_fixed()
+        for d in range(3):
+            assert np.array_equal(q[d], s.q[d])
+            np.testing.assert_allclose(u[d], s.u[d], rtol=0, atol=1e-12)
+        st = ctx.stats()
+        assert st["attempted"] == s.counters[0] and st["accepted"] == s.counters[1]
+        assert st["rejected_left_cell"] == s.counters[2] and st["rejected_hard_core"] == s.counters[3]
+        assert st["candidates"] == s.counters[4] and st["pairs"] >= s.counters[5]
+        assert st["cap_pairs"] == s.energy[0] and st["cap_colloid"] == s.energy[1]
+
+
+def test_pass_schedules_agree():
+    # sequential and speculative schedules: identical FP64 decisions, every wavefront count
+    ref = None
+    for sched in (0, 1):
+        for w in ((2,) if sched == 0 else (1, 2, 4, 8)):
+            q, u, L = layered_system(2500, 0.3, colloid_radius=3.0, seed=44)
+            p = m.gb_params(2500, 0.3, 1.0, colloid_radius=3.0, precision=64, seed=7070,
+                            energy_check_interval=0, waves_per_cell=w, pass_schedule=sched)
+            with m.Context(p) as ctx:
+                ctx.set_state_fixed(q, u, L)
+                ctx.sweep(5)
+                got = ctx.get_state_fixed()
+                acc = ctx.stats()["accepted"]
+            if ref is None:
+                ref = (got, acc)
+                continue
+            assert acc == ref[1], (sched, w)
+            for d in range(3):
+                assert np.array_equal(got[0][d], ref[0][0][d])
+                np.testing.assert_allclose(got[1][d], ref[0][1][d], rtol=0, atol=1e-13)
+
+
 def test_autotune_keeps_results_and_picks_a_supported_width():
     ctx, s, c = make_pair(N=1300, precision=64, seed=17)
     with ctx:
-        best, ms = ctx.autotune(2)
-        assert best in (1, 2, 4, 8) and all(v > 0 for v in ms.values())
-        for _ in range(8):
+        best, sched, ms = ctx.autotune(2)
+        assert best in (1, 2, 4, 8) and sched in (0, 1) and all(v > 0 for v in ms.values())
+        for _ in range(2 * 4 * 2):                   # 2 schedules x 4 widths x 2 sweeps
             s.sweep()
         q, u, _ = ctx.get_state_fixed()
         for d in range(3):

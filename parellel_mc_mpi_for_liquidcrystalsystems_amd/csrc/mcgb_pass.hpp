@@ -57,6 +57,20 @@ __device__ inline double wave_sum_d63(double v) {
 #undef S_
     return v;
 }
+// Wavefront-uniform values read from LDS or computed per lane: held in SGPRs from here on
+// (frees VGPRs for occupancy).
+__device__ inline int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ inline uint32_t uni(uint32_t v) { return (uint32_t)__builtin_amdgcn_readfirstlane((int)v); }
+__device__ inline float uni(float v) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, v)));
+}
+__device__ inline double uni(double v) {
+    const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+    const int lo = __builtin_amdgcn_readfirstlane((int)(u & 0xFFFFFFFFull));
+    const int hi = __builtin_amdgcn_readfirstlane((int)(u >> 32));
+    return __builtin_bit_cast(double, ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
+}
+
 // ------------------------------------------------------------------ arguments and LDS layout
 
 struct PassArgs {
@@ -213,7 +227,7 @@ __device__ inline MidFilter mid_filter(uint32_t qix, uint32_t qiy, uint32_t qiz,
     const float hx = (float)dqx * sx, hy = (float)dqy * sy, hz = (float)dqz * sz;
     const float rm = (sqrtf(rc2p) + 0.5f * sqrtf(hx * hx + hy * hy + hz * hz)) * (1.0f + 1e-6f) +
                      2.0f * (sx + sy + sz);
-    M.r2 = rm * rm;
+    M.r2 = uni(rm * rm);
     return M;
 }
 
@@ -390,19 +404,19 @@ __global__ __launch_bounds__(64 * W) void k_pass(PassArgs P) {
             const int mend = min(k, m0 + KD);
             for (int m = m0; m < mend; ++m) {
                 const int ml = m - m0;
-                const int jsel = (int)(((uint64_t)sj[ml] * (uint32_t)nc) >> 32);
+                const int jsel = uni((int)(((uint64_t)sj[ml] * (uint32_t)nc) >> 32));
                 const int li = coff + jsel;          // flat stencil index of the moved particle
                 const int gsel = cstart + jsel;      // its global slot
                 uint32_t qix, qiy, qiz;
                 Real uix, uiy, uiz;
                 if (staged) {
-                    qix = sqx[li]; qiy = sqy[li]; qiz = sqz[li];
-                    uix = sux[li]; uiy = suy[li]; uiz = suz[li];
+                    qix = uni(sqx[li]); qiy = uni(sqy[li]); qiz = uni(sqz[li]);
+                    uix = uni(sux[li]); uiy = uni(suy[li]); uiz = uni(suz[li]);
                 } else {
                     qix = P.qx[gsel]; qiy = P.qy[gsel]; qiz = P.qz[gsel];
                     uix = Ux[gsel]; uiy = Uy[gsel]; uiz = Uz[gsel];
                 }
-                const int dqx = sdqx[ml], dqy = sdqy[ml], dqz = sdqz[ml];
+                const int dqx = uni(sdqx[ml]), dqy = uni(sdqy[ml]), dqz = uni(sdqz[ml]);
                 const uint32_t qnx = qix + (uint32_t)dqx, qny = qiy + (uint32_t)dqy,
                                qnz = qiz + (uint32_t)dqz;
                 int outcome;
@@ -423,6 +437,7 @@ __global__ __launch_bounds__(64 * W) void k_pass(PassArgs P) {
                 } else {
                     Real unx, uny, unz;
                     rotate(drot, uix, uiy, uiz, svx[ml], svy[ml], svz[ml], unx, uny, unz);
+                    unx = uni(unx); uny = uni(uny); unz = uni(unz);
                     LaneAcc<Real> acc;
                     Real scl = 0;
                     int cpl = 0;

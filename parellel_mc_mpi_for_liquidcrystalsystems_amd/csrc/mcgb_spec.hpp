@@ -297,11 +297,11 @@ __global__ __launch_bounds__(64 * W) void k_pass_spec(PassArgs P) {
             const int mn = min(k - m0, KD);
             // ---------------- phase A: env and colloid terms from the chunk's start state
             for (int ml = w; ml < mn; ml += W) {
-                const int jsel = (int)(((uint64_t)sj[ml] * (uint32_t)nc) >> 32);
+                const int jsel = uni((int)(((uint64_t)sj[ml] * (uint32_t)nc) >> 32));
                 const int li = coff + jsel;
-                const uint32_t qix = sqx[li], qiy = sqy[li], qiz = sqz[li];
-                const Real uix = sux[li], uiy = suy[li], uiz = suz[li];
-                const int dqx = sdqx[ml], dqy = sdqy[ml], dqz = sdqz[ml];
+                const uint32_t qix = uni(sqx[li]), qiy = uni(sqy[li]), qiz = uni(sqz[li]);
+                const Real uix = uni(sux[li]), uiy = uni(suy[li]), uiz = uni(suz[li]);
+                const int dqx = uni(sdqx[ml]), dqy = uni(sdqy[ml]), dqz = uni(sdqz[ml]);
                 double rnx, rny, rnz;
                 const int out = early(qix + (uint32_t)dqx, qiy + (uint32_t)dqy, qiz + (uint32_t)dqz,
                                       rnx, rny, rnz);
@@ -309,6 +309,7 @@ __global__ __launch_bounds__(64 * W) void k_pass_spec(PassArgs P) {
                 if (out == 0) {
                     Real unx, uny, unz;
                     rotate(drot, uix, uiy, uiz, svx[ml], svy[ml], svz[ml], unx, uny, unz);
+                    unx = uni(unx); uny = uni(uny); unz = uni(unz);
                     SpecAcc<Real> A;
                     spec_scan<Real>(G, X, sx, sy, sz, rc2p, sqx, sqy, sqz, sux, suy, suz, ring, S, li,
                                     true, coff, coff + nc, qix, qiy, qiz, dqx, dqy, dqz, uix, uiy,
@@ -327,13 +328,13 @@ __global__ __launch_bounds__(64 * W) void k_pass_spec(PassArgs P) {
             if (w == 0) {
                 int nmv = 0;
                 for (int ml = 0; ml < mn; ++ml) {
-                    const int jsel = (int)(((uint64_t)sj[ml] * (uint32_t)nc) >> 32);
+                    const int jsel = uni((int)(((uint64_t)sj[ml] * (uint32_t)nc) >> 32));
                     const int li = coff + jsel;
                     const int gsel = cstart + jsel;
                     const bool dirty = __ballot(lane < nmv && (int)smv[lane] == jsel) != 0ull;
-                    const uint32_t qix = sqx[li], qiy = sqy[li], qiz = sqz[li];
-                    const Real uix = sux[li], uiy = suy[li], uiz = suz[li];
-                    const int dqx = sdqx[ml], dqy = sdqy[ml], dqz = sdqz[ml];
+                    const uint32_t qix = uni(sqx[li]), qiy = uni(sqy[li]), qiz = uni(sqz[li]);
+                    const Real uix = uni(sux[li]), uiy = uni(suy[li]), uiz = uni(suz[li]);
+                    const int dqx = uni(sdqx[ml]), dqy = uni(sdqy[ml]), dqz = uni(sdqz[ml]);
                     const uint32_t qnx = qix + (uint32_t)dqx, qny = qiy + (uint32_t)dqy,
                                    qnz = qiz + (uint32_t)dqz;
                     ++n_att;
@@ -348,6 +349,7 @@ __global__ __launch_bounds__(64 * W) void k_pass_spec(PassArgs P) {
                     if (out == MCGB_MOVE_HARD) { ++n_hard; continue; }
                     Real unx, uny, unz;
                     rotate(drot, uix, uiy, uiz, svx[ml], svy[ml], svz[ml], unx, uny, unz);
+                    unx = uni(unx); uny = uni(uny); unz = uni(unz);
                     double dpair, cacc;
                     int capd, ccd;
                     SpecAcc<Real> A;

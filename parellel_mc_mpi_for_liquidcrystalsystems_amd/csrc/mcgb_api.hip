@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <cstdlib>
 #include <vector>
 
 #include "mcgb_internal.h"
@@ -526,6 +527,12 @@ int mcgb_create(const mcgb_params* p, int n_replicas, mcgb_ctx** out) {
     HIPCHK(hipGetDevice(&c->device));
     HIPCHK(hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking));
     for (int k = 0; k < 22; ++k) HIPCHK(hipEventCreate(&c->tev[k]));
+    // MCGB_NO_GRAPH=1: launch every sweep kernel by kernel (for profilers and debuggers that
+    // do not follow graph launches); results are identical
+    {
+        const char* ng = getenv("MCGB_NO_GRAPH");
+        if (ng && ng[0] && strcmp(ng, "0") != 0) c->use_graph = false;
+    }
 
     // ---- layout of every replica
     c->hC.resize(n_replicas);

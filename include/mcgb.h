@@ -73,7 +73,7 @@ typedef struct mcgb_params {
     double   lattice_jitter;                /* init: uniform jitter amplitude of sites     */
     double   colloid_exclusion_radius;      /* init: sites closer than this are removed    */
     int      stage_capacity;                /* particles staged in LDS per stencil, 0: auto */
-    int      waves_per_cell;                /* wavefronts per active cell: 1, 2, 4 or 8     */
+    int      waves_per_cell;                /* wavefronts per active cell: 1, 2, 3, 4 or 8  */
     int      pass_schedule;                 /* 1: speculative (moves of a cell evaluated in
                                                parallel, decided in order), 0: sequential  */
 
@@ -220,12 +220,12 @@ int mcgb_series(mcgb_ctx *ctx, int replica, int max,
 int mcgb_series_clear(mcgb_ctx *ctx);
 
 /* Autotuner: times `sweeps` sweeps with each pass schedule (sequential, speculative) and each
- * supported number of wavefronts per active cell (1, 2, 4, 8; the LDS staging capacity
+ * supported number of wavefronts per active cell (1, 2, 3, 4, 8; the LDS staging capacity
  * follows) and keeps the fastest.  The timed sweeps are ordinary sweeps of the simulation (they
  * advance it).  No counterpart in the reference.  ms_per_pass (may be NULL) receives the
- * measured pass times, index 4*schedule + {0,1,2,3} for 1, 2, 4, 8 wavefronts. */
+ * measured pass times, index 5*schedule + {0,1,2,3,4} for 1, 2, 3, 4, 8 wavefronts. */
 int mcgb_autotune(mcgb_ctx *ctx, int sweeps, int *best_waves, int *best_schedule,
-                  double ms_per_pass[8]);
+                  double ms_per_pass[10]);
 
 /* Average device time per launch of the sweep-pass kernel, the cell-list rebuild and the whole
  * sweep, measured with HIP events on the context's stream over the last mcgb_sweep() call. */

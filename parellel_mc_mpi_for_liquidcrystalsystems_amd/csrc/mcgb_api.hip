@@ -263,6 +263,7 @@ void launch_pass_t(mcgb_ctx* c, const PassArgs& P) {
         switch (c->waves) {
         case 1: k_pass<Real, 1, false, false, true><<<c->nblk, 64, c->lds, c->st>>>(P); break;
         case 2: k_pass<Real, 2, false, false, true><<<c->nblk, 128, c->lds, c->st>>>(P); break;
+        case 3: k_pass<Real, 3, false, false, true><<<c->nblk, 192, c->lds, c->st>>>(P); break;
         case 8: k_pass<Real, 8, false, false, true><<<c->nblk, 512, c->lds, c->st>>>(P); break;
         default: k_pass<Real, 4, false, false, true><<<c->nblk, 256, c->lds, c->st>>>(P); break;
         }
@@ -270,6 +271,7 @@ void launch_pass_t(mcgb_ctx* c, const PassArgs& P) {
         switch (c->waves) {
         case 1: k_pass_spec<Real, 1><<<c->nblk, 64, c->lds_spec, c->st>>>(P); break;
         case 2: k_pass_spec<Real, 2><<<c->nblk, 128, c->lds_spec, c->st>>>(P); break;
+        case 3: k_pass_spec<Real, 3><<<c->nblk, 192, c->lds_spec, c->st>>>(P); break;
         case 8: k_pass_spec<Real, 8><<<c->nblk, 512, c->lds_spec, c->st>>>(P); break;
         default: k_pass_spec<Real, 4><<<c->nblk, 256, c->lds_spec, c->st>>>(P); break;
         }
@@ -277,6 +279,7 @@ void launch_pass_t(mcgb_ctx* c, const PassArgs& P) {
     switch (c->waves) {
     case 1: k_pass<Real, 1, false, TR><<<c->nblk, 64, c->lds, c->st>>>(P); break;
     case 2: k_pass<Real, 2, false, TR><<<c->nblk, 128, c->lds, c->st>>>(P); break;
+    case 3: k_pass<Real, 3, false, TR><<<c->nblk, 192, c->lds, c->st>>>(P); break;
     case 8: k_pass<Real, 8, false, TR><<<c->nblk, 512, c->lds, c->st>>>(P); break;
     default: k_pass<Real, 4, false, TR><<<c->nblk, 256, c->lds, c->st>>>(P); break;
     }
@@ -300,6 +303,7 @@ size_t pass_lds_seq(int waves, int cap) {
     switch (waves) {
     case 1: return PassLds<Real, 1>::bytes(cap);
     case 2: return PassLds<Real, 2>::bytes(cap);
+    case 3: return PassLds<Real, 3>::bytes(cap);
     case 8: return PassLds<Real, 8>::bytes(cap);
     default: return PassLds<Real, 4>::bytes(cap);
     }
@@ -309,6 +313,7 @@ size_t pass_lds_spec(int waves, int cap) {
     switch (waves) {
     case 1: return SpecLds<Real, 1>::bytes(cap);
     case 2: return SpecLds<Real, 2>::bytes(cap);
+    case 3: return SpecLds<Real, 3>::bytes(cap);
     case 8: return SpecLds<Real, 8>::bytes(cap);
     default: return SpecLds<Real, 4>::bytes(cap);
     }
@@ -591,8 +596,8 @@ int mcgb_create(const mcgb_params* p, int n_replicas, mcgb_ctx** out) {
     // wavefronts per active cell and LDS staging capacity (1.15x the mean 3x3x3 stencil plus
     // 64, within 64 KiB; a larger stencil takes the HBM path)
     c->waves = p[0].waves_per_cell;
-    if (c->waves != 1 && c->waves != 2 && c->waves != 4 && c->waves != 8) {
-        mcgb_set_error("waves_per_cell must be 1, 2, 4 or 8 (got %d)", c->waves);
+    if (c->waves != 1 && c->waves != 2 && c->waves != 3 && c->waves != 4 && c->waves != 8) {
+        mcgb_set_error("waves_per_cell must be 1, 2, 3, 4 or 8 (got %d)", c->waves);
         mcgb_destroy(c);
         return MCGB_ERR_ARG;
     }
@@ -1064,11 +1069,11 @@ static int set_waves(mcgb_ctx* c, int waves) {
 }
 
 int mcgb_autotune(mcgb_ctx* c, int sweeps, int* best_waves, int* best_schedule,
-                  double ms_per_pass[8]) {
+                  double ms_per_pass[10]) {
     int rc = check_ctx(c, 0);
     if (rc) return rc;
     if (sweeps <= 0) { mcgb_set_error("mcgb_autotune: sweeps must be > 0"); return MCGB_ERR_ARG; }
-    const int cand[4] = {1, 2, 4, 8};
+    const int cand[5] = {1, 2, 3, 4, 8};
     const bool tm = c->timing;
     c->timing = true;
     double best = 1e300;
@@ -1076,10 +1081,10 @@ int mcgb_autotune(mcgb_ctx* c, int sweeps, int* best_waves, int* best_schedule,
     bool bs = c->spec;
     for (int sched = 0; sched < 2; ++sched) {
         c->spec = sched == 1;
-        for (int k = 0; k < 4; ++k) {
+        for (int k = 0; k < 5; ++k) {
             set_waves(c, cand[k]);
             if ((rc = sweep_impl(c, sweeps, nullptr))) { c->timing = tm; return rc; }
-            if (ms_per_pass) ms_per_pass[4 * sched + k] = c->ms_pass;
+            if (ms_per_pass) ms_per_pass[5 * sched + k] = c->ms_pass;
             if (c->ms_pass < best) { best = c->ms_pass; bw = cand[k]; bs = c->spec; }
         }
     }

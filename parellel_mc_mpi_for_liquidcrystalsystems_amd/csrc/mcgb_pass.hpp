@@ -96,8 +96,7 @@ struct PassArgs {
 };
 
 struct WavePartial {
-    float hi, lo;     // FP32 path: compensated finite dE
-    double d;         // FP64 path: finite dE
+    double d;         // finite dE (FP64 lane sums, both precisions)
     int capd;         // change of capped pairs
     int pad;
     double scale;     // sum |terms| (trace only)
@@ -164,7 +163,6 @@ template <> struct LaneAcc<float> {
     __device__ void add(float v) { s += (double)v; }
     __device__ void to_partial(WavePartial& p) {
         p.d = wave_sum_d63(s);
-        p.hi = 0.0f; p.lo = 0.0f;
     }
     __device__ static double total(const WavePartial* p, int w) {
         double s = 0.0;
@@ -177,7 +175,6 @@ template <> struct LaneAcc<double> {
     __device__ void add(double v) { s += v; }
     __device__ void to_partial(WavePartial& p) {
         p.d = wave_sum_d63(s);
-        p.hi = 0.0f; p.lo = 0.0f;
     }
     __device__ static double total(const WavePartial* p, int w) {
         double s = 0.0;
@@ -447,13 +444,21 @@ __global__ __launch_bounds__(64 * W) void k_pass(PassArgs P) {
                         // within the cutoff of either state lies within rc + |dq|/2 of their
                         // midpoint (triangle inequality), so one FP32 distance per candidate
                         const MidFilter M = mid_filter(qix, qiy, qiz, dqx, dqy, dqz, sx, sy, sz, rc2p);
-                        for (int e0 = w * 64; e0 < S; e0 += 64 * W) {
-                            const int e = e0 + lane;
-                            bool keep = false;
-                            if (e < S && e != li) keep = M.near(sqx[e], sqy[e], sqz[e]);
-                            const unsigned long long mask = __ballot(keep);
-                            if (keep) slist[base + __popcll(mask & __lanemask_lt())] = (unsigned short)e;
-                            base += __popcll(mask);
+                        // two candidates per lane per step, branch-free (clamped in-bounds
+                        // reads, validity folded into the predicate): six LDS reads in flight
+                        for (int e0 = w * 64; e0 < S; e0 += 128 * W) {
+                            const int ea = e0 + lane, eb = e0 + 64 * W + lane;
+                            const int ca = min(ea, S - 1), cb = min(eb, S - 1);
+                            const uint32_t xa = sqx[ca], ya = sqy[ca], za = sqz[ca];
+                            const uint32_t xb = sqx[cb], yb = sqy[cb], zb = sqz[cb];
+                            const bool ka = M.near(xa, ya, za) & (ea < S) & (ea != li);
+                            const bool kb = M.near(xb, yb, zb) & (eb < S) & (eb != li);
+                            const unsigned long long ma = __ballot(ka);
+                            const unsigned long long mb = __ballot(kb);
+                            const int na = __popcll(ma);
+                            if (ka) slist[base + __popcll(ma & __lanemask_lt())] = (unsigned short)ea;
+                            if (kb) slist[base + na + __popcll(mb & __lanemask_lt())] = (unsigned short)eb;
+                            base += na + __popcll(mb);
                         }
                         // the list is private to this wavefront: a wavefront-scope fence suffices
                         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");

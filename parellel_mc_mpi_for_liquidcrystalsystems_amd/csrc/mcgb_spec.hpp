@@ -19,7 +19,7 @@ namespace mcgb {
 
 // Phase-A result of one move.
 struct SpecRes {
-    double env;       // finite env dE (FP32 path: compensated pair folded to FP64)
+    double env;       // finite env dE (FP64 lane sums)
     double cacc;      // colloid finite dE
     int capd;         // change of capped env pairs
     int ccd;          // change of capped colloid terms
@@ -111,9 +111,10 @@ __device__ inline void spec_scan(const G& Gc, const PairGeom& X, float sx, float
         head += n;
         wave_sync_lds();
     };
-    auto test = [&](int e) -> bool {
-        if (e >= S || e == li || (skip_cell && e >= c0 && e < c1)) return false;
-        return M.near(sqx[e], sqy[e], sqz[e]);
+    auto test = [&](int e) -> bool {      // branch-free: clamped in-bounds reads
+        const int ce = min(e, S - 1);
+        const bool valid = (e < S) & (e != li) & !(skip_cell & (e >= c0) & (e < c1));
+        return M.near(sqx[ce], sqy[ce], sqz[ce]) & valid;
     };
     auto push = [&](int e, bool keep) {
         const unsigned long long mask = __ballot(keep);

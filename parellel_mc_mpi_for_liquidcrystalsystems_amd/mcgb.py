@@ -372,9 +372,9 @@ class Context:
     def autotune(self, sweeps=4):
         """Returns (waves_per_cell, schedule, {"seq/W" or "spec/W": ms per pass})."""
         best, sched = C.c_int(), C.c_int()
-        ms = np.empty(8)
+        ms = np.empty(10)
         check(lib().mcgb_autotune(self.h, sweeps, C.byref(best), C.byref(sched), _d(ms)))
-        keys = [f"{s}/{w}" for s in ("seq", "spec") for w in (1, 2, 4, 8)]
+        keys = [f"{s}/{w}" for s in ("seq", "spec") for w in (1, 2, 3, 4, 8)]
         return best.value, sched.value, dict(zip(keys, ms.tolist()))
 
     def kernel_times(self):

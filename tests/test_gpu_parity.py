@@ -267,7 +267,7 @@ def test_stencil_beyond_lds_capacity_uses_global_path(precision):
 def test_waves_per_cell_do_not_change_fp64_results():
     # the wavefront split only reorders FP64 sums: identical decisions and states
     states = []
-    for w in (1, 2, 4, 8):
+    for w in (1, 2, 3, 4, 8):
         q, u, L = layered_system(1500, 0.25, colloid_radius=3.0, seed=41)
         p = m.gb_params(1500, 0.25, 1.0, colloid_radius=3.0, precision=64, seed=6060,
                         energy_check_interval=0, waves_per_cell=w)
@@ -305,7 +305,7 @@ def test_pass_schedules_agree():
     # sequential and speculative schedules: identical FP64 decisions, every wavefront count
     ref = None
     for sched in (0, 1):
-        for w in ((2,) if sched == 0 else (1, 2, 4, 8)):
+        for w in ((2, 3) if sched == 0 else (1, 2, 3, 4, 8)):
             q, u, L = layered_system(2500, 0.3, colloid_radius=3.0, seed=44)
             p = m.gb_params(2500, 0.3, 1.0, colloid_radius=3.0, precision=64, seed=7070,
                             energy_check_interval=0, waves_per_cell=w, pass_schedule=sched)
@@ -327,8 +327,8 @@ def test_autotune_keeps_results_and_picks_a_supported_width():
     ctx, s, c = make_pair(N=1300, precision=64, seed=17)
     with ctx:
         best, sched, ms = ctx.autotune(2)
-        assert best in (1, 2, 4, 8) and sched in (0, 1) and all(v > 0 for v in ms.values())
-        for _ in range(2 * 4 * 2):                   # 2 schedules x 4 widths x 2 sweeps
+        assert best in (1, 2, 3, 4, 8) and sched in (0, 1) and all(v > 0 for v in ms.values())
+        for _ in range(2 * 5 * 2):                   # 2 schedules x 5 widths x 2 sweeps
             s.sweep()
         q, u, _ = ctx.get_state_fixed()
         for d in range(3):

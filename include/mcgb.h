@@ -76,6 +76,8 @@ typedef struct mcgb_params {
     int      waves_per_cell;                /* wavefronts per active cell: 1, 2, 3, 4 or 8  */
     int      pass_schedule;                 /* 1: speculative (moves of a cell evaluated in
                                                parallel, decided in order), 0: sequential  */
+    int      adapt_moves;                   /* step-size adaptation every >= adapt_moves
+                                               attempted moves (the reference: 10000), 0 = off */
 
     uint32_t keys_read;                     /* bit i set: required key i was read          */
 } mcgb_params;
@@ -266,6 +268,51 @@ int mcgb_init_lattice(int n, double density, double colloid_radius, double exclu
                       double jitter, uint64_t seed,
                       double *x, double *y, double *z, double *ux, double *uy, double *uz,
                       double box[3]);
+
+/* initial_configuration_type 1 (initialize_random_positions, MCnptColloid.c:2835-2920): random
+ * sequential addition in the box V = N/rho + 4/3 pi R^3 (L = V^(1/3)), rejecting points within
+ * sigma_ee of an earlier particle (minimum image) or inside the colloid; orientations uniform on
+ * the sphere.  Philox(seed) replaces rand(); MCGB_ERR_GEOMETRY after max_tries rejections for
+ * one particle (the reference loops forever).  Host only. */
+int mcgb_init_random(int n, double density, double colloid_radius, double sigma_ee, uint64_t seed,
+                     int max_tries, double *x, double *y, double *z, double *ux, double *uy,
+                     double *uz, double box[3]);
+
+/* initial_configuration_type 2 (initialize_cubic_lattice_positions, MCnptColloid.c:2924-3033):
+ * n_s^3 sites (n_s >= 2, n_s^3 >= N) at (i, j, k) L/n_s filled x-fastest, sites within the
+ * colloid skipped; orientations uniform on the sphere (Philox).  The spacing stays L/n_s (the
+ * reference's reset to sigma_ee, defect D20, would put sites outside the box).  Host only. */
+int mcgb_init_cubic(int n, double density, double colloid_radius, uint64_t seed,
+                    double *x, double *y, double *z, double *ux, double *uy, double *uz,
+                    double box[3]);
+
+/* Trajectory writers, same text layouts as the reference's (append = 0 starts the file):
+ * print_Trayectoria (MCnptColloid.c:3081-3107, "Configuraciones.dat"), print_Qmga (:3110-3157,
+ * "qmga_mcnvtGB.cnf", positions relative to the box centre), print_pos_xyz (:3160-3180,
+ * "DiscosEsfera.xyz": append = 0 writes the two header lines, append = 1 the positions).
+ * Host only. */
+int mcgb_write_trajectory(const char *path, int append, int n, const double *x, const double *y,
+                          const double *z, const double *ux, const double *uy, const double *uz,
+                          const double box[3]);
+int mcgb_write_qmga(const char *path, int append, int n, const double *x, const double *y,
+                    const double *z, const double *ux, const double *uy, const double *uz,
+                    const double box[3]);
+int mcgb_write_xyz(const char *path, int append, int n, const double *x, const double *y,
+                   const double *z, const double box[3]);
+
+/* Step sizes of a replica (float, as the reference's max_displacement / max_rotation globals,
+ * MCnptColloid.c:108-109).  With adapt_moves > 0 the device adapts them after every sweep that
+ * brings the attempted moves since the last adaptation to >= adapt_moves, from the acceptance
+ * of those moves: the controller of MCnptColloid.c:987-1031 (ratio < 0.30: d_max -= 5 %, at
+ * d_max < 1e-4 min_thick reset to 0.01 min_thick and d_rot halved while > 1e-6; ratio > 0.55:
+ * d_max += 5 %, above 1.5 min_thick reset to 0.75 min_thick and d_rot x 1.5 while < 0.7;
+ * min_thick = min(sigma_ee, sigma_ss)).  *adaptations (may be NULL) counts the controller's
+ * actions since mcgb_create.  mcgb_set_step_sizes overrides both and restarts the count of
+ * moves towards the next adaptation; it requires 0 <= d_max < box/2 on every axis and
+ * d_rot >= 0. */
+int mcgb_get_step_sizes(mcgb_ctx *ctx, int replica, float *max_displacement, float *max_rotation,
+                        uint64_t *adaptations);
+int mcgb_set_step_sizes(mcgb_ctx *ctx, int replica, float max_displacement, float max_rotation);
 
 /* Integrated autocorrelation time with Sokal's automatic window (smallest M >= c*tau(M)),
  * tau = 1 + 2 sum_{t=1..M} rho(t).  ESS = n / tau.  Host only. */

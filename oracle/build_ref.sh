@@ -11,6 +11,9 @@
 #   2418-2430        apply_pbc
 #   2605-2721        g_omega, pij_GB, compute_dot_product, compute_distance_sq
 #   2750-2783        get_system_const
+#   2835-3033        initialize_random_positions, initialize_cubic_lattice_positions
+#   3050-3078        pto_ransphr
+#   3081-3180        print_Trayectoria, print_Qmga, print_pos_xyz
 #   3850-4133        error_checker, read_next_parameter, load_initial_conditions
 # mpi.h comes from the MPICH installation of this image (only a type in the globals needs it).
 # Usage: build_ref.sh <path/to/MCnptColloid.c> <mpi include dir> <output .so>
@@ -26,7 +29,7 @@ fi
 mkdir -p "$(dirname "$OUT")"
 {
     sed -n '1,365p;372,378p' "$SRC"
-    sed -n '1741,1854p;1883,2218p;2418,2430p;2605,2721p;2750,2783p;3850,4133p' "$SRC"
+    sed -n '1741,1854p;1883,2218p;2418,2430p;2605,2721p;2750,2783p;2835,3033p;3050,3078p;3081,3180p;3850,4133p' "$SRC"
     cat "$HERE/ref_shim.c"
 } | gcc -std=gnu11 -O2 -fPIC -shared -w -x c - -I"$MPI_INC" -o "$OUT" -lm
 echo "built $OUT"

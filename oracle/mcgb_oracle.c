@@ -647,3 +647,28 @@ void orc_order_parameter(int N, const double *ux, const double *uy, const double
     *p2 = 1.5 * a[im][im];
     director[0] = V[0][im]; director[1] = V[1][im]; director[2] = V[2][im];
 }
+
+/* ------------------------------------------------------------------ step-size controller */
+
+void orc_adapt_steps(float *max_displacement, float *max_rotation, int64_t accepted,
+                     int64_t attempted, float min_thick, double half_box)
+{
+    const float ratio = (float)accepted / (float)attempted;       /* :988 */
+    float d = *max_displacement, r = *max_rotation;
+    if (ratio < 0.3) {                                             /* :999-1014 */
+        d = (float)((double)d - (double)d * 0.05);
+        if ((double)d < 0.0001 * (double)min_thick) {
+            d = (float)(0.01 * (double)min_thick);
+            if ((double)r > 0.000001) r = (float)((double)r - 0.5 * (double)r);
+        }
+    } else if (ratio > 0.55) {                                     /* :1016-1030 */
+        d = (float)((double)d + (double)d * 0.05);
+        if ((double)d > 1.5 * (double)min_thick) {
+            d = (float)(0.75 * (double)min_thick);
+            if ((double)r < 0.7) r = (float)((double)r + 0.5 * (double)r);
+        }
+        if ((double)d >= half_box) return;
+    }
+    *max_displacement = d;
+    *max_rotation = r;
+}

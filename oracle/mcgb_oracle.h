@@ -78,6 +78,13 @@ int    orc_sweep(const orc_consts *c, int N, uint32_t *qx, uint32_t *qy, uint32_
                  double energy[3], int64_t counters[9], orc_move_trace *trace, uint32_t info[13],
                  int resum, int nthreads);
 
+/* Step-size controller of the reference's serial driver (MCnptColloid.c:987-1031), applied to
+ * the acceptance accepted/attempted (a float quotient there) of the moves since its last
+ * action.  The step sizes are floats (:108-109) updated by double expressions.  An increase
+ * that would reach half_box is not made (the reference has no such bound). */
+void   orc_adapt_steps(float *max_displacement, float *max_rotation, int64_t accepted,
+                       int64_t attempted, float min_thick, double half_box);
+
 /* FP64 total energy over the cell list: out = {cap_pairs, cap_colloid, finite, hard_core_count}.
  * Each unordered pair counted once (tot_ene, MCnptColloid.c:1598-1622, counts both orders and
  * halves). */

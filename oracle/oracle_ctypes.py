@@ -63,6 +63,9 @@ def lib():
                                 dp, C.POINTER(C.c_int64), C.POINTER(MoveTrace), up, C.c_int,
                                 C.c_int]
         L.orc_sweep.restype = C.c_int
+        L.orc_adapt_steps.argtypes = [C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_int64,
+                                      C.c_int64, C.c_float, C.c_double]
+        L.orc_adapt_steps.restype = None
         L.orc_total_energy.argtypes = [C.POINTER(OrcConsts), C.c_int, up, up, up, dp, dp, dp, dp,
                                        dp, C.c_int]
         L.orc_total_energy.restype = C.c_int
@@ -162,8 +165,15 @@ class System:
     (3, N) float64, box L (3,), running energy {cap_pairs, cap_colloid, finite}."""
 
     def __init__(self, c, q, u, L, seed=1, replica=0, npt=False, npt_max_dlnv=0.01, nthreads=1,
-                 resum_interval=0):
+                 resum_interval=0, adapt_moves=0):
         self.c = c
+        self.adapt_moves = adapt_moves
+        if adapt_moves:                    # the controller changes the move sizes: own copy
+            self.c = OrcConsts.from_buffer_copy(c)
+        self.dmax = np.float32(self.c.max_displacement)
+        self.drot = np.float32(self.c.max_rotation)
+        self.adapts = 0
+        self._ad = [0, 0]
         self.q = [np.ascontiguousarray(a, dtype=np.uint32).copy() for a in q]
         self.u = [np.ascontiguousarray(a, dtype=np.float64).copy() for a in u]
         self.L = np.ascontiguousarray(L, dtype=np.float64).copy()
@@ -213,6 +223,20 @@ class System:
             raise ValueError("box too small for the cell grid")
         self.t += 1
         self.counters += cnt
+        if self.adapt_moves:
+            self._ad[0] += int(cnt[0])
+            self._ad[1] += int(cnt[1])
+            if self._ad[0] >= self.adapt_moves:
+                d, r = C.c_float(self.dmax), C.c_float(self.drot)
+                s_ee = self.c.sigma_ee
+                min_thick = np.float32(min(s_ee, self.c.sigma0))
+                lib().orc_adapt_steps(C.byref(d), C.byref(r), self._ad[1], self._ad[0],
+                                      float(min_thick), 0.5 * float(self.L.min()))
+                self.dmax, self.drot = np.float32(d.value), np.float32(r.value)
+                self.c.max_displacement = float(self.dmax)
+                self.c.max_rotation = float(self.drot)
+                self._ad = [0, 0]
+                self.adapts += 1
         return (tr, info, cnt)
 
     def order_parameter(self):

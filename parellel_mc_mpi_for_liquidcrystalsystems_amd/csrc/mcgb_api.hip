@@ -157,6 +157,8 @@ void fill_consts(const mcgb_params& p, RepConst* C) {
     C->one_minus_chi = 1.0 - dv[4];
     C->one_minus_chip = 1.0 - dv[7];
     C->npt_max_dlnv = p.npt_max_dlnv;
+    C->adapt_moves = p.adapt_moves;
+    C->min_thick = (float)(dv[8] < p.gb_sigma_sidetoside ? dv[8] : p.gb_sigma_sidetoside);
     C->npt = p.mc_ensemble_type == 2 ? 1 : 0;
     C->vcoll = C->colloid_on ? (4.0 / 3.0) * kPi * p.colloid_radius * p.colloid_radius * p.colloid_radius : 0.0;
     C->f_sigma0 = (float)C->sigma0;
@@ -188,6 +190,10 @@ int check_params(const mcgb_params& p, int r) {
     }
     if (p.max_displacement < 0.0f || p.max_rotation < 0.0f) {
         mcgb_set_error("replica %d: negative move size", r);
+        return MCGB_ERR_ARG;
+    }
+    if (p.adapt_moves < 0) {
+        mcgb_set_error("replica %d: adapt_moves must be >= 0", r);
         return MCGB_ERR_ARG;
     }
     return MCGB_OK;
@@ -592,6 +598,8 @@ int mcgb_create(const mcgb_params* p, int n_replicas, mcgb_ctx** out) {
         RepDyn& Dr = D[r];
         memset(&Dr, 0, sizeof(Dr));
         for (int d = 0; d < 3; ++d) Dr.L[d] = L;
+        Dr.dmax = pr.max_displacement;
+        Dr.drot = pr.max_rotation;
     }
     // wavefronts per active cell and LDS staging capacity (1.15x the mean 3x3x3 stencil plus
     // 64, within 64 KiB; a larger stencil takes the HBM path)
@@ -734,9 +742,20 @@ static int set_state_impl(mcgb_ctx* c, int r, int n, const uint32_t* qx, const u
     RepDyn& Dr = D[r];
     uint32_t t = Dr.t;
     int sc = Dr.series_count;
+    const float dmax = Dr.dmax, drot = Dr.drot;
+    const unsigned long long adapts = Dr.adapts;
+    for (int d = 0; d < 3; ++d)
+        if ((double)dmax >= 0.5 * box[d]) {
+            mcgb_set_error("replica %d: the current max_displacement %g must be below half the box",
+                           r, (double)dmax);
+            return MCGB_ERR_GEOMETRY;
+        }
     memset(&Dr, 0, sizeof(Dr));
     Dr.t = c->state_set[r] ? t : 0u;
     Dr.series_count = sc;
+    Dr.dmax = dmax;
+    Dr.drot = drot;
+    Dr.adapts = adapts;
     for (int d = 0; d < 3; ++d) Dr.L[d] = box[d];
     if ((rc = write_dyn(c, D))) return rc;
     if ((rc = recompute(c, 0))) return rc;
@@ -954,6 +973,38 @@ int mcgb_get_stats(mcgb_ctx* c, int r, mcgb_stats* out) {
     out->da_stage1_rejected = d.da_s1;
     out->da_stage2_rejected = d.da_s2;
     return MCGB_OK;
+}
+
+int mcgb_get_step_sizes(mcgb_ctx* c, int r, float* dmax, float* drot, uint64_t* adaptations) {
+    int rc = check_ctx(c, r);
+    if (rc) return rc;
+    std::vector<RepDyn> D;
+    if ((rc = read_dyn(c, D))) return rc;
+    if (dmax) *dmax = D[r].dmax;
+    if (drot) *drot = D[r].drot;
+    if (adaptations) *adaptations = D[r].adapts;
+    return MCGB_OK;
+}
+
+int mcgb_set_step_sizes(mcgb_ctx* c, int r, float dmax, float drot) {
+    int rc = check_ctx(c, r);
+    if (rc) return rc;
+    std::vector<RepDyn> D;
+    if ((rc = read_dyn(c, D))) return rc;
+    RepDyn& d = D[r];
+    if (!(dmax >= 0.0f) || !(drot >= 0.0f)) {
+        mcgb_set_error("mcgb_set_step_sizes: step sizes must be >= 0");
+        return MCGB_ERR_ARG;
+    }
+    for (int k = 0; k < 3; ++k)
+        if ((double)dmax >= 0.5 * d.L[k]) {
+            mcgb_set_error("mcgb_set_step_sizes: max_displacement must be below half the box");
+            return MCGB_ERR_GEOMETRY;
+        }
+    d.dmax = dmax;
+    d.drot = drot;
+    d.ad_att = d.ad_acc = 0;
+    return write_dyn(c, D);
 }
 
 int mcgb_reset_counters(mcgb_ctx* c, int r) {

@@ -35,6 +35,8 @@ struct RepConst {
     double temperature, beta, pressure, dmax, drot;
     double R, R2, colloid_cut, kappa_prime, one_minus_chi, one_minus_chip;
     double npt_max_dlnv;
+    int adapt_moves;       // step-size adaptation threshold (attempted moves), 0 = off
+    float min_thick;       // min(sigma_ee, sigma_ss) as the reference's float (:833-834)
     double vcoll;          // 4/3 pi R^3 (density convention, MCnptColloid.c:2799-2804)
     // FP32 copies for the pair math
     float f_sigma0, f_four_eps0, f_chi, f_chi2, f_chip, f_chip2, f_mu, f_half_nu, f_rc2, f_drot;
@@ -71,6 +73,9 @@ struct RepDyn {
     double e_res[4];             // last full recompute {cap_pairs, cap_coll, fin, hard}
     unsigned long long resums;   // re-summations of the running energy
     unsigned long long da_s1, da_s2;   // delayed-acceptance rejections, surrogate / exact stage
+    float dmax, drot;            // step sizes (float, as the reference's globals :108-109)
+    unsigned long long ad_att, ad_acc;   // moves since the last step-size adaptation
+    unsigned long long adapts;   // step-size adaptations made
 };
 
 // Draws of one trial move (k_draws): particle selector, fixed-point step, rotation vector,

@@ -49,6 +49,7 @@ void mcgb_params_default(mcgb_params *p)
     p->colloid_exclusion_radius = -1.0;      /* < 0: colloid_radius + 0.5 */
     p->waves_per_cell = 2;
     p->pass_schedule = 1;
+    p->adapt_moves = 0;
 }
 
 enum { T_INT, T_DOUBLE, T_FLOAT, T_U64 };
@@ -112,6 +113,7 @@ int mcgb_params_load(const char *path, mcgb_params *out)
         {"stage_capacity", T_INT, &p.stage_capacity},
         {"waves_per_cell", T_INT, &p.waves_per_cell},
         {"pass_schedule", T_INT, &p.pass_schedule},
+        {"adapt_moves", T_INT, &p.adapt_moves},
     };
     const int nkeys = (int)(sizeof(keys) / sizeof(keys[0]));
     char line[256], name[256], value[256];
@@ -472,5 +474,230 @@ int mcgb_read_fort1(const char *path, int n, double *x, double *y, double *z,
     }
     fclose(f);
     if (nval != 12 * n + 5) { mcgb_set_error("error reading %s", path); return MCGB_ERR_IO; }
+    return MCGB_OK;
+}
+
+/* ------------------------------------------------------------------ trajectory writers */
+
+static FILE *open_for(const char *path, int append)
+{
+    FILE *f = fopen(path, append ? "a" : "w");
+    if (!f) mcgb_set_error("cannot open %s for %s", path, append ? "appending" : "writing");
+    return f;
+}
+
+static int close_checked(FILE *f, const char *path)
+{
+    if (fclose(f) != 0) { mcgb_set_error("error writing %s", path); return MCGB_ERR_IO; }
+    return MCGB_OK;
+}
+
+/* print_Trayectoria, MCnptColloid.c:3081-3107 (same layout, caller's path). */
+int mcgb_write_trajectory(const char *path, int append, int n, const double *x, const double *y,
+                          const double *z, const double *ux, const double *uy, const double *uz,
+                          const double box[3])
+{
+    if (!path || n < 0 || (n > 0 && (!x || !y || !z || !ux || !uy || !uz)) || !box) {
+        mcgb_set_error("mcgb_write_trajectory: bad arguments");
+        return MCGB_ERR_ARG;
+    }
+    FILE *f = open_for(path, append);
+    if (!f) return MCGB_ERR_IO;
+    fprintf(f, "%d \n", n);
+    fprintf(f, "%3.5lf %3.5lf %3.5lf \n", box[0], box[1], box[2]);
+    for (int i = 0; i < n; ++i)
+        fprintf(f, "%5.7lf \t %5.7lf \t %5.7lf \t %5.7lf \t %5.7lf \t %5.7lf \n",
+                x[i], y[i], z[i], ux[i], uy[i], uz[i]);
+    return close_checked(f, path);
+}
+
+/* print_Qmga, MCnptColloid.c:3110-3157: the qmga snapshot layout, positions relative to the
+ * box centre, 12 columns then "0 0"; the reference's format string carries a 36-space gap
+ * between column groups (a line splice inside the literal), reproduced here. */
+int mcgb_write_qmga(const char *path, int append, int n, const double *x, const double *y,
+                    const double *z, const double *ux, const double *uy, const double *uz,
+                    const double box[3])
+{
+    if (!path || n < 0 || (n > 0 && (!x || !y || !z || !ux || !uy || !uz)) || !box) {
+        mcgb_set_error("mcgb_write_qmga: bad arguments");
+        return MCGB_ERR_ARG;
+    }
+    FILE *f = open_for(path, append);
+    if (!f) return MCGB_ERR_IO;
+    fprintf(f, "%d\n", n);
+    fprintf(f, "%lf\n", box[0]);
+    fprintf(f, "%lf\n", box[1]);
+    fprintf(f, "%lf\n", box[2]);
+    fprintf(f, "%lf %lf\n", 0.0, 0.0);
+    static const char gap[] = "                                   ";   /* 35 spaces */
+    for (int i = 0; i < n; ++i) {
+        fprintf(f, "%15.8lf %15.8lf %15.8lf %s", x[i] - 0.5 * box[0], y[i] - 0.5 * box[1],
+                z[i] - 0.5 * box[2], gap);
+        fprintf(f, "%15.8lf %15.8lf %15.8lf %s", 0.0, 0.0, 0.0, gap);
+        fprintf(f, "%15.8lf %15.8lf %15.8lf %s", ux[i], uy[i], uz[i], gap);
+        fprintf(f, "%15.8lf %15.8lf %15.8lf 0 0\n", 0.0, 0.0, 0.0);
+    }
+    return close_checked(f, path);
+}
+
+/* print_pos_xyz, MCnptColloid.c:3160-3180: append = 0 starts the file with the two header
+ * lines only (the reference's option 0); append = 1 adds one "H x y z" line per particle. */
+int mcgb_write_xyz(const char *path, int append, int n, const double *x, const double *y,
+                   const double *z, const double box[3])
+{
+    if (!path || n < 0 || (append && n > 0 && (!x || !y || !z)) || !box) {
+        mcgb_set_error("mcgb_write_xyz: bad arguments");
+        return MCGB_ERR_ARG;
+    }
+    FILE *f = open_for(path, append);
+    if (!f) return MCGB_ERR_IO;
+    if (!append) {
+        fprintf(f, "%10d %15lf %15lf %15lf \n", n, box[0], box[1], box[2]);
+        fprintf(f, "Hola, cara de bola  \n");
+    } else {
+        for (int i = 0; i < n; ++i) fprintf(f, "H %15lf %15lf %15lf \n", x[i], y[i], z[i]);
+    }
+    return close_checked(f, path);
+}
+
+/* ------------------------------------------------------------------ initial configurations */
+
+/* Box of initialize_random_positions / initialize_cubic_lattice_positions
+ * (MCnptColloid.c:2843-2846, :2932-2935): V = N/rho + 4/3 pi R^3, L = V^(1/3), cubic.  pi is
+ * the reference's 14-digit literal (MCnptColloid.c:10), so the box agrees to the last bit. */
+static double init_box(int n, double density, double colloid_radius)
+{
+    const double pi = 3.1415926535898;
+    double vcoll = (4. / 3.) * pi * pow(colloid_radius, 3);
+    return pow((double)n / density + vcoll, 1.0 / 3.0);
+}
+
+static void sphere_point(host_rng *g, double *ux, double *uy, double *uz)
+{
+    const double pi = 3.14159265358979323846;
+    double zz = 1.0 - 2.0 * rng_u01(g), phi = 2.0 * pi * rng_u01(g);
+    double sxy = sqrt(fmax(0.0, 1.0 - zz * zz));
+    *ux = sxy * cos(phi);
+    *uy = sxy * sin(phi);
+    *uz = zz;
+}
+
+/* Minimum-image squared distance (compute_distance_sq, MCnptColloid.c:2690-2721). */
+static double min_image_sq(double ax, double ay, double az, double bx, double by, double bz,
+                           double L)
+{
+    double d[3] = {ax - bx, ay - by, az - bz};
+    double r2 = 0.0;
+    for (int k = 0; k < 3; ++k) {
+        if (d[k] > 0.5 * L) d[k] -= L;
+        else if (d[k] < -0.5 * L) d[k] += L;
+        r2 += d[k] * d[k];
+    }
+    return r2;
+}
+
+/* initial_configuration_type 1: initialize_random_positions (MCnptColloid.c:2835-2920).
+ * Random sequential addition: a uniform point of the box is kept when no earlier particle lies
+ * within sigma_ee (minimum image) and it is not inside the colloid (distance to the centre
+ * >= R); orientations uniform on the sphere.  Philox-seeded (the reference uses rand());
+ * after max_tries rejected points for one particle it stops with MCGB_ERR_GEOMETRY (the
+ * reference loops forever).  A cell grid of width >= sigma_ee replaces the O(N^2) scan. */
+int mcgb_init_random(int n, double density, double colloid_radius, double sigma_ee, uint64_t seed,
+                     int max_tries, double *x, double *y, double *z, double *ux, double *uy,
+                     double *uz, double box[3])
+{
+    if (n <= 0 || density <= 0.0 || sigma_ee < 0.0 || max_tries <= 0 || !x || !y || !z || !ux ||
+        !uy || !uz || !box) {
+        mcgb_set_error("mcgb_init_random: bad arguments");
+        return MCGB_ERR_ARG;
+    }
+    const double L = init_box(n, density, colloid_radius);
+    const double c = 0.5 * L, R2 = colloid_radius * colloid_radius, s2 = sigma_ee * sigma_ee;
+    int nc = sigma_ee > 0.0 ? (int)floor(L / sigma_ee) : 1;
+    if (nc < 3) nc = 1;                          /* small box: every cell is a neighbour */
+    if (nc > 1024) nc = 1024;
+    const size_t ncell = (size_t)nc * nc * nc;
+    int *head = (int *)malloc(sizeof(int) * ncell);
+    int *next = (int *)malloc(sizeof(int) * (size_t)n);
+    if (!head || !next) {
+        free(head); free(next);
+        mcgb_set_error("mcgb_init_random: out of memory");
+        return MCGB_ERR_NOMEM;
+    }
+    for (size_t k = 0; k < ncell; ++k) head[k] = -1;
+    host_rng g = {{(uint32_t)seed, (uint32_t)(seed >> 32)}, 0u, {0, 0, 0, 0}, 4};
+    for (int i = 0; i < n; ++i) {
+        int tries = 0, ok = 0, ci = 0;
+        double px = 0.0, py = 0.0, pz = 0.0;
+        while (!ok) {
+            if (tries++ >= max_tries) {
+                free(head); free(next);
+                mcgb_set_error("mcgb_init_random: no room for particle %d after %d tries "
+                               "(density %g, sigma_ee %g)", i, max_tries, density, sigma_ee);
+                return MCGB_ERR_GEOMETRY;
+            }
+            px = rng_u01(&g) * L; py = rng_u01(&g) * L; pz = rng_u01(&g) * L;
+            ok = min_image_sq(px, py, pz, c, c, c, L) >= R2;
+            int cx = (int)(px / L * nc), cy = (int)(py / L * nc), cz = (int)(pz / L * nc);
+            cx = cx >= nc ? nc - 1 : cx; cy = cy >= nc ? nc - 1 : cy; cz = cz >= nc ? nc - 1 : cz;
+            ci = cx + nc * (cy + nc * cz);
+            const int span = nc == 1 ? 0 : 1;
+            for (int dz = -span; dz <= span && ok; ++dz)
+                for (int dy = -span; dy <= span && ok; ++dy)
+                    for (int dx = -span; dx <= span && ok; ++dx) {
+                        int ex = (cx + dx + nc) % nc, ey = (cy + dy + nc) % nc, ez = (cz + dz + nc) % nc;
+                        for (int j = head[ex + nc * (ey + nc * ez)]; j >= 0 && ok; j = next[j])
+                            if (min_image_sq(px, py, pz, x[j], y[j], z[j], L) < s2) ok = 0;
+                    }
+        }
+        x[i] = px; y[i] = py; z[i] = pz;
+        sphere_point(&g, &ux[i], &uy[i], &uz[i]);
+        next[i] = head[ci];
+        head[ci] = i;
+    }
+    free(head); free(next);
+    box[0] = box[1] = box[2] = L;
+    return MCGB_OK;
+}
+
+/* initial_configuration_type 2: initialize_cubic_lattice_positions (MCnptColloid.c:2924-3033).
+ * n_s = smallest integer >= 2 with n_s^3 >= N; sites (i, j, k) * L/n_s filled x-fastest,
+ * skipping sites within the colloid (distance to the centre <= R when R > 0); orientations
+ * uniform on the sphere (Philox-seeded).  The reference resets the spacing to sigma_ee when
+ * L/n_s < 1.1 sigma_ee (D20), which places sites outside the box; here the spacing stays
+ * L/n_s, so every site is inside.  Running out of sites is MCGB_ERR_GEOMETRY. */
+int mcgb_init_cubic(int n, double density, double colloid_radius, uint64_t seed,
+                    double *x, double *y, double *z, double *ux, double *uy, double *uz,
+                    double box[3])
+{
+    if (n <= 0 || density <= 0.0 || !x || !y || !z || !ux || !uy || !uz || !box) {
+        mcgb_set_error("mcgb_init_cubic: bad arguments");
+        return MCGB_ERR_ARG;
+    }
+    const double L = init_box(n, density, colloid_radius);
+    const double c = 0.5 * L, R2 = colloid_radius * colloid_radius;
+    int ns = 2;
+    while ((long long)ns * ns * ns < n) ns++;
+    const double delta = L / (double)ns;
+    int ix = 0, iy = 0, iz = 0;
+    host_rng g = {{(uint32_t)seed, (uint32_t)(seed >> 32)}, 0u, {0, 0, 0, 0}, 4};
+    for (int i = 0; i < n; ++i) {
+        for (;;) {
+            if (iz >= ns) {
+                mcgb_set_error("mcgb_init_cubic: %d^3 lattice sites outside the colloid hold "
+                               "only %d particles", ns, i);
+                return MCGB_ERR_GEOMETRY;
+            }
+            const double px = (double)ix * delta, py = (double)iy * delta, pz = (double)iz * delta;
+            if (!(R2 > 0.0 && min_image_sq(px, py, pz, c, c, c, L) <= R2)) break;
+            if (++ix == ns) { ix = 0; if (++iy == ns) { iy = 0; ++iz; } }
+        }
+        x[i] = (double)ix * delta;
+        y[i] = (double)iy * delta;
+        z[i] = (double)iz * delta;
+        sphere_point(&g, &ux[i], &uy[i], &uz[i]);
+        if (++ix == ns) { ix = 0; if (++iy == ns) { iy = 0; ++iz; } }
+    }
+    box[0] = box[1] = box[2] = L;
     return MCGB_OK;
 }

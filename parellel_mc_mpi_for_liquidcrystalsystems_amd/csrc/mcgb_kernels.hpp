@@ -36,6 +36,44 @@ __device__ inline T wave_sum(T v) {
 
 // ------------------------------------------------------------------ per-sweep set-up
 
+// Fixed-point step amplitude round(d_max 2^32 / L), held below 2^31 so that step_q's product
+// stays within 63 bits (d_max < L/2 is enforced where the step sizes are set).
+__device__ inline long long step_amplitude(float dmax, double L) {
+    const long long K = (long long)floor((double)dmax * kTwo32 / L + 0.5);
+    return K < 2147483647LL ? K : 2147483647LL;
+}
+
+// Step-size controller of the reference's serial driver (MCnptColloid.c:987-1031) on the
+// acceptance acc/att of the moves since its last action.  The step sizes are floats and the
+// constants doubles there, so each update is a double expression rounded to float; the
+// explicit-rounding builtins keep the compiler from fusing them into an FMA.  An increase
+// that would reach half the smallest box edge is not made (the reference has no such bound;
+// fixed-point steps need it).
+__device__ inline void adapt_steps(float& dmax, float& drot, unsigned long long acc,
+                                   unsigned long long att, float min_thick, double half_box) {
+    const float ratio = (float)acc / (float)att;
+    const double d = (double)dmax, mt = (double)min_thick;
+    if ((double)ratio < 0.3) {
+        float nd = (float)__dsub_rn(d, __dmul_rn(d, 0.05));
+        if ((double)nd < __dmul_rn(0.0001, mt)) {
+            nd = (float)__dmul_rn(0.01, mt);
+            if ((double)drot > 0.000001) drot = (float)__dsub_rn((double)drot, __dmul_rn(0.5, (double)drot));
+        }
+        dmax = nd;
+    } else if ((double)ratio > 0.55) {
+        float nd = (float)__dadd_rn(d, __dmul_rn(d, 0.05));
+        float nr = drot;
+        if ((double)nd > __dmul_rn(1.5, mt)) {
+            nd = (float)__dmul_rn(0.75, mt);
+            if ((double)drot < 0.7) nr = (float)__dadd_rn((double)drot, __dmul_rn(0.5, (double)drot));
+        }
+        if ((double)nd < half_box) {
+            dmax = nd;
+            drot = nr;
+        }
+    }
+}
+
 // Draws of sweep t (grid shift, sublattice order, NPT draws), the cell grid and step sizes.
 __global__ void k_begin(const RepConst* __restrict__ RC, RepDyn* __restrict__ RD, int nrep,
                         int resum_interval) {
@@ -65,7 +103,7 @@ __global__ void k_begin(const RepConst* __restrict__ RC, RepDyn* __restrict__ RD
         D.n[d] = n;
         nc *= n;
         D.s[d] = D.L[d] / kTwo32;
-        D.K[d] = (long long)floor(C.dmax * kTwo32 / D.L[d] + 0.5);
+        D.K[d] = step_amplitude(D.dmax, D.L[d]);
     }
     D.ncell = nc;
     D.nact = nc / 8;
@@ -91,7 +129,7 @@ __global__ void k_grid_g0(const RepConst* __restrict__ RC, RepDyn* __restrict__ 
         D.n[d] = n;
         nc *= n;
         D.s[d] = D.L[d] / kTwo32;
-        D.K[d] = (long long)floor(C.dmax * kTwo32 / D.L[d] + 0.5);
+        D.K[d] = step_amplitude(D.dmax, D.L[d]);
         D.g[d] = 0u;
     }
     D.ncell = nc;
@@ -319,6 +357,16 @@ __global__ __launch_bounds__(256) void k_end_moves(const RepConst* __restrict__ 
         D.e_fin += sd[0][0];
         D.moved_abs = sd[0][1];
         if (sd[0][1] > 1.0e10) D.resum = 1;
+        if (C.adapt_moves > 0) {
+            D.ad_att += (unsigned long long)si[0][0];
+            D.ad_acc += (unsigned long long)si[0][1];
+            if (D.ad_att >= (unsigned long long)C.adapt_moves) {
+                const double half_box = 0.5 * fmin(D.L[0], fmin(D.L[1], D.L[2]));
+                adapt_steps(D.dmax, D.drot, D.ad_acc, D.ad_att, C.min_thick, half_box);
+                D.ad_att = D.ad_acc = 0;
+                D.adapts += 1;
+            }
+        }
     }
 }
 

@@ -306,7 +306,7 @@ __global__ __launch_bounds__(64 * W) void k_pass(PassArgs P) {
     const int part_off = C.part_off;
     const int colloid_on = C.colloid_on;
     const double R2 = C.R2;
-    const double drot = C.drot;
+    const double drot = (double)D.drot;
     const int nx = D.n[0], ny = D.n[1], nz = D.n[2];
     const int hx = nx >> 1, hy = ny >> 1;
     const uint32_t sig = D.order[P.pass];

@@ -5,11 +5,14 @@
  *   mcgb [parametersFile.txt]
  *
  * Reads the reference's 23 keys (plus optional ones), builds the start configuration
- * (initial_configuration_type 0: fortSphere.1 as at MCnptColloid.c:2786-2831; otherwise a
- * seeded lattice), runs total_mc_cycles checkerboard sweeps on the GPU, prints statistics every
- * statistics_print_interval sweeps, writes fortSphere.1.BAK every configuration_save_interval
- * sweeps and fortSphere.1 at the end (:3193-3229).  Every error prints its message and exits
- * with status 1, as the reference does.
+ * (initial_configuration_type 0: fortSphere.1 as at MCnptColloid.c:2786-2831; 1: random
+ * sequential addition, :2835-2920; 2: cubic lattice, :2924-3033; 3: the seeded jittered
+ * lattice of the benchmark configurations), runs total_mc_cycles checkerboard sweeps on the GPU,
+ * prints statistics every statistics_print_interval sweeps, and writes the reference's outputs:
+ * Configuraciones.dat at the start and every configuration_save_interval sweeps together with
+ * fortSphere.1.BAK (:877, :958-960), and qmga_mcnvtGB.cnf, fortSphere.1 and DiscosEsfera.xyz at
+ * the end (:1048-1061).  Every error prints its message and exits with status 1, as the
+ * reference does.
  */
 #include <stdio.h>
 #include <stdlib.h>
@@ -41,15 +44,23 @@ int main(int argc, char **argv)
            *uy = malloc(sizeof(double) * N), *uz = malloc(sizeof(double) * N);
     if (!x || !y || !z || !ux || !uy || !uz) die("out of memory");
     double box[3], R = p.colloid_radius;
-    if (p.initial_configuration_type == 0) {
-        if (mcgb_read_fort1("fortSphere.1", N, x, y, z, ux, uy, uz, box, &R) != MCGB_OK)
-            die("cannot read fortSphere.1");
-    } else {
-        double Rc = p.colloid_wall_option == 1 ? p.colloid_radius : 0.0;
-        if (mcgb_init_lattice(N, p.system_density, Rc, p.colloid_exclusion_radius, p.lattice_jitter,
-                              p.seed, x, y, z, ux, uy, uz, box) != MCGB_OK)
-            die("cannot build the start configuration");
+    const double Rc = p.colloid_wall_option == 1 ? p.colloid_radius : 0.0;
+    int ok = MCGB_OK;
+    switch (p.initial_configuration_type) {
+    case 0: ok = mcgb_read_fort1("fortSphere.1", N, x, y, z, ux, uy, uz, box, &R); break;
+    /* types 1 and 2 size the box with colloid_radius whatever colloid_wall_option says, as the
+     * reference's generators do (MCnptColloid.c:2843, :2932) */
+    case 1: ok = mcgb_init_random(N, p.system_density, p.colloid_radius,
+                                  p.gb_kappa * p.gb_sigma_sidetoside, p.seed, 1000000,
+                                  x, y, z, ux, uy, uz, box); break;
+    case 2: ok = mcgb_init_cubic(N, p.system_density, p.colloid_radius, p.seed,
+                                 x, y, z, ux, uy, uz, box); break;
+    default: ok = mcgb_init_lattice(N, p.system_density, Rc, p.colloid_exclusion_radius,
+                                    p.lattice_jitter, p.seed, x, y, z, ux, uy, uz, box); break;
     }
+    if (ok != MCGB_OK) die("cannot build the start configuration");
+    if (mcgb_write_trajectory("Configuraciones.dat", 0, N, x, y, z, ux, uy, uz, box) != MCGB_OK)
+        die("cannot write Configuraciones.dat");
     mcgb_ctx *ctx;
     if (mcgb_create(&p, 1, &ctx) != MCGB_OK) die("cannot create the GPU context");
     if (mcgb_set_state(ctx, 0, N, x, y, z, ux, uy, uz, box) != MCGB_OK) die("cannot upload the state");
@@ -82,13 +93,20 @@ int main(int argc, char **argv)
                s.attempted ? (double)s.accepted / s.attempted : 0.0, rate);
         if (save > 0 && (done % save == 0 || done == total)) {
             if (mcgb_get_state(ctx, 0, N, x, y, z, ux, uy, uz, box) != MCGB_OK) die("cannot read state");
+            if (mcgb_write_trajectory("Configuraciones.dat", 1, N, x, y, z, ux, uy, uz, box) != MCGB_OK)
+                die("cannot write Configuraciones.dat");
             if (mcgb_write_fort1("fortSphere.1.BAK", N, x, y, z, ux, uy, uz, box, R) != MCGB_OK)
                 die("cannot write fortSphere.1.BAK");
         }
     }
     if (mcgb_get_state(ctx, 0, N, x, y, z, ux, uy, uz, box) != MCGB_OK) die("cannot read state");
+    if (mcgb_write_qmga("qmga_mcnvtGB.cnf", 0, N, x, y, z, ux, uy, uz, box) != MCGB_OK)
+        die("cannot write qmga_mcnvtGB.cnf");
     if (mcgb_write_fort1("fortSphere.1", N, x, y, z, ux, uy, uz, box, R) != MCGB_OK)
         die("cannot write fortSphere.1");
+    if (mcgb_write_xyz("DiscosEsfera.xyz", 0, N, x, y, z, box) != MCGB_OK ||
+        mcgb_write_xyz("DiscosEsfera.xyz", 1, N, x, y, z, box) != MCGB_OK)
+        die("cannot write DiscosEsfera.xyz");
     mcgb_stats s;
     mcgb_get_stats(ctx, 0, &s);
     printf("done: %llu trial moves in %.3f s, acceptance %.4f, max FP64 drift %.3e\n",

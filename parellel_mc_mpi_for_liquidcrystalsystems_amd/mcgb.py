@@ -13,6 +13,7 @@ LIB_PATH = os.path.join(PKG, "libmcgb.so")
 
 ERRORS = {0: "OK", -1: "ARG", -2: "IO", -3: "PARAM", -4: "HIP", -5: "NOMEM", -6: "GEOMETRY",
           -7: "STATE", -8: "CAPACITY"}
+globals().update({"ERR_" + v: k for k, v in ERRORS.items() if k < 0})
 REQUIRED_KEYS = [
     "mc_ensemble_type", "num_particles", "system_density", "system_temperature",
     "system_pressure", "gb_sigma_sidetoside", "gb_epsilon0", "gb_kappa",
@@ -45,7 +46,8 @@ class Params(C.Structure):
         ("seed", C.c_uint64), ("precision", C.c_int), ("npt_max_dlnv", C.c_double),
         ("energy_check_interval", C.c_int), ("lattice_jitter", C.c_double),
         ("colloid_exclusion_radius", C.c_double), ("stage_capacity", C.c_int),
-        ("waves_per_cell", C.c_int), ("pass_schedule", C.c_int), ("keys_read", C.c_uint32)]
+        ("waves_per_cell", C.c_int), ("pass_schedule", C.c_int), ("adapt_moves", C.c_int),
+        ("keys_read", C.c_uint32)]
 
 
 class MoveTrace(C.Structure):
@@ -122,6 +124,15 @@ def lib():
         "mcgb_init_lattice": (C.c_int, [C.c_int, C.c_double, C.c_double, C.c_double, C.c_double,
                                         C.c_uint64, dp, dp, dp, dp, dp, dp, dp]),
         "mcgb_tau_int": (C.c_int, [dp, C.c_int, C.c_double, dp, ip]),
+        "mcgb_get_step_sizes": (C.c_int, [ctx, C.c_int, fp, fp, u64p]),
+        "mcgb_set_step_sizes": (C.c_int, [ctx, C.c_int, C.c_float, C.c_float]),
+        "mcgb_init_random": (C.c_int, [C.c_int, C.c_double, C.c_double, C.c_double, C.c_uint64,
+                                       C.c_int, dp, dp, dp, dp, dp, dp, dp]),
+        "mcgb_init_cubic": (C.c_int, [C.c_int, C.c_double, C.c_double, C.c_uint64,
+                                      dp, dp, dp, dp, dp, dp, dp]),
+        "mcgb_write_trajectory": (C.c_int, [C.c_char_p, C.c_int, C.c_int, dp, dp, dp, dp, dp, dp, dp]),
+        "mcgb_write_qmga": (C.c_int, [C.c_char_p, C.c_int, C.c_int, dp, dp, dp, dp, dp, dp, dp]),
+        "mcgb_write_xyz": (C.c_int, [C.c_char_p, C.c_int, C.c_int, dp, dp, dp, dp]),
         "mcgb_eigen_sym3": (C.c_int, [dp, dp, dp]),
         "mcgb_write_fort1": (C.c_int, [C.c_char_p, C.c_int, dp, dp, dp, dp, dp, dp, dp, C.c_double]),
         "mcgb_read_fort1": (C.c_int, [C.c_char_p, C.c_int, dp, dp, dp, dp, dp, dp, dp, dp]),
@@ -219,6 +230,42 @@ def init_lattice(N, density, colloid_radius=0.0, exclusion_radius=-1.0, jitter=0
     check(lib().mcgb_init_lattice(N, density, colloid_radius, exclusion_radius, jitter, seed,
                                   *(_d(a) for a in arrs), _d(box)))
     return arrs[:3], arrs[3:], box
+
+
+def init_random(N, density, colloid_radius=0.0, sigma_ee=3.0, seed=1, max_tries=1000000):
+    arrs = [np.empty(N) for _ in range(6)]
+    box = np.empty(3)
+    check(lib().mcgb_init_random(N, density, colloid_radius, sigma_ee, seed, max_tries,
+                                 *(_d(a) for a in arrs), _d(box)))
+    return arrs[:3], arrs[3:], box
+
+
+def init_cubic(N, density, colloid_radius=0.0, seed=1):
+    arrs = [np.empty(N) for _ in range(6)]
+    box = np.empty(3)
+    check(lib().mcgb_init_cubic(N, density, colloid_radius, seed, *(_d(a) for a in arrs), _d(box)))
+    return arrs[:3], arrs[3:], box
+
+
+def write_trajectory(path, pos, u, box, append=False):
+    a = [np.ascontiguousarray(v, dtype=np.float64) for v in list(pos) + list(u)]
+    b = np.ascontiguousarray(box, dtype=np.float64)
+    check(lib().mcgb_write_trajectory(os.fsencode(path), 1 if append else 0, a[0].shape[0],
+                                      *(_d(v) for v in a), _d(b)))
+
+
+def write_qmga(path, pos, u, box, append=False):
+    a = [np.ascontiguousarray(v, dtype=np.float64) for v in list(pos) + list(u)]
+    b = np.ascontiguousarray(box, dtype=np.float64)
+    check(lib().mcgb_write_qmga(os.fsencode(path), 1 if append else 0, a[0].shape[0],
+                                *(_d(v) for v in a), _d(b)))
+
+
+def write_xyz(path, pos, box, append=False):
+    a = [np.ascontiguousarray(v, dtype=np.float64) for v in pos]
+    b = np.ascontiguousarray(box, dtype=np.float64)
+    check(lib().mcgb_write_xyz(os.fsencode(path), 1 if append else 0, a[0].shape[0],
+                               *(_d(v) for v in a), _d(b)))
 
 
 def tau_int(x, c=5.0):
@@ -465,3 +512,13 @@ class Context:
         h = np.zeros(DA_BINS, dtype=np.uint64)
         check(lib().mcgb_da_histogram(self.h, replica, h.ctypes.data_as(u64p), 1 if clear else 0))
         return h
+
+    def step_sizes(self, replica=0):
+        """(max_displacement, max_rotation, adaptations) of a replica; float32 values."""
+        d, r, n = C.c_float(), C.c_float(), C.c_uint64()
+        check(lib().mcgb_get_step_sizes(self.h, replica, C.byref(d), C.byref(r), C.byref(n)))
+        return np.float32(d.value), np.float32(r.value), n.value
+
+    def set_step_sizes(self, max_displacement, max_rotation, replica=0):
+        check(lib().mcgb_set_step_sizes(self.h, replica, float(max_displacement),
+                                        float(max_rotation)))

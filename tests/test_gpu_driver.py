@@ -1,0 +1,79 @@
+"""The C driver end to end on the GPU: start configuration, sweeps, the reference's output files
+(Configuraciones.dat at the start and every save interval, fortSphere.1(.BAK), qmga_mcnvtGB.cnf,
+DiscosEsfera.xyz; MCnptColloid.c:877, :958-960, :1048-1061)."""
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+from parellel_mc_mpi_for_liquidcrystalsystems_amd import mcgb as m
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+DRIVER = os.path.join(REPO, "parellel_mc_mpi_for_liquidcrystalsystems_amd", "mcgb")
+GOLD = os.path.join(REPO, "tests", "golden")
+
+pytestmark = pytest.mark.gpu
+
+
+def _params(tmp_path, **over):
+    txt = open(os.path.join(GOLD, "params_cfg0.txt")).read()
+    for k, v in over.items():
+        txt = re.sub(r"(?m)^%s\s+\S+" % k, "%s %s" % (k, v), txt)
+    path = tmp_path / "parametersFile.txt"
+    path.write_text(txt)
+    return path
+
+
+def _frames(path, n):
+    lines = path.read_text().splitlines()
+    assert len(lines) % (n + 2) == 0
+    out = []
+    for f in range(len(lines) // (n + 2)):
+        blk = lines[f * (n + 2):(f + 1) * (n + 2)]
+        assert int(blk[0]) == n
+        box = np.array([float(v) for v in blk[1].split()])
+        data = np.array([[float(v) for v in ln.split()] for ln in blk[2:]])
+        out.append((box, data))
+    return out
+
+
+@pytest.mark.parametrize("ctype", [1, 2])
+def test_driver_runs_and_writes_reference_outputs(tmp_path, ctype):
+    if not os.path.exists(DRIVER):
+        pytest.fail("driver not built")
+    n = 3000
+    path = _params(tmp_path, num_particles=n, system_density=0.02, initial_configuration_type=ctype,
+                   total_mc_cycles=20, configuration_save_interval=10, statistics_print_interval=10)
+    print(f"driver: type {ctype}, N={n}", flush=True)
+    p = subprocess.run([DRIVER, str(path)], capture_output=True, text=True, cwd=tmp_path,
+                       timeout=120)
+    assert p.returncode == 0, p.stderr
+    frames = _frames(tmp_path / "Configuraciones.dat", n)
+    assert len(frames) == 3                                # start, sweep 10, sweep 20
+    seed = m.load_params(str(path)).seed
+    if ctype == 2:
+        pos, u, box = m.init_cubic(n, 0.02, 0.0, seed=seed)
+    else:
+        pos, u, box = m.init_random(n, 0.02, 0.0, 3.0, seed=seed)
+    assert np.allclose(frames[0][1][:, 0], pos[0], atol=1e-7)
+    L = frames[0][0][0]
+    for box_f, d in frames[1:]:
+        assert np.allclose(box_f, L, atol=1e-5)
+        assert d[:, :3].min() >= 0.0 and d[:, :3].max() <= L + 1e-6
+        assert np.allclose(np.linalg.norm(d[:, 3:], axis=1), 1.0, atol=1e-6)
+    assert not np.allclose(frames[2][1], frames[0][1])     # the sweeps moved particles
+    q = (tmp_path / "qmga_mcnvtGB.cnf").read_text().splitlines()
+    assert int(q[0]) == n and len(q) == 5 + n and q[5].endswith(" 0 0")
+    qpos = np.array([[float(v) for v in ln.split()[:3]] for ln in q[5:]])
+    assert np.allclose(qpos + 0.5 * L, frames[2][1][:, :3], atol=2e-6)
+    xyz = (tmp_path / "DiscosEsfera.xyz").read_text().splitlines()
+    assert len(xyz) == 2 + n and xyz[1] == "Hola, cara de bola  " and xyz[2].startswith("H ")
+    for name in ("fortSphere.1", "fortSphere.1.BAK"):
+        assert (tmp_path / name).exists()
+    arrs = [np.empty(n) for _ in range(6)]
+    b2, R = np.empty(3), np.empty(1)
+    m.check(m.lib().mcgb_read_fort1(str(tmp_path / "fortSphere.1").encode(), n,
+                                    *(m._d(a) for a in arrs), m._d(b2), m._d(R)))
+    assert np.allclose(arrs[0], frames[2][1][:, 0], atol=2e-6)

@@ -163,3 +163,42 @@ def test_driver_reports_missing_parameter_file(tmp_path):
         pytest.skip("driver not built")
     p = subprocess.run([drv, str(tmp_path / "nope.txt")], capture_output=True, text=True)
     assert p.returncode == 1 and "Error opening the parameters file" in p.stderr
+
+
+def _driver_params(tmp_path, **over):
+    txt = open(os.path.join(GOLD, "params_cfg0.txt")).read()
+    for k, v in over.items():
+        txt = re.sub(r"(?m)^%s\s+\S+" % k, "%s %s" % (k, v), txt)
+    path = tmp_path / "parametersFile.txt"
+    path.write_text(txt)
+    return path
+
+
+def test_driver_cubic_start_writes_trajectory_before_the_gpu(tmp_path):
+    """initial_configuration_type 2 in the driver: the first Configuraciones.dat frame is the
+    cubic start of mcgb_init_cubic (written before the GPU context exists, as the reference
+    writes it before its first sweep, MCnptColloid.c:877)."""
+    drv = os.path.join(REPO, "parellel_mc_mpi_for_liquidcrystalsystems_amd", "mcgb")
+    if not os.path.exists(drv):
+        pytest.skip("driver not built")
+    if m.device_count() > 0:
+        pytest.skip("a GPU is visible (the GPU suite runs the driver end to end)")
+    path = _driver_params(tmp_path, num_particles=1000, system_density=0.02,
+                          initial_configuration_type=2, total_mc_cycles=0)
+    p = subprocess.run([drv, str(path)], capture_output=True, text=True, cwd=tmp_path)
+    assert p.returncode == 1 and "cannot create the GPU context" in p.stderr
+    pos, u, box = m.init_cubic(1000, 0.02, 0.0, seed=m.load_params(str(path)).seed)
+    want = tmp_path / "want.dat"
+    m.write_trajectory(want, pos, u, box)
+    assert (tmp_path / "Configuraciones.dat").read_bytes() == want.read_bytes()
+
+
+def test_driver_random_start_reports_jammed_density(tmp_path):
+    """Type 1 at rho* = 0.3 with sigma_ee = 3 (packing fraction 4.2) cannot be built by random
+    sequential addition; the reference loops forever there, the driver stops with an error."""
+    drv = os.path.join(REPO, "parellel_mc_mpi_for_liquidcrystalsystems_amd", "mcgb")
+    if not os.path.exists(drv):
+        pytest.skip("driver not built")
+    path = _driver_params(tmp_path, num_particles=2000, initial_configuration_type=1)
+    p = subprocess.run([drv, str(path)], capture_output=True, text=True, cwd=tmp_path, timeout=120)
+    assert p.returncode == 1 and "no room for particle" in p.stderr

@@ -300,6 +300,22 @@ int mcgb_write_qmga(const char *path, int append, int n, const double *x, const 
 int mcgb_write_xyz(const char *path, int append, int n, const double *x, const double *y,
                    const double *z, const double box[3]);
 
+/* Asynchronous sweeps and snapshots (no counterpart in the reference, whose output is
+ * synchronous: print_Trayectoria / backup_restart_file at the save interval,
+ * MCnptColloid.c:958-960).  mcgb_sweep_async enqueues n sweeps of every replica on the
+ * context's stream and returns at once; mcgb_sync waits for them and reports what mcgb_sweep
+ * would (a non-finite running energy).  Every other entry point waits for enqueued work first.
+ * mcgb_snapshot_begin enqueues, after the work already enqueued, a device-side conversion of
+ * replica r's configuration to FP64 in particle order and its copy into pinned host memory;
+ * mcgb_snapshot_wait waits for that copy only (sweeps enqueued after the snapshot keep
+ * running) and returns the same arrays as mcgb_get_state would have at that point.  One
+ * snapshot per replica is in flight at a time; a second begin replaces the first. */
+int mcgb_sweep_async(mcgb_ctx *ctx, int n_sweeps);
+int mcgb_sync(mcgb_ctx *ctx);
+int mcgb_snapshot_begin(mcgb_ctx *ctx, int replica);
+int mcgb_snapshot_wait(mcgb_ctx *ctx, int replica, int n, double *x, double *y, double *z,
+                       double *ux, double *uy, double *uz, double box[3]);
+
 /* Step sizes of a replica (float, as the reference's max_displacement / max_rotation globals,
  * MCnptColloid.c:108-109).  With adapt_moves > 0 the device adapts them after every sweep that
  * brings the attempted moves since the last adaptation to >= adapt_moves, from the acceptance

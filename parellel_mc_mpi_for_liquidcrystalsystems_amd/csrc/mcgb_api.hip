@@ -101,6 +101,11 @@ struct mcgb_ctx {
     float* tr_scratch = nullptr;
     double *tr_lpart = nullptr, *tr_loss = nullptr;
     int tr_nblk = 0, adam_t = 0;
+    // asynchronous snapshots: one staging record and one pinned host record per replica
+    double* snap_dev = nullptr;                // [6 Ntot + 3 nrep]
+    std::vector<double*> snap_host;            // hipHostMalloc, 6 N + 3 doubles each
+    std::vector<hipEvent_t> snap_ev;
+    std::vector<int> snap_pending;
     bool use_graph = true;
     bool timing = true;
     double ms_pass = 0, ms_rebuild = 0, ms_sweep = 0;
@@ -379,6 +384,7 @@ int build_graph(mcgb_ctx* c) {
     const int par = c->cur;
     if (c->graph[par]) return MCGB_OK;
     hipGraph_t g;
+    HIPCHK(hipStreamSynchronize(c->st));      // capture starts on an idle stream
     HIPCHK(hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
     int rc = enqueue_one<Real>(c, nullptr, nullptr);
     if (!rc) rc = enqueue_one<Real>(c, nullptr, nullptr);
@@ -399,9 +405,9 @@ int build_graph(mcgb_ctx* c) {
 }
 
 template <typename Real>
-int enqueue_sweeps(mcgb_ctx* c, int n_sweeps, mcgb_move_trace* trace) {
+int enqueue_sweeps(mcgb_ctx* c, int n_sweeps, mcgb_move_trace* trace, bool async = false) {
     int rc;
-    const bool tm = c->timing;
+    const bool tm = c->timing && !async;
     int timed_sets = 0;   // event sets (of 11) holding the last sweeps, in c->tev
     if (n_sweeps <= 0) return MCGB_OK;
     // graph replay for the bulk; with timing on, the last two sweeps are launched one kernel
@@ -423,6 +429,7 @@ int enqueue_sweeps(mcgb_ctx* c, int n_sweeps, mcgb_move_trace* trace) {
     for (int s = 0; s < tail; ++s)
         if ((rc = enqueue_one<Real>(c, trace, c->tev + 11 * s))) return rc;
     timed_sets = tail;
+    if (async) return MCGB_OK;
     HIPCHK(hipStreamSynchronize(c->st));
     if (timed_sets > 0) {
         double tp = 0, tr = 0, tsw = 0;
@@ -499,6 +506,10 @@ void mcgb_destroy(mcgb_ctx* c) {
         if (c->graph[k]) (void)hipGraphExecDestroy(c->graph[k]);
     for (int k = 0; k < 22; ++k)
         if (c->tev[k]) (void)hipEventDestroy(c->tev[k]);
+    for (double* h : c->snap_host)
+        if (h) (void)hipHostFree(h);
+    for (hipEvent_t e : c->snap_ev)
+        if (e) (void)hipEventDestroy(e);
     for (void* p : c->allocs) (void)hipFree(p);
     if (c->st) (void)hipStreamDestroy(c->st);
     delete c;
@@ -885,6 +896,104 @@ static int sweep_impl(mcgb_ctx* c, int n_sweeps, mcgb_move_trace* trace) {
             return MCGB_ERR_STATE;
         }
     }
+    return MCGB_OK;
+}
+
+int mcgb_sweep_async(mcgb_ctx* c, int n_sweeps) {
+    int rc = check_ctx(c, 0);
+    if (rc) return rc;
+    if (n_sweeps < 0) {
+        mcgb_set_error("n_sweeps must be >= 0");
+        return MCGB_ERR_ARG;
+    }
+    for (int r = 0; r < c->nrep; ++r)
+        if (!c->state_set[r]) {
+            mcgb_set_error("replica %d has no configuration (call mcgb_set_state first)", r);
+            return MCGB_ERR_STATE;
+        }
+    return c->prec == 64 ? enqueue_sweeps<double>(c, n_sweeps, nullptr, true)
+                         : enqueue_sweeps<float>(c, n_sweeps, nullptr, true);
+}
+
+int mcgb_sync(mcgb_ctx* c) {
+    int rc = check_ctx(c, 0);
+    if (rc) return rc;
+    std::vector<RepDyn> D;
+    if ((rc = read_dyn(c, D))) return rc;
+    for (int r = 0; r < c->nrep; ++r) {
+        double e = (double)D[r].cap_pairs * 1e9 + (double)D[r].cap_coll * 1e8 + D[r].e_fin;
+        if (!std::isfinite(e)) {
+            mcgb_set_error("replica %d: running energy is not finite", r);
+            return MCGB_ERR_STATE;
+        }
+    }
+    return MCGB_OK;
+}
+
+int mcgb_snapshot_begin(mcgb_ctx* c, int r) {
+    int rc = check_ctx(c, r);
+    if (rc) return rc;
+    if (!c->state_set[r]) {
+        mcgb_set_error("replica %d has no configuration", r);
+        return MCGB_ERR_STATE;
+    }
+    if (c->snap_host.empty()) {
+        c->snap_host.assign(c->nrep, nullptr);
+        c->snap_ev.assign(c->nrep, nullptr);
+        c->snap_pending.assign(c->nrep, 0);
+    }
+    const RepConst& C = c->hC[r];
+    const size_t rec = 6 * (size_t)C.N + 3;
+    if (!c->snap_dev && (rc = dalloc_t(c, &c->snap_dev, 6 * (size_t)c->Ntot + 3 * (size_t)c->nrep)))
+        return rc;
+    if (!c->snap_ev[r]) HIPCHK(hipEventCreateWithFlags(&c->snap_ev[r], hipEventDisableTiming));
+    if (!c->snap_host[r]) {
+        hipError_t e = hipHostMalloc((void**)&c->snap_host[r], rec * sizeof(double), hipHostMallocDefault);
+        if (e != hipSuccess) {
+            c->snap_host[r] = nullptr;
+            mcgb_set_error("hipHostMalloc(%zu bytes) failed: %s", rec * sizeof(double), hipGetErrorString(e));
+            return MCGB_ERR_NOMEM;
+        }
+    }
+    double* stage = c->snap_dev + 6 * (size_t)C.part_off + 3 * (size_t)r;
+    const int b = c->cur;
+    const dim3 grid((C.N + 255) / 256);
+    if (c->prec == 64)
+        k_snapshot<double><<<grid, 256, 0, c->st>>>(
+            c->dC, c->dD, r, c->q[b][0], c->q[b][1], c->q[b][2], (const double*)c->u[b][0],
+            (const double*)c->u[b][1], (const double*)c->u[b][2], c->pid[b], stage);
+    else
+        k_snapshot<float><<<grid, 256, 0, c->st>>>(
+            c->dC, c->dD, r, c->q[b][0], c->q[b][1], c->q[b][2], (const float*)c->u[b][0],
+            (const float*)c->u[b][1], (const float*)c->u[b][2], c->pid[b], stage);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(c->snap_host[r], stage, rec * sizeof(double), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipEventRecord(c->snap_ev[r], c->st));
+    c->snap_pending[r] = 1;
+    return MCGB_OK;
+}
+
+int mcgb_snapshot_wait(mcgb_ctx* c, int r, int n, double* x, double* y, double* z, double* ux,
+                       double* uy, double* uz, double box[3]) {
+    int rc = check_ctx(c, r);
+    if (rc) return rc;
+    if (c->snap_pending.empty() || !c->snap_pending[r]) {
+        mcgb_set_error("replica %d: no snapshot in flight (call mcgb_snapshot_begin first)", r);
+        return MCGB_ERR_STATE;
+    }
+    const int N = c->hC[r].N;
+    if (n != N) {
+        mcgb_set_error("replica %d holds %d particles, %d requested", r, N, n);
+        return MCGB_ERR_ARG;
+    }
+    HIPCHK(hipEventSynchronize(c->snap_ev[r]));
+    c->snap_pending[r] = 0;
+    const double* h = c->snap_host[r];
+    double* dst[6] = {x, y, z, ux, uy, uz};
+    for (int k = 0; k < 6; ++k)
+        if (dst[k]) memcpy(dst[k], h + (size_t)k * N, (size_t)N * sizeof(double));
+    if (box)
+        for (int d = 0; d < 3; ++d) box[d] = h[6 * (size_t)N + d];
     return MCGB_OK;
 }
 

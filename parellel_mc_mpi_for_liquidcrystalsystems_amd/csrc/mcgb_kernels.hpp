@@ -666,6 +666,39 @@ __global__ void k_observe_reduce(const RepConst* __restrict__ RC, RepDyn* __rest
     if (advance) D.t += 1;
 }
 
+// ------------------------------------------------------------------ snapshots
+
+// Configuration of replica r in particle order, FP64, into a staging record
+// {x[N], y[N], z[N], ux[N], uy[N], uz[N], L[3]} (the host conversion of mcgb_get_state, done
+// where the data is: x = q L / 2^32, the same product as the host's, so the bits agree).
+template <typename Real>
+__global__ __launch_bounds__(256) void k_snapshot(const RepConst* __restrict__ RC,
+                                                  const RepDyn* __restrict__ RD, int r,
+                                                  const uint32_t* __restrict__ qx,
+                                                  const uint32_t* __restrict__ qy,
+                                                  const uint32_t* __restrict__ qz,
+                                                  const Real* __restrict__ ux,
+                                                  const Real* __restrict__ uy,
+                                                  const Real* __restrict__ uz,
+                                                  const int* __restrict__ pid,
+                                                  double* __restrict__ out) {
+    const RepConst& C = RC[r];
+    const RepDyn& D = RD[r];
+    const int N = C.N;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < 3) out[6 * (size_t)N + i] = D.L[i];
+    if (i >= N) return;
+    const int s = C.part_off + i;
+    const int p = pid[s];
+    if ((unsigned)p >= (unsigned)N) return;
+    out[p] = (double)qx[s] * (D.L[0] / kTwo32);
+    out[N + p] = (double)qy[s] * (D.L[1] / kTwo32);
+    out[2 * (size_t)N + p] = (double)qz[s] * (D.L[2] / kTwo32);
+    out[3 * (size_t)N + p] = (double)ux[s];
+    out[4 * (size_t)N + p] = (double)uy[s];
+    out[5 * (size_t)N + p] = (double)uz[s];
+}
+
 // ------------------------------------------------------------------ batch kernels (parity hooks)
 
 __global__ void k_pair_batch(RepConst C, int n, int prec, const double* __restrict__ ei,

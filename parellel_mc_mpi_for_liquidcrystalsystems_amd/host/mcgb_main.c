@@ -33,6 +33,19 @@ static double now_s(void)
     return ts.tv_sec + 1e-9 * ts.tv_nsec;
 }
 
+/* Save point (MCnptColloid.c:958-960): the snapshot begun after the save sweep, appended to
+ * Configuraciones.dat and written to fortSphere.1.BAK. */
+static void save_point(mcgb_ctx *ctx, int N, double *x, double *y, double *z, double *ux,
+                       double *uy, double *uz, double *box, double R)
+{
+    if (mcgb_snapshot_wait(ctx, 0, N, x, y, z, ux, uy, uz, box) != MCGB_OK)
+        die("cannot read the snapshot");
+    if (mcgb_write_trajectory("Configuraciones.dat", 1, N, x, y, z, ux, uy, uz, box) != MCGB_OK)
+        die("cannot write Configuraciones.dat");
+    if (mcgb_write_fort1("fortSphere.1.BAK", N, x, y, z, ux, uy, uz, box, R) != MCGB_OK)
+        die("cannot write fortSphere.1.BAK");
+}
+
 int main(int argc, char **argv)
 {
     const char *path = argc > 1 ? argv[1] : "parametersFile.txt";
@@ -77,9 +90,15 @@ int main(int argc, char **argv)
     double t0 = now_s();
     uint64_t last_att = 0;
     double last_t = t0;
+    int snap = 0;   /* a save-point snapshot is in flight: written while the next sweeps run */
     for (int done = 0; done < total;) {
         int chunk = every < total - done ? every : total - done;
-        if (mcgb_sweep(ctx, chunk) != MCGB_OK) die("sweep failed");
+        if (mcgb_sweep_async(ctx, chunk) != MCGB_OK) die("sweep failed");
+        if (snap) {
+            save_point(ctx, N, x, y, z, ux, uy, uz, box, R);
+            snap = 0;
+        }
+        if (mcgb_sync(ctx) != MCGB_OK) die("sweep failed");
         done += chunk;
         mcgb_stats s;
         double p2, dir[3];
@@ -92,13 +111,11 @@ int main(int argc, char **argv)
         printf("%10d %16.8f %12.6f %10.6f %10.4f %10.3e\n", done, s.energy / N, p2, s.density,
                s.attempted ? (double)s.accepted / s.attempted : 0.0, rate);
         if (save > 0 && (done % save == 0 || done == total)) {
-            if (mcgb_get_state(ctx, 0, N, x, y, z, ux, uy, uz, box) != MCGB_OK) die("cannot read state");
-            if (mcgb_write_trajectory("Configuraciones.dat", 1, N, x, y, z, ux, uy, uz, box) != MCGB_OK)
-                die("cannot write Configuraciones.dat");
-            if (mcgb_write_fort1("fortSphere.1.BAK", N, x, y, z, ux, uy, uz, box, R) != MCGB_OK)
-                die("cannot write fortSphere.1.BAK");
+            if (mcgb_snapshot_begin(ctx, 0) != MCGB_OK) die("cannot take a snapshot");
+            snap = 1;
         }
     }
+    if (snap) save_point(ctx, N, x, y, z, ux, uy, uz, box, R);
     if (mcgb_get_state(ctx, 0, N, x, y, z, ux, uy, uz, box) != MCGB_OK) die("cannot read state");
     if (mcgb_write_qmga("qmga_mcnvtGB.cnf", 0, N, x, y, z, ux, uy, uz, box) != MCGB_OK)
         die("cannot write qmga_mcnvtGB.cnf");

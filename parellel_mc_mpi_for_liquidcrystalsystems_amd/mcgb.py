@@ -125,6 +125,10 @@ def lib():
                                         C.c_uint64, dp, dp, dp, dp, dp, dp, dp]),
         "mcgb_tau_int": (C.c_int, [dp, C.c_int, C.c_double, dp, ip]),
         "mcgb_get_step_sizes": (C.c_int, [ctx, C.c_int, fp, fp, u64p]),
+        "mcgb_sweep_async": (C.c_int, [ctx, C.c_int]),
+        "mcgb_sync": (C.c_int, [ctx]),
+        "mcgb_snapshot_begin": (C.c_int, [ctx, C.c_int]),
+        "mcgb_snapshot_wait": (C.c_int, [ctx, C.c_int, C.c_int, dp, dp, dp, dp, dp, dp, dp]),
         "mcgb_set_step_sizes": (C.c_int, [ctx, C.c_int, C.c_float, C.c_float]),
         "mcgb_init_random": (C.c_int, [C.c_int, C.c_double, C.c_double, C.c_double, C.c_uint64,
                                        C.c_int, dp, dp, dp, dp, dp, dp, dp]),
@@ -512,6 +516,23 @@ class Context:
         h = np.zeros(DA_BINS, dtype=np.uint64)
         check(lib().mcgb_da_histogram(self.h, replica, h.ctypes.data_as(u64p), 1 if clear else 0))
         return h
+
+    def sweep_async(self, n=1):
+        check(lib().mcgb_sweep_async(self.h, n))
+
+    def sync(self):
+        check(lib().mcgb_sync(self.h))
+
+    def snapshot_begin(self, replica=0):
+        check(lib().mcgb_snapshot_begin(self.h, replica))
+
+    def snapshot_wait(self, replica=0):
+        """(pos, u, box) of the snapshot begun last on this replica, FP64, particle order."""
+        N = self.params[replica].num_particles
+        arrs = [np.empty(N) for _ in range(6)]
+        box = np.empty(3)
+        check(lib().mcgb_snapshot_wait(self.h, replica, N, *(_d(a) for a in arrs), _d(box)))
+        return arrs[:3], arrs[3:], box
 
     def step_sizes(self, replica=0):
         """(max_displacement, max_rotation, adaptations) of a replica; float32 values."""

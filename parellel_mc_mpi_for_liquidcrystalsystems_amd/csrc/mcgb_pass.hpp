@@ -12,6 +12,8 @@
 // Two workgroup barriers per move.  Stencils beyond the staging capacity read HBM directly.
 #pragma once
 
+#include <type_traits>
+
 #include "mcgb_device.hpp"
 #include "mcgb_surrogate.hpp"
 #include "mcgb.h"
@@ -108,11 +110,22 @@ struct PassLds {
     __host__ __device__ static inline int list_cap(int cap) {
         return ((cap + 64 * W - 1) / (64 * W)) * 64;
     }
+    // staged positions: the capacity rounded up to whole wavefronts, the tail filled with a
+    // far-away sentinel so that the scan needs no bounds test
+    __host__ __device__ static inline int stage_cap(int cap) { return ((cap + 63) / 64) * 64; }
+    // per-wavefront candidate list: list_cap entries and a slot that unselected lanes write
+    __host__ __device__ static inline int list_stride(int cap) { return list_cap(cap) + 8; }
+    // byte offsets: stencil tables 0, positions 256, orientations, lists, draws (16-aligned);
+    // 12 * stage_cap is a multiple of 768, so every region keeps its alignment
+    __host__ __device__ static inline int off_su(int cap) { return 256 + 12 * stage_cap(cap); }
+    __host__ __device__ static inline int off_list(int cap) {
+        return off_su(cap) + cap * 3 * (int)sizeof(Real);
+    }
+    __host__ __device__ static inline int off_draws(int cap) {
+        return (off_list(cap) + W * list_stride(cap) * 2 + 15) & ~15;
+    }
     __host__ __device__ static inline size_t bytes(int cap) {
-        size_t b = 256;                                   // stencil tables
-        b += (size_t)cap * (12 + 3 * sizeof(Real));       // staged SoA
-        b += (size_t)W * list_cap(cap) * 2;               // wavefront lists (ushort)
-        b = (b + 15) & ~(size_t)15;
+        size_t b = (size_t)off_draws(cap);
         b += kDraws * (8 + 8 + 3 * sizeof(Real) + 12 + 4);    // draws
         b += W * sizeof(WavePartial) + 16;
         b += W * sizeof(float) + 16;                          // surrogate partials
@@ -158,11 +171,16 @@ __device__ inline bool pair_energy(const GBd& G, const PairGeom& X, double uix, 
 // (each lane adds only a few terms per move; FP64 adds and DPP moves are full rate), reduced
 // across lanes in a fixed DPP order and across wavefronts in a fixed order: deterministic.
 template <typename Real> struct LaneAcc;
+// FP32 pair terms: each lane sums its few terms in FP32 and the wavefront reduces them in FP32
+// (error <= ~9 u sum|terms| for <= 256 terms, far inside the 1e-5 tolerance); when a lane holds
+// a near-cap energy (|sum| > 1e30, where 64 of them could overflow FP32) the reduction runs in
+// FP64 instead (uniform branch).
 template <> struct LaneAcc<float> {
-    double s = 0.0;
-    __device__ void add(float v) { s += (double)v; }
+    float s = 0.0f;
+    __device__ void add(float v) { s += v; }
     __device__ void to_partial(WavePartial& p) {
-        p.d = wave_sum_d63(s);
+        if (__ballot(!(fabsf(s) <= 1.0e30f)) != 0ull) p.d = wave_sum_d63((double)s);
+        else p.d = (double)wave_sum_f63(s);
     }
     __device__ static double total(const WavePartial* p, int w) {
         double s = 0.0;
@@ -205,26 +223,63 @@ __device__ inline void rotate(double drot, double ux, double uy, double uz, doub
 
 // Candidate pre-filter of a move: sphere around the midpoint of the old and trial positions
 // with radius sqrt(rc2p) + |dq|/2 (slightly enlarged for FP32 rounding), FP32 distances.
+// For a cubic box (one scale on every axis) the test runs in fixed-point units, r2u = r2/s^2,
+// one multiply per axis fewer.
 struct MidFilter {
     uint32_t mx, my, mz;
-    float sx, sy, sz, r2;
+    float sx, sy, sz, r2, r2u;
+    bool iso;
     __device__ inline bool near(uint32_t qx, uint32_t qy, uint32_t qz) const {
         const float fx = (float)(int)(qx - mx) * sx, fy = (float)(int)(qy - my) * sy,
                     fz = (float)(int)(qz - mz) * sz;
         return fx * fx + fy * fy + fz * fz < r2;
     }
+    template <typename Q>
+    __device__ inline bool near(const Q& q, std::false_type) const { return near(q.x, q.y, q.z); }
+    template <typename Q>
+    __device__ inline bool near(const Q& q, std::true_type) const {
+        const float fx = (float)(int)(q.x - mx), fy = (float)(int)(q.y - my), fz = (float)(int)(q.z - mz);
+        return fx * fx + fy * fy + fz * fz < r2u;
+    }
 };
+
+// Rank of this lane among the set lanes of a ballot below it.
+__device__ inline int lane_rank(unsigned long long m) {
+    return (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                          __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+// Staged records: one address per particle, the components at fixed offsets.
+struct Pos3 { uint32_t x, y, z; };
+template <typename Real> struct Vec3 { Real x, y, z; };
+// Per-pass constants of the filter: sqrt(rc2p) and 1/s^2 for the cubic-box test.
+struct FilterConst {
+    float sx, sy, sz, rc_root, inv_s2;
+    bool iso;
+};
+__device__ inline FilterConst filter_const(float sx, float sy, float sz, float rc2p) {
+    FilterConst F;
+    F.sx = sx; F.sy = sy; F.sz = sz;
+    F.rc_root = sqrtf(rc2p);
+    F.inv_s2 = 1.0f / (sx * sx);
+    F.iso = sx == sy && sy == sz;
+    return F;
+}
+// The hardware square root (<= 1 ulp) is covered by the (1 + 1e-6) factors.
 __device__ inline MidFilter mid_filter(uint32_t qix, uint32_t qiy, uint32_t qiz, int dqx, int dqy,
-                                       int dqz, float sx, float sy, float sz, float rc2p) {
+                                       int dqz, const FilterConst& F) {
     MidFilter M;
     M.mx = qix + (uint32_t)(dqx >> 1);
     M.my = qiy + (uint32_t)(dqy >> 1);
     M.mz = qiz + (uint32_t)(dqz >> 1);
-    M.sx = sx; M.sy = sy; M.sz = sz;
-    const float hx = (float)dqx * sx, hy = (float)dqy * sy, hz = (float)dqz * sz;
-    const float rm = (sqrtf(rc2p) + 0.5f * sqrtf(hx * hx + hy * hy + hz * hz)) * (1.0f + 1e-6f) +
-                     2.0f * (sx + sy + sz);
+    M.sx = F.sx; M.sy = F.sy; M.sz = F.sz;
+    const float hx = (float)dqx * F.sx, hy = (float)dqy * F.sy, hz = (float)dqz * F.sz;
+    const float rm = (F.rc_root + 0.5f * __builtin_amdgcn_sqrtf(hx * hx + hy * hy + hz * hz)) *
+                         (1.0f + 1e-6f) + 2.0f * (F.sx + F.sy + F.sz);
     M.r2 = uni(rm * rm);
+    M.iso = F.iso;
+    // the radius in fixed-point units, rounded up by one part in 1e6 (conservative)
+    M.r2u = uni(rm * rm * F.inv_s2 * (1.0f + 1e-6f));
     return M;
 }
 
@@ -277,15 +332,11 @@ __global__ __launch_bounds__(64 * W) void k_pass(PassArgs P) {
 
     int* s_start = (int*)smem;                                 // 32
     int* s_off = s_start + 32;                                 // 32 (28 used)
-    uint32_t* sqx = (uint32_t*)(smem + 256);
-    uint32_t* sqy = sqx + cap;
-    uint32_t* sqz = sqy + cap;
-    Real* sux = (Real*)(sqz + cap);
-    Real* suy = sux + cap;
-    Real* suz = suy + cap;
-    unsigned short* slist = (unsigned short*)(suz + cap) + (size_t)w * lcap;
-    unsigned char* dp = (unsigned char*)((unsigned short*)(suz + cap) + (size_t)W * lcap);
-    dp = (unsigned char*)(((uintptr_t)dp + 15) & ~(uintptr_t)15);
+    using R3 = Vec3<Real>;
+    Pos3* sq = (Pos3*)(smem + 256);                            // stage_cap positions
+    R3* su = (R3*)(smem + L::off_su(cap));                     // cap orientations
+    unsigned short* slist = (unsigned short*)(smem + L::off_list(cap)) + w * L::list_stride(cap);
+    unsigned char* dp = smem + L::off_draws(cap);
     constexpr int KD = L::kDraws;                              // moves per draw chunk
     double* sthr = (double*)dp;                                // KD
     double* sthr1 = sthr + KD;                                 // KD
@@ -296,8 +347,9 @@ __global__ __launch_bounds__(64 * W) void k_pass(PassArgs P) {
     int* sdqy = sdqx + KD;
     int* sdqz = sdqy + KD;
     uint32_t* sj = (uint32_t*)(sdqz + KD);                     // KD
-    WavePartial* sred = (WavePartial*)(((uintptr_t)(sj + KD) + 15) & ~(uintptr_t)15);
-    float* ssur = (float*)(((uintptr_t)(sred + W) + 15) & ~(uintptr_t)15);
+    constexpr int kDrawBytes = KD * (32 + 3 * (int)sizeof(Real));        // a multiple of 16
+    WavePartial* sred = (WavePartial*)(dp + kDrawBytes);
+    float* ssur = (float*)(dp + ((kDrawBytes + W * (int)sizeof(WavePartial) + 15) & ~15));
     MlpRegs MR;
     if (DA) mlp_load(P.mlp, lane, MR);
 
@@ -353,12 +405,21 @@ __global__ __launch_bounds__(64 * W) void k_pass(PassArgs P) {
         }
         return;
     }
+    // the stencil padded to whole wavefronts with a sentinel: the point opposite the centre of
+    // the active cell (half a box away on every axis).  With >= 4 cells per axis it lies
+    // >= 1.5 cells from any cell member per axis, beyond every move's filter radius
+    // (<= r_c + half a cell diagonal, trial positions being confined to the cell).
+    const int S_pad = (S + 63) & ~63;
     if (staged) {
         for (int e = tid; e < S; e += 64 * W) {
             int gi = stencil_slot(s_off, s_start, part_off, e);
-            sqx[e] = P.qx[gi]; sqy[e] = P.qy[gi]; sqz[e] = P.qz[gi];
-            sux[e] = Ux[gi]; suy[e] = Uy[gi]; suz[e] = Uz[gi];
+            sq[e] = Pos3{P.qx[gi], P.qy[gi], P.qz[gi]};
+            su[e] = R3{Ux[gi], Uy[gi], Uz[gi]};
         }
+        const uint32_t fx = gx + (uint32_t)(((uint64_t)(2 * cx + 1) << 31) / (uint64_t)nx) + 0x80000000u;
+        const uint32_t fy = gy + (uint32_t)(((uint64_t)(2 * cy + 1) << 31) / (uint64_t)ny) + 0x80000000u;
+        const uint32_t fz = gz + (uint32_t)(((uint64_t)(2 * cz + 1) << 31) / (uint64_t)nz) + 0x80000000u;
+        for (int e = S + tid; e < S_pad; e += 64 * W) sq[e] = Pos3{fx, fy, fz};
     }
 
     const int k = D.k;
@@ -375,6 +436,20 @@ __global__ __launch_bounds__(64 * W) void k_pass(PassArgs P) {
     // colloid-rod terms only matter within R + 4 sigma_ee of the colloid centre
     const double creach = C.R + C.colloid_cut + 0.5;
     const double creach2 = creach * creach;
+    const FilterConst FC = filter_const(sx, sy, sz, rc2p);
+    // colloid tests only in cells that reach within creach of the box centre: the distance of
+    // the cell centre minus half the cell diagonal (with a margin) bounds every position of
+    // the cell, old and trial (trial positions are confined to the cell)
+    bool cnear = false;
+    if (colloid_on) {
+        const double wx = D.L[0] / nx, wy = D.L[1] / ny, wz = D.L[2] / nz;
+        const double ccx = (double)(int32_t)(gx + (uint32_t)(((uint64_t)(2 * cx + 1) << 31) / (uint64_t)nx) - 0x80000000u) * X.s0;
+        const double ccy = (double)(int32_t)(gy + (uint32_t)(((uint64_t)(2 * cy + 1) << 31) / (uint64_t)ny) - 0x80000000u) * X.s1;
+        const double ccz = (double)(int32_t)(gz + (uint32_t)(((uint64_t)(2 * cz + 1) << 31) / (uint64_t)nz) - 0x80000000u) * X.s2;
+        const double dc = sqrt(ccx * ccx + ccy * ccy + ccz * ccz);
+        const double hd = 0.5 * sqrt(wx * wx + wy * wy + wz * wz);
+        cnear = dc - hd <= creach * (1.0 + 1e-9) + 1e-6;
+    }
 
     if (nc == 0) {
         if (tracing)
@@ -407,8 +482,10 @@ __global__ __launch_bounds__(64 * W) void k_pass(PassArgs P) {
                 uint32_t qix, qiy, qiz;
                 Real uix, uiy, uiz;
                 if (staged) {
-                    qix = uni(sqx[li]); qiy = uni(sqy[li]); qiz = uni(sqz[li]);
-                    uix = uni(sux[li]); uiy = uni(suy[li]); uiz = uni(suz[li]);
+                    const Pos3 qi = sq[li];
+                    const R3 ui = su[li];
+                    qix = uni(qi.x); qiy = uni(qi.y); qiz = uni(qi.z);
+                    uix = uni(ui.x); uiy = uni(ui.y); uiz = uni(ui.z);
                 } else {
                     qix = P.qx[gsel]; qiy = P.qy[gsel]; qiz = P.qz[gsel];
                     uix = Ux[gsel]; uiy = Uy[gsel]; uiz = Uz[gsel];
@@ -428,7 +505,7 @@ __global__ __launch_bounds__(64 * W) void k_pass(PassArgs P) {
                     cell_coord(qnz, gz, (uint32_t)nz) != (uint32_t)cz) {
                     outcome = MCGB_MOVE_LEFT;       // moves are confined to the active cell
                     ++n_left;
-                } else if (colloid_on && rnx * rnx + rny * rny + rnz * rnz <= R2) {
+                } else if (cnear && rnx * rnx + rny * rny + rnz * rnz <= R2) {
                     outcome = MCGB_MOVE_HARD;       // MCnptColloid.c:2516-2520
                     ++n_hard;
                 } else {
@@ -443,23 +520,29 @@ __global__ __launch_bounds__(64 * W) void k_pass(PassArgs P) {
                         // candidates of both states, wavefront-private compaction: a partner
                         // within the cutoff of either state lies within rc + |dq|/2 of their
                         // midpoint (triangle inequality), so one FP32 distance per candidate
-                        const MidFilter M = mid_filter(qix, qiy, qiz, dqx, dqy, dqz, sx, sy, sz, rc2p);
-                        // two candidates per lane per step, branch-free (clamped in-bounds
-                        // reads, validity folded into the predicate): six LDS reads in flight
-                        for (int e0 = w * 64; e0 < S; e0 += 128 * W) {
-                            const int ea = e0 + lane, eb = e0 + 64 * W + lane;
-                            const int ca = min(ea, S - 1), cb = min(eb, S - 1);
-                            const uint32_t xa = sqx[ca], ya = sqy[ca], za = sqz[ca];
-                            const uint32_t xb = sqx[cb], yb = sqy[cb], zb = sqz[cb];
-                            const bool ka = M.near(xa, ya, za) & (ea < S) & (ea != li);
-                            const bool kb = M.near(xb, yb, zb) & (eb < S) & (eb != li);
-                            const unsigned long long ma = __ballot(ka);
-                            const unsigned long long mb = __ballot(kb);
-                            const int na = __popcll(ma);
-                            if (ka) slist[base + __popcll(ma & __lanemask_lt())] = (unsigned short)ea;
-                            if (kb) slist[base + na + __popcll(mb & __lanemask_lt())] = (unsigned short)eb;
-                            base += na + __popcll(mb);
-                        }
+                        const MidFilter M = mid_filter(qix, qiy, qiz, dqx, dqy, dqz, FC);
+                        // two candidates per lane per step, branch-free: the padded tail holds
+                        // sentinels the filter rejects; the second wavefront-wide group of a
+                        // step exists when the stencil reaches it (uniform), else lane a's
+                        // record is read again and dropped
+                        auto scan = [&](auto iso) {
+                            for (int e0 = w * 64; e0 < S_pad; e0 += 128 * W) {
+                                const bool hasb = e0 + 64 * W < S_pad;
+                                const int ea = e0 + lane, eb = e0 + 64 * W + lane;
+                                const Pos3 pa = sq[ea], pb = sq[hasb ? eb : ea];
+                                const bool ka = M.near(pa, iso) & (ea != li);
+                                const bool kb = M.near(pb, iso) & (eb != li) & hasb;
+                                const unsigned long long ma = __ballot(ka);
+                                const unsigned long long mb = __ballot(kb);
+                                const int na = __popcll(ma);
+                                // every lane writes: unselected ones into the spare slot
+                                slist[ka ? base + lane_rank(ma) : lcap] = (unsigned short)ea;
+                                slist[kb ? base + na + lane_rank(mb) : lcap] = (unsigned short)eb;
+                                base += na + __popcll(mb);
+                            }
+                        };
+                        if (M.iso) scan(std::true_type{});
+                        else scan(std::false_type{});
                         // the list is private to this wavefront: a wavefront-scope fence suffices
                         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
                         __builtin_amdgcn_wave_barrier();
@@ -467,13 +550,13 @@ __global__ __launch_bounds__(64 * W) void k_pass(PassArgs P) {
                     }
                     // colloid-rod term of both states in FP64, only near the colloid
                     double cacc = 0.0, cscl = 0.0;
-                    if (colloid_on) {
+                    if (colloid_on) n_coll += 2;
+                    if (cnear) {
                         const double rox = (double)(int32_t)(qix - 0x80000000u) * X.s0;
                         const double roy = (double)(int32_t)(qiy - 0x80000000u) * X.s1;
                         const double roz = (double)(int32_t)(qiz - 0x80000000u) * X.s2;
                         const bool nearc = rox * rox + roy * roy + roz * roz <= creach2 ||
                                            rnx * rnx + rny * rny + rnz * rnz <= creach2;
-                        n_coll += 2;
                         if (nearc) {
                             bool co, cn;
                             const double eo = colloid_f64(C, rox, roy, roz, (double)uix, (double)uiy,
@@ -499,15 +582,17 @@ __global__ __launch_bounds__(64 * W) void k_pass(PassArgs P) {
                             if (lane < 32 && it < items) {
                                 const int e = slist[it >> 1];
                                 const bool nw = it & 1;
-                                const double dx = (double)(int)(sqx[e] - (nw ? qnx : qix)) * X.s0;
-                                const double dy = (double)(int)(sqy[e] - (nw ? qny : qiy)) * X.s1;
-                                const double dz = (double)(int)(sqz[e] - (nw ? qnz : qiz)) * X.s2;
+                                const Pos3 pe = sq[e];
+                                const double dx = (double)(int)(pe.x - (nw ? qnx : qix)) * X.s0;
+                                const double dy = (double)(int)(pe.y - (nw ? qny : qiy)) * X.s1;
+                                const double dz = (double)(int)(pe.z - (nw ? qnz : qiz)) * X.s2;
                                 const double r2 = dx * dx + dy * dy + dz * dz;
                                 if (r2 < X.rc2) {
                                     on = true;
                                     const float ex = (float)(nw ? unx : uix), ey = (float)(nw ? uny : uiy),
                                                 ez = (float)(nw ? unz : uiz);
-                                    const float jx = (float)sux[e], jy = (float)suy[e], jz = (float)suz[e];
+                                    const R3 ue = su[e];
+                                    const float jx = (float)ue.x, jy = (float)ue.y, jz = (float)ue.z;
                                     const float fx = (float)dx, fy = (float)dy, fz = (float)dz;
                                     fr = sqrtf((float)r2);
                                     const float ir = 1.0f / fr;
@@ -539,11 +624,13 @@ __global__ __launch_bounds__(64 * W) void k_pass(PassArgs P) {
                             const int e = slist[it >> 1];
                             const bool nw = it & 1;
                             const uint32_t px = nw ? qnx : qix, py = nw ? qny : qiy, pz = nw ? qnz : qiz;
+                            const Pos3 pe = sq[e];
+                            const R3 ue = su[e];
                             bool cp;
                             Real u;
                             if (pair_energy(G, X, nw ? unx : uix, nw ? uny : uiy, nw ? unz : uiz,
-                                            sux[e], suy[e], suz[e], (int)(sqx[e] - px),
-                                            (int)(sqy[e] - py), (int)(sqz[e] - pz), &u, &cp)) {
+                                            ue.x, ue.y, ue.z, (int)(pe.x - px),
+                                            (int)(pe.y - py), (int)(pe.z - pz), &u, &cp)) {
                                 ++my_pairs;
                                 if (cp) cpl += nw ? 1 : -1;
                                 else {
@@ -622,8 +709,8 @@ __global__ __launch_bounds__(64 * W) void k_pass(PassArgs P) {
                         moved += fabs(dfin);
                         if (tid == 0) {
                             if (staged) {
-                                sqx[li] = qnx; sqy[li] = qny; sqz[li] = qnz;
-                                sux[li] = unx; suy[li] = uny; suz[li] = unz;
+                                sq[li] = Pos3{qnx, qny, qnz};
+                                su[li] = R3{unx, uny, unz};
                             }
                             P.qx[gsel] = qnx; P.qy[gsel] = qny; P.qz[gsel] = qnz;
                             Ux[gsel] = unx; Uy[gsel] = uny; Uz[gsel] = unz;

@@ -99,7 +99,7 @@ __device__ inline void spec_scan(const G& Gc, const PairGeom& X, float sx, float
                                  Real uny, Real unz, int lane, SpecAcc<Real>& A) {
     constexpr int R = SpecLds<Real, 1>::kRing;
     const uint32_t qnx = qix + (uint32_t)dqx, qny = qiy + (uint32_t)dqy, qnz = qiz + (uint32_t)dqz;
-    const MidFilter M = mid_filter(qix, qiy, qiz, dqx, dqy, dqz, sx, sy, sz, rc2p);
+    const MidFilter M = mid_filter(qix, qiy, qiz, dqx, dqy, dqz, filter_const(sx, sy, sz, rc2p));
     int head = 0, tail = 0;
     auto flush = [&](int n) {          // n <= 32 slots, wavefront-uniform
         wave_sync_lds();

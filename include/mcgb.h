@@ -78,6 +78,8 @@ typedef struct mcgb_params {
                                                parallel, decided in order), 0: sequential  */
     int      adapt_moves;                   /* step-size adaptation every >= adapt_moves
                                                attempted moves (the reference: 10000), 0 = off */
+    int      sublattice_colors;             /* 2: 8 sublattices of cells >= r_c wide (default);
+                                               3: 27 sublattices of cells >= r_c/2 wide      */
 
     uint32_t keys_read;                     /* bit i set: required key i was read          */
 } mcgb_params;
@@ -184,17 +186,18 @@ int mcgb_get_state_fixed(mcgb_ctx *ctx, int replica, int n,
 /* ---------------------------------------------------------------- hot path */
 
 /* Runs n_sweeps checkerboard sweeps on every replica.  One sweep = random grid shift, cell-list
- * rebuild, 8 sublattice passes of ceil(N/n_cells) trial moves per active cell, then (NPT) one
- * ln V move.  Replaces the sub-cell loop of main() (MCnptColloid.c:643-706) with its TODO body
+ * rebuild, P sublattice passes of ceil(N/n_cells) trial moves per active cell, then (NPT) one
+ * ln V move; P = 8 with sublattice_colors 2 (cells >= r_c wide), 27 with 3 (cells >= r_c/2
+ * wide, partners within +-2 cells).  Replaces the sub-cell loop of main() (MCnptColloid.c:643-706) with its TODO body
  * filled by the move of attempt_monte_carlo_move() (:2445-2604), and build_neighbor_cells()
  * (:5048-5264) / clean_neighbor_cells() (:5035-5046) by the device cell list. */
 int mcgb_sweep(mcgb_ctx *ctx, int n_sweeps);
 
 /* Runs ONE plain Metropolis sweep of a single-replica context and records every trial move
  * (MCGB_ERR_ARG with more than one replica or with delayed acceptance enabled).
- * Record index = pass*(n_cells/8)*k + active_cell*k + move; capacity must be >= n_cells*k.
- * info receives {shift_x, shift_y, shift_z, order[0..7], k, n_cells}. */
-int mcgb_sweep_traced(mcgb_ctx *ctx, mcgb_move_trace *trace, int capacity, uint32_t info[13]);
+ * Record index = pass*(n_cells/P)*k + active_cell*k + move; capacity must be >= n_cells*k.
+ * info receives {shift_x, shift_y, shift_z, k, n_cells, P, order[0..P-1]}, the rest 0. */
+int mcgb_sweep_traced(mcgb_ctx *ctx, mcgb_move_trace *trace, int capacity, uint32_t info[36]);
 
 /* Running statistics of replica r. */
 int mcgb_get_stats(mcgb_ctx *ctx, int replica, mcgb_stats *out);
@@ -225,7 +228,8 @@ int mcgb_series_clear(mcgb_ctx *ctx);
  * supported number of wavefronts per active cell (1, 2, 3, 4, 8; the LDS staging capacity
  * follows) and keeps the fastest.  The timed sweeps are ordinary sweeps of the simulation (they
  * advance it).  No counterpart in the reference.  ms_per_pass (may be NULL) receives the
- * measured pass times, index 5*schedule + {0,1,2,3,4} for 1, 2, 3, 4, 8 wavefronts. */
+ * measured pass times, index 5*schedule + {0,1,2,3,4} for 1, 2, 3, 4, 8 wavefronts (0 for the
+ * speculative schedule with sublattice_colors 3, which it does not run). */
 int mcgb_autotune(mcgb_ctx *ctx, int sweeps, int *best_waves, int *best_schedule,
                   double ms_per_pass[10]);
 
@@ -239,7 +243,7 @@ int mcgb_kernel_times(mcgb_ctx *ctx, double *ms_pass, double *ms_rebuild, double
  * dist_batch_to_cells() (:4942-4964). */
 int mcgb_cell_list(mcgb_ctx *ctx, int replica, uint32_t t,
                    int *cell_start, int cell_start_len, int *pid_sorted, int n,
-                   uint32_t shift[3], uint32_t order[8], int dims[3]);
+                   uint32_t shift[3], uint32_t order[27], int dims[3]);
 
 /* ---------------------------------------------------------------- kernels on batches */
 

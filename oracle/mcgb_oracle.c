@@ -25,13 +25,18 @@
  *   key = {lo32(seed), hi32(seed) + replica}; sweep index t.
  *   set-up draws S_j = philox({j, 0, t, 0xFFFF0000}, key), j = 0, 1, 2:
  *     grid shift g = S_0[0..2] (fixed-point), NPT d lnV draw S_0[3], NPT accept draw S_2[3];
- *     sublattice order = Fisher-Yates of 0..7, i = 7..1 using S_1[0..3], S_2[0..2] in turn,
- *     j = (r*(i+1)) >> 32.
+ *     sublattice order = Fisher-Yates of 0..P-1, i = P-1..1 using S_1[0..3], S_2[0..2], then
+ *     S_3..S_7 in turn, j = (r*(i+1)) >> 32.
  *   positions are 32-bit fixed-point fractions of the box: x = q*L/2^32; minimum image is the
- *     wrap-around int32 difference; the cell of q is ((q - g)*n) >> 32 per axis, n the largest
- *     even count with L/n >= r_c (n >= 4); sublattice (cx&1) + 2(cy&1) + 4(cz&1) as in
- *     cell_coords_to_index (MCnptColloid.c:3692-3695).
- *   pass p = 0..7 activates the cells of sublattice order[p]; every active cell makes
+ *     wrap-around int32 difference; the cell of q is ((q - g)*n) >> 32 per axis.
+ *   c colours per axis (consts.colors), P = c^3 sublattices:
+ *     c = 2: n the largest even count with L/n >= r_c (n >= 4); sublattice (cx&1) + 2(cy&1) +
+ *            4(cz&1) as in cell_coords_to_index (MCnptColloid.c:3692-3695); stencil +-1 cell;
+ *     c = 3: n the largest multiple of 3 with L/n >= r_c/2 (n >= 6); sublattice
+ *            (cx%3) + 3(cy%3) + 9(cz%3); stencil +-2 cells.
+ *     Cells of one sublattice are separated by c-1 cells (>= r_c) and moves are confined to
+ *     their cell, so the cells of a pass are independent.
+ *   pass p = 0..P-1 activates the cells of sublattice order[p]; every active cell makes
  *     k = ceil(N/n_cells) moves, move m drawing A = philox({m, cell, t, p<<8}, key) and
  *     B = philox({m, cell, t, (p<<8)|1}, key):
  *       particle  = sorted cell member (A0*n_c) >> 32
@@ -191,6 +196,13 @@ int orc_cell_dim(double L, double rc)
     return n >= 4 ? n : 0;
 }
 
+int orc_grid_dim(double L, double rc, int colors)
+{
+    if (colors != 3) return orc_cell_dim(L, rc);
+    int n = 3 * (int)floor(L / (1.5 * rc * (1.0 + 1e-6)));
+    return n >= 6 ? n : 0;
+}
+
 static inline uint32_t cell_coord(uint32_t q, uint32_t g, uint32_t n)
 {
     return (uint32_t)(((uint64_t)(uint32_t)(q - g) * n) >> 32);
@@ -219,22 +231,28 @@ void orc_cell_list(int N, const uint32_t *qx, const uint32_t *qy, const uint32_t
     free(cid);
 }
 
-void orc_sweep_setup(uint64_t seed, uint32_t replica, uint32_t t, uint32_t g[3], uint32_t order[8],
-                     uint32_t vol[2])
+void orc_sweep_setup(uint64_t seed, uint32_t replica, uint32_t t, int colors, uint32_t g[3],
+                     uint32_t order[27], uint32_t vol[2])
 {
-    uint32_t key[2], s0[4], s1[4], s2[4];
+    uint32_t key[2], s[8][4];
     make_key(seed, replica, key);
-    uint32_t c0[4] = {0u, 0u, t, 0xFFFF0000u}, c1[4] = {1u, 0u, t, 0xFFFF0000u},
-             c2[4] = {2u, 0u, t, 0xFFFF0000u};
-    orc_philox(c0, key, s0);
-    orc_philox(c1, key, s1);
-    orc_philox(c2, key, s2);
-    g[0] = s0[0]; g[1] = s0[1]; g[2] = s0[2];
-    vol[0] = s0[3]; vol[1] = s2[3];
-    /* shuffle_int_array (MCnptColloid.c:3721-3737) with Philox draws */
-    uint32_t draws[7] = {s1[0], s1[1], s1[2], s1[3], s2[0], s2[1], s2[2]};
-    for (int i = 0; i < 8; ++i) order[i] = (uint32_t)i;
-    for (int i = 7, d = 0; i >= 1; --i, ++d) {
+    for (uint32_t j = 0; j < 8; ++j) {
+        uint32_t cj[4] = {j, 0u, t, 0xFFFF0000u};
+        orc_philox(cj, key, s[j]);
+    }
+    g[0] = s[0][0]; g[1] = s[0][1]; g[2] = s[0][2];
+    vol[0] = s[0][3]; vol[1] = s[2][3];
+    /* shuffle_int_array (MCnptColloid.c:3721-3737) with Philox draws: S_1[0..3], S_2[0..2],
+     * then (27 colours) S_3..S_7 in order, S_7[3] unused */
+    uint32_t draws[26];
+    int nd = 0;
+    for (int w = 0; w < 4; ++w) draws[nd++] = s[1][w];
+    for (int w = 0; w < 3; ++w) draws[nd++] = s[2][w];
+    for (int j = 3; j < 8; ++j)
+        for (int w = 0; w < 4 && nd < 26; ++w) draws[nd++] = s[j][w];
+    const int P = colors == 3 ? 27 : 8;
+    for (int i = 0; i < P; ++i) order[i] = (uint32_t)i;
+    for (int i = P - 1, d = 0; i >= 1; --i, ++d) {
         uint32_t j = (uint32_t)(((uint64_t)draws[d] * (uint64_t)(i + 1)) >> 32);
         uint32_t tmp = order[i]; order[i] = order[j]; order[j] = tmp;
     }
@@ -259,18 +277,18 @@ typedef struct {
 } energy_view;
 
 /* finite energy difference and cap changes of moving particle i from (q, u) to (qn, un),
- * summed over the 27-cell stencil of cell (cx, cy, cz). */
+ * summed over the (2R+1)^3-cell stencil of cell (cx, cy, cz). */
 static void move_delta(const energy_view *v, int i, const uint32_t qn[3], const double un[3],
-                       int cx, int cy, int cz, int *dcap, double *dfin, double *scale,
+                       int cx, int cy, int cz, int R, int *dcap, double *dfin, double *scale,
                        int64_t *ncand, int64_t *npair)
 {
     const orc_consts *c = v->c;
     const double ui[3] = {v->ux[i], v->uy[i], v->uz[i]};
     int capo_n = 0, capn_n = 0;
     double acc = 0.0, sc = 0.0;
-    for (int dz = -1; dz <= 1; ++dz)
-        for (int dy = -1; dy <= 1; ++dy)
-            for (int dx = -1; dx <= 1; ++dx) {
+    for (int dz = -R; dz <= R; ++dz)
+        for (int dy = -R; dy <= R; ++dy)
+            for (int dx = -R; dx <= R; ++dx) {
                 int ex = (cx + dx + v->n[0]) % v->n[0];
                 int ey = (cy + dy + v->n[1]) % v->n[1];
                 int ez = (cz + dz + v->n[2]) % v->n[2];
@@ -419,26 +437,31 @@ int orc_total_energy_brute(const orc_consts *c, int N, const double *x, const do
 int orc_sweep(const orc_consts *c, int N, uint32_t *qx, uint32_t *qy, uint32_t *qz,
               double *ux, double *uy, double *uz, double L[3],
               uint64_t seed, uint32_t replica, uint32_t t, int mc_npt, double npt_max_dlnv,
-              double energy[3], int64_t counters[9], orc_move_trace *trace, uint32_t info[13],
+              double energy[3], int64_t counters[9], orc_move_trace *trace, uint32_t info[36],
               int resum, int nthreads)
 {
+    const int col = c->colors == 3 ? 3 : 2;        /* sublattice colours per axis */
+    const int P = col * col * col;                 /* passes per sweep */
+    const int R = col - 1;                         /* stencil reach in cells */
     int n[3];
     for (int d = 0; d < 3; ++d) {
-        n[d] = orc_cell_dim(L[d], c->cutoff);
+        n[d] = orc_grid_dim(L[d], c->cutoff, col);
         if (n[d] == 0) return -1;
     }
     const int nc = n[0] * n[1] * n[2];
-    const int hx = n[0] / 2, hy = n[1] / 2, hz = n[2] / 2;
+    const int hx = n[0] / col, hy = n[1] / col, hz = n[2] / col;
     const int nact = hx * hy * hz;
     const int k = (N + nc - 1) / nc;
-    uint32_t g[3], order[8], vol[2], key[2];
-    orc_sweep_setup(seed, replica, t, g, order, vol);
+    uint32_t g[3], order[27], vol[2], key[2];
+    orc_sweep_setup(seed, replica, t, col, g, order, vol);
     make_key(seed, replica, key);
     if (info) {
+        memset(info, 0, 36 * sizeof(uint32_t));
         for (int d = 0; d < 3; ++d) info[d] = g[d];
-        for (int p = 0; p < 8; ++p) info[3 + p] = order[p];
-        info[11] = (uint32_t)k;
-        info[12] = (uint32_t)nc;
+        info[3] = (uint32_t)k;
+        info[4] = (uint32_t)nc;
+        info[5] = (uint32_t)P;
+        for (int p = 0; p < P; ++p) info[6 + p] = order[p];
     }
     int *cs = (int *)malloc(sizeof(int) * (size_t)(nc + 1));
     int *so = (int *)malloc(sizeof(int) * (size_t)(N > 0 ? N : 1));
@@ -456,16 +479,17 @@ int orc_sweep(const orc_consts *c, int N, uint32_t *qx, uint32_t *qy, uint32_t *
     double *acc_f = (double *)calloc((size_t)nact * 2, sizeof(double));
     double moved_abs = 0.0;     /* sum of |finite dE| of accepted moves in this sweep */
 
-    for (int p = 0; p < 8; ++p) {
+    for (int p = 0; p < P; ++p) {
         const uint32_t sig = order[p];
+        const int sx = (int)(sig % (uint32_t)col), sy = (int)((sig / (uint32_t)col) % (uint32_t)col),
+                  sz = (int)(sig / (uint32_t)(col * col));
         memset(acc_i, 0, sizeof(int64_t) * (size_t)nact * 9);
         memset(acc_f, 0, sizeof(double) * (size_t)nact * 2);
         (void)nthreads;
 #pragma omp parallel for schedule(dynamic, 4) num_threads(nthreads > 0 ? nthreads : 1)
         for (int a = 0; a < nact; ++a) {
             int ax = a % hx, ay = (a / hx) % hy, az = a / (hx * hy);
-            int cx = 2 * ax + (int)(sig & 1u), cy = 2 * ay + (int)((sig >> 1) & 1u),
-                cz = 2 * az + (int)((sig >> 2) & 1u);
+            int cx = col * ax + sx, cy = col * ay + sy, cz = col * az + sz;
             int cell = cx + n[0] * (cy + n[1] * cz);
             int c0 = cs[cell], ncnt = cs[cell + 1] - cs[cell];
             int64_t *ai = acc_i + (size_t)a * 9;
@@ -519,7 +543,7 @@ int orc_sweep(const orc_consts *c, int N, uint32_t *qx, uint32_t *qy, uint32_t *
                 int dcap = 0, dccap = 0;
                 double dfin = 0.0, scale = 0.0;
                 int64_t ncand = 0, npair = 0;
-                move_delta(&v, i, qn, un, cx, cy, cz, &dcap, &dfin, &scale, &ncand, &npair);
+                move_delta(&v, i, qn, un, cx, cy, cz, R, &dcap, &dfin, &scale, &ncand, &npair);
                 ai[4] += ncand;
                 ai[5] += npair;
                 if (c->colloid_on) {

@@ -27,7 +27,8 @@ typedef struct orc_consts {
     double max_displacement, max_rotation;
     double colloid_radius;
     int    colloid_on;                 /* colloid_wall_option == 1 */
-    int    pad_;
+    int    colors;                     /* sublattice colours per axis: 2 (default, 0 means 2)
+                                          or 3 */
     /* derived */
     double beta, four_eps0, rc2, kappa2, chi, chi2, k_imu, chi_prime, sigma_ee, colloid_cut, colloid_R2;
 } orc_consts;
@@ -54,28 +55,35 @@ void   orc_philox(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4])
 /* Largest even n with L/n >= rc*(1+1e-6); 0 if that n is below 4. */
 int    orc_cell_dim(double L, double rc);
 
+/* Sweep grid for `colors` sublattice colours per axis: 2 -> orc_cell_dim; 3 -> the largest
+ * multiple of 3 with L/n >= (rc/2)(1+1e-6), 0 if below 6 (cells of one colour are then two
+ * cells, >= rc, apart, and a move's partners lie within +-2 cells). */
+int    orc_grid_dim(double L, double rc, int colors);
+
 /* Cell list of one configuration for grid shift g: cell_start[n_cells+1] and particle
  * indices sorted by (cell, index). */
 void   orc_cell_list(int N, const uint32_t *qx, const uint32_t *qy, const uint32_t *qz,
                      const int n[3], const uint32_t g[3], int *cell_start, int *sorted);
 
-/* Per-sweep set-up draws: grid shift, sublattice order, NPT draws (vol[0]: d lnV, vol[1]: accept). */
-void   orc_sweep_setup(uint64_t seed, uint32_t replica, uint32_t t, uint32_t g[3], uint32_t order[8],
-                       uint32_t vol[2]);
+/* Per-sweep set-up draws: grid shift, sublattice order (colors^3 entries), NPT draws
+ * (vol[0]: d lnV, vol[1]: accept). */
+void   orc_sweep_setup(uint64_t seed, uint32_t replica, uint32_t t, int colors, uint32_t g[3],
+                       uint32_t order[27], uint32_t vol[2]);
 
-/* One checkerboard sweep (grid shift, cell list, 8 passes, and for mc_npt != 0 one ln V move).
+/* One checkerboard sweep (grid shift, cell list, colors^3 passes, and for mc_npt != 0 one ln V
+ * move).
  * energy = {cap_pairs, cap_colloid, finite} running total, updated in place.
  * counters: [0] attempted [1] accepted [2] left cell [3] hard core [4] candidates [5] pairs
  *           [6] colloid evals [7] volume attempted [8] volume accepted.
  * trace may be NULL; else it must hold n_cells*k records.  info (may be NULL) receives
- * {g[3], order[8], k, n_cells}.  nthreads > 1 runs the independent cells of a pass with OpenMP
+ * {g[3], k, n_cells, passes, order[passes]} (36 words, the rest 0).  nthreads > 1 runs the independent cells of a pass with OpenMP
  * (results do not depend on it).  After the passes the running energy is replaced by an FP64
  * recompute when resum != 0 or when the accepted moves carried sum |finite dE| > 1e10.
  * Returns 0, or -1 on a geometry error. */
 int    orc_sweep(const orc_consts *c, int N, uint32_t *qx, uint32_t *qy, uint32_t *qz,
                  double *ux, double *uy, double *uz, double L[3],
                  uint64_t seed, uint32_t replica, uint32_t t, int mc_npt, double npt_max_dlnv,
-                 double energy[3], int64_t counters[9], orc_move_trace *trace, uint32_t info[13],
+                 double energy[3], int64_t counters[9], orc_move_trace *trace, uint32_t info[36],
                  int resum, int nthreads);
 
 /* Step-size controller of the reference's serial driver (MCnptColloid.c:987-1031), applied to

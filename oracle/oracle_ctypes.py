@@ -20,7 +20,7 @@ class OrcConsts(C.Structure):
     _fields_ = [(n, C.c_double) for n in (
         "sigma0", "epsilon0", "kappa", "kappa_prime", "mu", "nu",
         "temperature", "pressure", "cutoff", "max_displacement", "max_rotation",
-        "colloid_radius")] + [("colloid_on", C.c_int), ("pad_", C.c_int)] + [
+        "colloid_radius")] + [("colloid_on", C.c_int), ("colors", C.c_int)] + [
         (n, C.c_double) for n in (
             "beta", "four_eps0", "rc2", "kappa2", "chi", "chi2", "k_imu", "chi_prime",
             "sigma_ee", "colloid_cut", "colloid_R2")]
@@ -56,8 +56,10 @@ def lib():
         L.orc_philox.argtypes = [up, up, up]
         L.orc_cell_dim.argtypes = [C.c_double, C.c_double]
         L.orc_cell_dim.restype = C.c_int
+        L.orc_grid_dim.argtypes = [C.c_double, C.c_double, C.c_int]
+        L.orc_grid_dim.restype = C.c_int
         L.orc_cell_list.argtypes = [C.c_int, up, up, up, ip, up, ip, ip]
-        L.orc_sweep_setup.argtypes = [C.c_uint64, C.c_uint32, C.c_uint32, up, up, up]
+        L.orc_sweep_setup.argtypes = [C.c_uint64, C.c_uint32, C.c_uint32, C.c_int, up, up, up]
         L.orc_sweep.argtypes = [C.POINTER(OrcConsts), C.c_int, up, up, up, dp, dp, dp, dp,
                                 C.c_uint64, C.c_uint32, C.c_uint32, C.c_int, C.c_double,
                                 dp, C.POINTER(C.c_int64), C.POINTER(MoveTrace), up, C.c_int,
@@ -90,11 +92,11 @@ def _i(a):
 
 def consts(sigma0=1.0, epsilon0=1.0, kappa=3.0, kappa_prime=5.0, mu=2.0, nu=1.0,
            temperature=1.0, pressure=0.0, cutoff=4.0, max_displacement=0.1, max_rotation=0.1,
-           colloid_radius=0.0, colloid_on=0):
+           colloid_radius=0.0, colloid_on=0, colors=2):
     c = OrcConsts(sigma0=sigma0, epsilon0=epsilon0, kappa=kappa, kappa_prime=kappa_prime, mu=mu,
                   nu=nu, temperature=temperature, pressure=pressure, cutoff=cutoff,
                   max_displacement=max_displacement, max_rotation=max_rotation,
-                  colloid_radius=colloid_radius, colloid_on=colloid_on)
+                  colloid_radius=colloid_radius, colloid_on=colloid_on, colors=colors)
     lib().orc_consts_init(C.byref(c))
     return c
 
@@ -141,6 +143,11 @@ def cell_dim(L, rc):
     return lib().orc_cell_dim(float(L), float(rc))
 
 
+def grid_dim(L, rc, colors=2):
+    """Cells per axis of the sweep grid for 2 or 3 sublattice colours per axis."""
+    return lib().orc_grid_dim(float(L), float(rc), int(colors))
+
+
 def cell_list(q, n, g):
     q = [np.ascontiguousarray(a, dtype=np.uint32) for a in q]
     N = q[0].shape[0]
@@ -152,12 +159,12 @@ def cell_list(q, n, g):
     return cs, so[:N]
 
 
-def sweep_setup(seed, replica, t):
+def sweep_setup(seed, replica, t, colors=2):
     g = np.empty(3, dtype=np.uint32)
-    order = np.empty(8, dtype=np.uint32)
+    order = np.empty(27, dtype=np.uint32)
     vol = np.empty(2, dtype=np.uint32)
-    lib().orc_sweep_setup(seed, replica, t, _u(g), _u(order), _u(vol))
-    return g, order, vol
+    lib().orc_sweep_setup(seed, replica, t, int(colors), _u(g), _u(order), _u(vol))
+    return g, order[:colors ** 3], vol
 
 
 class System:
@@ -204,12 +211,13 @@ class System:
         return [self.q[d].astype(np.float64) * (self.L[d] / TWO32) for d in range(3)]
 
     def sweep(self, trace=False):
-        nc = int(np.prod([cell_dim(self.L[d], self.c.cutoff) for d in range(3)]))
+        col = 3 if self.c.colors == 3 else 2
+        nc = int(np.prod([grid_dim(self.L[d], self.c.cutoff, col) for d in range(3)]))
         if nc == 0:
             raise ValueError("box too small for the cell grid")
         k = -(-self.N // nc)
         tr = np.zeros(nc * k, dtype=TRACE_DTYPE) if trace else None
-        info = np.zeros(13, dtype=np.uint32)
+        info = np.zeros(36, dtype=np.uint32)
         cnt = np.zeros(9, dtype=np.int64)
         rc = lib().orc_sweep(
             C.byref(self.c), self.N, _u(self.q[0]), _u(self.q[1]), _u(self.q[2]),

@@ -27,6 +27,7 @@ namespace {
 
 constexpr double kPi = 3.14159265358979323846;
 constexpr int kSeriesCapDefault = 1 << 16;
+constexpr int kEvSet = 27 + 3;     // events of one timed sweep: start, rebuild, passes, end
 
 struct Events {
     std::vector<hipEvent_t> ev;
@@ -87,7 +88,8 @@ struct mcgb_ctx {
     size_t dtrace_cap = 0;
     std::vector<void*> allocs;
     Events events;
-    hipEvent_t tev[22] = {};
+    int col = 2, npass = 8;    // sublattice colours per axis and passes per sweep
+    hipEvent_t tev[2 * kEvSet] = {};
     hipGraphExec_t graph[2] = {nullptr, nullptr};
     // learned first stage + delayed acceptance
     float* sur_P = nullptr;            // FP32 master parameters (kNParam)
@@ -163,6 +165,7 @@ void fill_consts(const mcgb_params& p, RepConst* C) {
     C->one_minus_chip = 1.0 - dv[7];
     C->npt_max_dlnv = p.npt_max_dlnv;
     C->adapt_moves = p.adapt_moves;
+    C->col = p.sublattice_colors == 3 ? 3 : 2;
     C->min_thick = (float)(dv[8] < p.gb_sigma_sidetoside ? dv[8] : p.gb_sigma_sidetoside);
     C->npt = p.mc_ensemble_type == 2 ? 1 : 0;
     C->vcoll = C->colloid_on ? (4.0 / 3.0) * kPi * p.colloid_radius * p.colloid_radius * p.colloid_radius : 0.0;
@@ -199,6 +202,10 @@ int check_params(const mcgb_params& p, int r) {
     }
     if (p.adapt_moves < 0) {
         mcgb_set_error("replica %d: adapt_moves must be >= 0", r);
+        return MCGB_ERR_ARG;
+    }
+    if (p.sublattice_colors != 2 && p.sublattice_colors != 3) {
+        mcgb_set_error("replica %d: sublattice_colors must be 2 or 3 (got %d)", r, p.sublattice_colors);
         return MCGB_ERR_ARG;
     }
     return MCGB_OK;
@@ -335,15 +342,15 @@ size_t pass_lds(int waves, int cap) {
     return std::max(pass_lds_seq<Real>(waves, cap), pass_lds_spec<Real>(waves, cap));
 }
 
-// One sweep of every replica.  ev (may be null): 11 events recorded at the sweep start, after
-// the rebuild, after each of the 8 passes and at the end.
+// One sweep of every replica.  ev (may be null): npass + 3 events recorded at the sweep start,
+// after the rebuild, after each of the npass passes and at the end.
 template <typename Real>
 int enqueue_one(mcgb_ctx* c, mcgb_move_trace* trace, const hipEvent_t* ev) {
     int rc;
     if (ev) HIPCHK(hipEventRecord(ev[0], c->st));
     k_begin<<<(c->nrep + 63) / 64, 64, 0, c->st>>>(c->dC, c->dD, c->nrep, c->resum_interval);
     k_draws<<<dim3((c->rec_max + 255) / 256, c->nrep), 256, 0, c->st>>>(c->dC, c->dD, c->draws);
-    k_zero8<<<1, 64, 0, c->st>>>(c->ovf_count);
+    k_zero_passes<<<1, 64, 0, c->st>>>(c->ovf_count);
     if ((rc = enqueue_rebuild<Real>(c))) return rc;
     if (ev) HIPCHK(hipEventRecord(ev[1], c->st));
     PassArgs P;
@@ -363,7 +370,7 @@ int enqueue_one(mcgb_ctx* c, mcgb_move_trace* trace, const hipEvent_t* ev) {
     P.mlp = c->sur_M;
     P.da_hist = c->da_hist;
     P.da_judge = c->da_judge;
-    for (int p = 0; p < 8; ++p) {
+    for (int p = 0; p < c->npass; ++p) {
         P.pass = p;
         launch_pass<Real>(c, P);
         if (ev) HIPCHK(hipEventRecord(ev[2 + p], c->st));
@@ -373,7 +380,7 @@ int enqueue_one(mcgb_ctx* c, mcgb_move_trace* trace, const hipEvent_t* ev) {
     if ((rc = enqueue_energy<Real>(c, 1))) return rc;
     if (c->any_npt && (rc = enqueue_npt<Real>(c))) return rc;
     if ((rc = enqueue_observe<Real>(c, 1, 1))) return rc;
-    if (ev) HIPCHK(hipEventRecord(ev[10], c->st));
+    if (ev) HIPCHK(hipEventRecord(ev[2 + c->npass], c->st));
     return MCGB_OK;
 }
 
@@ -408,7 +415,7 @@ template <typename Real>
 int enqueue_sweeps(mcgb_ctx* c, int n_sweeps, mcgb_move_trace* trace, bool async = false) {
     int rc;
     const bool tm = c->timing && !async;
-    int timed_sets = 0;   // event sets (of 11) holding the last sweeps, in c->tev
+    int timed_sets = 0;   // event sets (npass + 3 events) holding the last sweeps, in c->tev
     if (n_sweeps <= 0) return MCGB_OK;
     // graph replay for the bulk; with timing on, the last two sweeps are launched one kernel
     // at a time between HIP events (events inside a graph cannot be timed)
@@ -427,26 +434,26 @@ int enqueue_sweeps(mcgb_ctx* c, int n_sweeps, mcgb_move_trace* trace, bool async
         for (int i = 0; i < bulk / 2; ++i) HIPCHK(hipGraphLaunch(c->graph[par], c->st));
     }
     for (int s = 0; s < tail; ++s)
-        if ((rc = enqueue_one<Real>(c, trace, c->tev + 11 * s))) return rc;
+        if ((rc = enqueue_one<Real>(c, trace, c->tev + kEvSet * s))) return rc;
     timed_sets = tail;
     if (async) return MCGB_OK;
     HIPCHK(hipStreamSynchronize(c->st));
     if (timed_sets > 0) {
         double tp = 0, tr = 0, tsw = 0;
         for (int k = 0; k < timed_sets; ++k) {
-            const hipEvent_t* e = c->tev + 11 * k;
+            const hipEvent_t* e = c->tev + kEvSet * k;
             float ms;
             HIPCHK(hipEventElapsedTime(&ms, e[0], e[1]));
             tr += ms;
-            for (int p = 0; p < 8; ++p) {
+            for (int p = 0; p < c->npass; ++p) {
                 HIPCHK(hipEventElapsedTime(&ms, e[1 + p], e[2 + p]));
                 tp += ms;
             }
-            HIPCHK(hipEventElapsedTime(&ms, e[0], e[10]));
+            HIPCHK(hipEventElapsedTime(&ms, e[0], e[2 + c->npass]));
             tsw += ms;
         }
         c->ms_rebuild = tr / timed_sets;
-        c->ms_pass = tp / (8.0 * timed_sets);
+        c->ms_pass = tp / ((double)c->npass * timed_sets);
         c->ms_sweep = tsw / timed_sets;
     }
     return MCGB_OK;
@@ -504,7 +511,7 @@ void mcgb_destroy(mcgb_ctx* c) {
     if (c->st) (void)hipStreamSynchronize(c->st);
     for (int k = 0; k < 2; ++k)
         if (c->graph[k]) (void)hipGraphExecDestroy(c->graph[k]);
-    for (int k = 0; k < 22; ++k)
+    for (int k = 0; k < 2 * kEvSet; ++k)
         if (c->tev[k]) (void)hipEventDestroy(c->tev[k]);
     for (double* h : c->snap_host)
         if (h) (void)hipHostFree(h);
@@ -548,7 +555,7 @@ int mcgb_create(const mcgb_params* p, int n_replicas, mcgb_ctx** out) {
     c->resum_interval = p[0].energy_check_interval;
     HIPCHK(hipGetDevice(&c->device));
     HIPCHK(hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking));
-    for (int k = 0; k < 22; ++k) HIPCHK(hipEventCreate(&c->tev[k]));
+    for (int k = 0; k < 2 * kEvSet; ++k) HIPCHK(hipEventCreate(&c->tev[k]));
     // MCGB_NO_GRAPH=1: launch every sweep kernel by kernel (for profilers and debuggers that
     // do not follow graph launches); results are identical
     {
@@ -570,8 +577,8 @@ int mcgb_create(const mcgb_params* p, int n_replicas, mcgb_ctx** out) {
         double Lcap = C.npt ? L * 1.3 : L;
         int ncell = 1;
         for (int d = 0; d < 3; ++d) {
-            int n = mcgb_cell_dim(Lcap, pr.cutoff_radius);
-            if (n == 0) {
+            int n = grid_dim(Lcap, pr.cutoff_radius, C.col);
+            if (n == 0 || mcgb_cell_dim(Lcap, pr.cutoff_radius) == 0) {
                 mcgb_set_error("replica %d: box %.4f is below 4 cells of the cutoff %.4f (8-sublattice "
                                "grid impossible)", r, L, pr.cutoff_radius);
                 mcgb_destroy(c);
@@ -590,7 +597,7 @@ int mcgb_create(const mcgb_params* p, int n_replicas, mcgb_ctx** out) {
         C.cell_off = c->ncell_tot;
         C.cs_off = c->ncs_tot;
         C.act_off = c->nblk;
-        C.nact_cap = ncell / 8;
+        C.nact_cap = ncell / (C.col * C.col * C.col);
         C.key0 = (uint32_t)pr.seed;
         C.key1 = (uint32_t)(pr.seed >> 32) + (uint32_t)r;
         C.rec_off = c->Ntot + c->ncell_tot;
@@ -603,7 +610,7 @@ int mcgb_create(const mcgb_params* p, int n_replicas, mcgb_ctx** out) {
         for (int b = 0; b < C.nact_cap; ++b) c->blk_rep.push_back(r);
         c->nblk += C.nact_cap;
         c->any_npt = c->any_npt || C.npt;
-        int n0 = mcgb_cell_dim(L, pr.cutoff_radius);
+        int n0 = grid_dim(L, pr.cutoff_radius, C.col);
         double occ = (double)C.N / ((double)n0 * n0 * n0);
         occ_max = std::max(occ_max, occ);
         RepDyn& Dr = D[r];
@@ -612,15 +619,25 @@ int mcgb_create(const mcgb_params* p, int n_replicas, mcgb_ctx** out) {
         Dr.dmax = pr.max_displacement;
         Dr.drot = pr.max_rotation;
     }
-    // wavefronts per active cell and LDS staging capacity (1.15x the mean 3x3x3 stencil plus
-    // 64, within 64 KiB; a larger stencil takes the HBM path)
+    // sublattice colours (one value for the whole batch: the passes are shared launches)
+    c->col = p[0].sublattice_colors == 3 ? 3 : 2;
+    c->npass = c->col * c->col * c->col;
+    for (int r = 1; r < n_replicas; ++r)
+        if (p[r].sublattice_colors != p[0].sublattice_colors) {
+            mcgb_set_error("replica %d: all replicas must share one sublattice_colors", r);
+            mcgb_destroy(c);
+            return MCGB_ERR_ARG;
+        }
+    // wavefronts per active cell and LDS staging capacity (1.15x the mean stencil, 27 or 125
+    // cells, plus 64, within 64 KiB; a larger stencil takes the HBM path)
     c->waves = p[0].waves_per_cell;
     if (c->waves != 1 && c->waves != 2 && c->waves != 3 && c->waves != 4 && c->waves != 8) {
         mcgb_set_error("waves_per_cell must be 1, 2, 3, 4 or 8 (got %d)", c->waves);
         mcgb_destroy(c);
         return MCGB_ERR_ARG;
     }
-    int cap = (int)(27.0 * occ_max * 1.15 + 64.0);
+    const double ncells_stencil = c->col == 3 ? 125.0 : 27.0;
+    int cap = (int)(ncells_stencil * occ_max * 1.15 + 64.0);
     cap = (cap + 31) & ~31;
     if (p[0].stage_capacity > 0) cap = (p[0].stage_capacity + 31) & ~31;
     cap = std::max(32, cap);
@@ -630,7 +647,7 @@ int mcgb_create(const mcgb_params* p, int n_replicas, mcgb_ctx** out) {
     c->cap = cap;
     c->lds = c->rsize == 8 ? pass_lds_seq<double>(c->waves, cap) : pass_lds_seq<float>(c->waves, cap);
     c->lds_spec = c->rsize == 8 ? pass_lds_spec<double>(c->waves, cap) : pass_lds_spec<float>(c->waves, cap);
-    c->spec = p[0].pass_schedule != 0;
+    c->spec = p[0].pass_schedule != 0 && c->col == 2;    // the speculative pass: 8 sublattices
 
     // ---- device memory
     int rc = 0;
@@ -648,8 +665,8 @@ int mcgb_create(const mcgb_params* p, int n_replicas, mcgb_ctx** out) {
     if (!rc) rc = dalloc_t(c, &c->cell_start2, c->ncs_tot);
     if (!rc) rc = dalloc_t(c, &c->acc, c->ncell_tot);
     if (!rc) rc = dalloc_t(c, &c->draws, (size_t)c->Ntot + c->ncell_tot);
-    if (!rc) rc = dalloc_t(c, &c->ovf_count, 8);
-    if (!rc) rc = dalloc_t(c, &c->ovf_list, (size_t)8 * c->nblk);
+    if (!rc) rc = dalloc_t(c, &c->ovf_count, 27);
+    if (!rc) rc = dalloc_t(c, &c->ovf_list, (size_t)c->npass * c->nblk);
     if (!rc) rc = dalloc_t(c, &c->efin, N);
     if (!rc) rc = dalloc_t(c, &c->ecap, N);
     if (!rc) rc = dalloc_t(c, &c->eccap, N);
@@ -1007,7 +1024,7 @@ int mcgb_sweep(mcgb_ctx* c, int n_sweeps) {
     return sweep_impl(c, n_sweeps, nullptr);
 }
 
-int mcgb_sweep_traced(mcgb_ctx* c, mcgb_move_trace* trace, int capacity, uint32_t info[13]) {
+int mcgb_sweep_traced(mcgb_ctx* c, mcgb_move_trace* trace, int capacity, uint32_t info[36]) {
     int rc = check_ctx(c, 0);
     if (rc) return rc;
     if (!trace || capacity <= 0) {
@@ -1044,10 +1061,12 @@ int mcgb_sweep_traced(mcgb_ctx* c, mcgb_move_trace* trace, int capacity, uint32_
     }
     HIPCHK(hipMemcpy(trace, c->dtrace, nrec * sizeof(mcgb_move_trace), hipMemcpyDeviceToHost));
     if (info) {
+        memset(info, 0, 36 * sizeof(uint32_t));
         for (int d = 0; d < 3; ++d) info[d] = D[0].g[d];
-        for (int p = 0; p < 8; ++p) info[3 + p] = D[0].order[p];
-        info[11] = (uint32_t)D[0].k;
-        info[12] = (uint32_t)D[0].ncell;
+        info[3] = (uint32_t)D[0].k;
+        info[4] = (uint32_t)D[0].ncell;
+        info[5] = (uint32_t)c->npass;
+        for (int p = 0; p < c->npass; ++p) info[6 + p] = D[0].order[p];
     }
     return MCGB_OK;
 }
@@ -1241,6 +1260,11 @@ int mcgb_autotune(mcgb_ctx* c, int sweeps, int* best_waves, int* best_schedule,
     bool bs = c->spec;
     for (int sched = 0; sched < 2; ++sched) {
         c->spec = sched == 1;
+        if (c->spec && c->col != 2) {           // the speculative pass runs 8 sublattices only
+            if (ms_per_pass)
+                for (int k = 0; k < 5; ++k) ms_per_pass[5 * sched + k] = 0.0;
+            continue;
+        }
         for (int k = 0; k < 5; ++k) {
             set_waves(c, cand[k]);
             if ((rc = sweep_impl(c, sweeps, nullptr))) { c->timing = tm; return rc; }
@@ -1265,7 +1289,7 @@ int mcgb_kernel_times(mcgb_ctx* c, double* ms_pass, double* ms_rebuild, double* 
 }
 
 int mcgb_cell_list(mcgb_ctx* c, int r, uint32_t t, int* cell_start, int cell_start_len,
-                   int* pid_sorted, int n, uint32_t shift[3], uint32_t order[8], int dims[3]) {
+                   int* pid_sorted, int n, uint32_t shift[3], uint32_t order[27], int dims[3]) {
     int rc = check_ctx(c, r);
     if (rc) return rc;
     const RepConst& C = c->hC[r];
@@ -1288,7 +1312,7 @@ int mcgb_cell_list(mcgb_ctx* c, int r, uint32_t t, int* cell_start, int cell_sta
     HIPCHK(hipMemcpy(cell_start, c->cell_start + C.cs_off, (ncell + 1) * sizeof(int), hipMemcpyDeviceToHost));
     HIPCHK(hipMemcpy(pid_sorted, c->pid[c->cur] + C.part_off, n * sizeof(int), hipMemcpyDeviceToHost));
     if (shift) for (int d = 0; d < 3; ++d) shift[d] = D2[r].g[d];
-    if (order) for (int p = 0; p < 8; ++p) order[p] = D2[r].order[p];
+    if (order) for (int p = 0; p < 27; ++p) order[p] = p < c->npass ? D2[r].order[p] : 0u;
     if (dims) for (int d = 0; d < 3; ++d) dims[d] = D2[r].n[d];
     D2[r].t = t_saved;
     return write_dyn(c, D2);

@@ -36,6 +36,7 @@ struct RepConst {
     double R, R2, colloid_cut, kappa_prime, one_minus_chi, one_minus_chip;
     double npt_max_dlnv;
     int adapt_moves;       // step-size adaptation threshold (attempted moves), 0 = off
+    int col;               // sublattice colours per axis (2 or 3): col^3 passes per sweep
     float min_thick;       // min(sigma_ee, sigma_ss) as the reference's float (:833-834)
     double vcoll;          // 4/3 pi R^3 (density convention, MCnptColloid.c:2799-2804)
     // FP32 copies for the pair math
@@ -54,7 +55,7 @@ struct RepDyn {
     long long K[3];              // fixed-point step amplitude round(d_max 2^32 / L)
     uint32_t t;                  // sweep index
     uint32_t g[3];               // grid shift
-    uint32_t order[8];           // sublattice order
+    uint32_t order[27];          // sublattice order (col^3 entries)
     uint32_t vol[2];             // NPT draws
     long long cap_pairs, cap_coll;
     double e_fin;                // running energy = cap_pairs*1e9 + cap_coll*1e8 + e_fin
@@ -146,6 +147,15 @@ __host__ __device__ inline uint32_t cell_coord(uint32_t q, uint32_t g, uint32_t 
 __host__ __device__ inline int cell_dim(double L, double rc) {
     int n = 2 * (int)floor(L / (2.0 * rc * (1.0 + 1e-6)));
     return n >= 4 ? n : 0;
+}
+
+// Sweep grid for col sublattice colours per axis: col = 2 -> cell_dim; col = 3 -> the largest
+// multiple of 3 with L/n >= r_c/2 (n >= 6).  Cells of one colour are col - 1 cells (>= r_c)
+// apart, and a move's partners lie within +-(col - 1) cells.
+__host__ __device__ inline int grid_dim(double L, double rc, int col) {
+    if (col != 3) return cell_dim(L, rc);
+    int n = 3 * (int)floor(L / (1.5 * rc * (1.0 + 1e-6)));
+    return n >= 6 ? n : 0;
 }
 
 // ------------------------------------------------------------------ GB pair, FP32

@@ -50,6 +50,7 @@ void mcgb_params_default(mcgb_params *p)
     p->waves_per_cell = 2;
     p->pass_schedule = 1;
     p->adapt_moves = 0;
+    p->sublattice_colors = 2;
 }
 
 enum { T_INT, T_DOUBLE, T_FLOAT, T_U64 };
@@ -114,6 +115,7 @@ int mcgb_params_load(const char *path, mcgb_params *out)
         {"waves_per_cell", T_INT, &p.waves_per_cell},
         {"pass_schedule", T_INT, &p.pass_schedule},
         {"adapt_moves", T_INT, &p.adapt_moves},
+        {"sublattice_colors", T_INT, &p.sublattice_colors},
     };
     const int nkeys = (int)(sizeof(keys) / sizeof(keys[0]));
     char line[256], name[256], value[256];

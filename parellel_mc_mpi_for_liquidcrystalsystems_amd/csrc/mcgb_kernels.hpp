@@ -2,7 +2,7 @@
 //
 // One sweep of every replica is the launch sequence
 //   k_begin -> k_zero -> k_count -> k_scan -> k_scatter -> k_sort_gather   (cell-list rebuild)
-//   -> k_pass x 8 (sublattice passes) -> k_end_moves -> k_energy/k_energy_reduce (re-sum, only
+//   -> k_pass x P (sublattice passes, P = 8 or 27) -> k_end_moves -> k_energy/k_energy_reduce (re-sum, only
 //   when flagged) -> [NPT: trial cell list, k_energy, k_volume] -> k_observe -> k_observe_reduce.
 // Nothing in it synchronises with the host.
 #pragma once
@@ -82,23 +82,30 @@ __global__ void k_begin(const RepConst* __restrict__ RC, RepDyn* __restrict__ RD
     const RepConst& C = RC[r];
     RepDyn& D = RD[r];
     const uint32_t t = D.t;
-    uint32_t s0[4], s1[4], s2[4];
-    philox4x32_10(0u, 0u, t, kSetupTag, C.key0, C.key1, s0);
-    philox4x32_10(1u, 0u, t, kSetupTag, C.key0, C.key1, s1);
-    philox4x32_10(2u, 0u, t, kSetupTag, C.key0, C.key1, s2);
-    D.g[0] = s0[0]; D.g[1] = s0[1]; D.g[2] = s0[2];
-    D.vol[0] = s0[3]; D.vol[1] = s2[3];
-    // shuffle_int_array (MCnptColloid.c:3721-3737) with Philox draws
-    uint32_t ord[8] = {0, 1, 2, 3, 4, 5, 6, 7};
-    uint32_t dr[7] = {s1[0], s1[1], s1[2], s1[3], s2[0], s2[1], s2[2]};
-    for (int i = 7, d = 0; i >= 1; --i, ++d) {
+    const int P = C.col * C.col * C.col;
+    uint32_t s[8][4];
+    const int nblk = P == 27 ? 8 : 3;                   // set-up Philox blocks needed
+    for (int j = 0; j < nblk; ++j) philox4x32_10((uint32_t)j, 0u, t, kSetupTag, C.key0, C.key1, s[j]);
+    D.g[0] = s[0][0]; D.g[1] = s[0][1]; D.g[2] = s[0][2];
+    D.vol[0] = s[0][3]; D.vol[1] = s[2][3];
+    // shuffle_int_array (MCnptColloid.c:3721-3737) with Philox draws: S_1[0..3], S_2[0..2],
+    // then S_3..S_7 in order
+    uint32_t dr[26];
+    int nd = 0;
+    for (int w = 0; w < 4; ++w) dr[nd++] = s[1][w];
+    for (int w = 0; w < 3; ++w) dr[nd++] = s[2][w];
+    for (int j = 3; j < nblk; ++j)
+        for (int w = 0; w < 4 && nd < 26; ++w) dr[nd++] = s[j][w];
+    uint32_t ord[27];
+    for (int i = 0; i < P; ++i) ord[i] = (uint32_t)i;
+    for (int i = P - 1, d = 0; i >= 1; --i, ++d) {
         uint32_t j = (uint32_t)(((uint64_t)dr[d] * (uint64_t)(i + 1)) >> 32);
         uint32_t tmp = ord[i]; ord[i] = ord[j]; ord[j] = tmp;
     }
-    for (int i = 0; i < 8; ++i) D.order[i] = ord[i];
+    for (int i = 0; i < P; ++i) D.order[i] = ord[i];
     int nc = 1;
     for (int d = 0; d < 3; ++d) {
-        int n = cell_dim(D.L[d], C.rc);
+        int n = grid_dim(D.L[d], C.rc, C.col);
         if (n > C.ncap[d]) n = C.ncap[d];
         D.n[d] = n;
         nc *= n;
@@ -106,14 +113,14 @@ __global__ void k_begin(const RepConst* __restrict__ RC, RepDyn* __restrict__ RD
         D.K[d] = step_amplitude(D.dmax, D.L[d]);
     }
     D.ncell = nc;
-    D.nact = nc / 8;
+    D.nact = nc / P;
     D.k = (C.N + nc - 1) / nc;
     D.moved_abs = 0.0;
     D.resum = (resum_interval > 0 && ((t + 1) % (uint32_t)resum_interval) == 0) ? 1 : 0;
 }
 
-__global__ void k_zero8(int* __restrict__ p) {
-    if (threadIdx.x < 8) p[threadIdx.x] = 0;
+__global__ void k_zero_passes(int* __restrict__ p) {
+    if (threadIdx.x < 27) p[threadIdx.x] = 0;
 }
 
 // Host-requested cell list with zero shift (set_state, explicit recompute): grid from D.L.
@@ -124,7 +131,7 @@ __global__ void k_grid_g0(const RepConst* __restrict__ RC, RepDyn* __restrict__ 
     RepDyn& D = RD[r];
     int nc = 1;
     for (int d = 0; d < 3; ++d) {
-        int n = cell_dim(D.L[d], C.rc);
+        int n = grid_dim(D.L[d], C.rc, C.col);
         if (n > C.ncap[d]) n = C.ncap[d];
         D.n[d] = n;
         nc *= n;
@@ -133,7 +140,7 @@ __global__ void k_grid_g0(const RepConst* __restrict__ RC, RepDyn* __restrict__ 
         D.g[d] = 0u;
     }
     D.ncell = nc;
-    D.nact = nc / 8;
+    D.nact = nc / (C.col * C.col * C.col);
     D.k = (C.N + nc - 1) / nc;
 }
 
@@ -279,7 +286,8 @@ __global__ __launch_bounds__(256) void k_draws(const RepConst* __restrict__ RC,
     const RepConst& C = RC[r];
     const RepDyn& D = RD[r];
     const int nact = D.nact, k = D.k;
-    const int total = 8 * nact * k;
+    const int col = C.col;
+    const int total = col * col * col * nact * k;
     const int idx = blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= total) return;
     const int p = idx / (nact * k);
@@ -287,10 +295,11 @@ __global__ __launch_bounds__(256) void k_draws(const RepConst* __restrict__ RC,
     const int a = rem / k;
     const int m = rem - a * k;
     const int nx = D.n[0], ny = D.n[1];
-    const int hx = nx >> 1, hy = ny >> 1;
+    const int hx = nx / col, hy = ny / col;
     const uint32_t sig = D.order[p];
-    const int cx = 2 * (a % hx) + (int)(sig & 1u), cy = 2 * ((a / hx) % hy) + (int)((sig >> 1) & 1u),
-              cz = 2 * (a / (hx * hy)) + (int)((sig >> 2) & 1u);
+    const int cx = col * (a % hx) + (int)(sig % (uint32_t)col),
+              cy = col * ((a / hx) % hy) + (int)((sig / (uint32_t)col) % (uint32_t)col),
+              cz = col * (a / (hx * hy)) + (int)(sig / (uint32_t)(col * col));
     const uint32_t cell = (uint32_t)(cx + nx * (cy + ny * cz));
     uint32_t A[4], B[4];
     philox4x32_10((uint32_t)m, cell, D.t, (uint32_t)p << 8, C.key0, C.key1, A);
@@ -378,7 +387,7 @@ __global__ __launch_bounds__(256) void k_end_moves(const RepConst* __restrict__ 
 // re-summation; 2 the NPT trial box (indirect list, zero shift).  tot_ene,
 // MCnptColloid.c:1598-1622.
 //
-// Each lane owns one particle and walks its 27 neighbour cells; in-cutoff partners are
+// Each lane owns one particle and walks its (2R+1)^3 neighbour cells; in-cutoff partners are
 // compacted (ballot) into a wavefront-private ring in LDS and evaluated 64 at a time with every
 // lane busy, so the FP64 pair body never runs under divergence.  Every lane's running sums and
 // the final shuffle tree are in a fixed order: the result is deterministic.
@@ -454,10 +463,14 @@ __global__ __launch_bounds__(128) void k_energy(
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     };
 
-    for (int c = 0; c < 27; ++c) {
+    // stencil reach: +-1 cell on grids of cells >= r_c (8 colours, the NPT trial grid), +-2 on
+    // the 27-colour grid of cells >= r_c/2
+    const int R = ind ? 1 : C.col - 1;
+    const int S1 = 2 * R + 1, NS = S1 * S1 * S1;
+    for (int c = 0; c < NS; ++c) {
         int st = 0, len = 0;
         if (act) {
-            int ex = cx + (c % 3) - 1, ey = cy + ((c / 3) % 3) - 1, ez = cz + c / 9 - 1;
+            int ex = cx + (c % S1) - R, ey = cy + ((c / S1) % S1) - R, ez = cz + c / (S1 * S1) - R;
             ex += ex < 0 ? n0 : 0; ex -= ex >= n0 ? n0 : 0;
             ey += ey < 0 ? n1 : 0; ey -= ey >= n1 ? n1 : 0;
             ez += ez < 0 ? n2 : 0; ez -= ez >= n2 ? n2 : 0;

@@ -2,8 +2,8 @@
 // trial moves in sequence (the TODO body of MCnptColloid.c:683-700 filled with the move of
 // attempt_monte_carlo_move, :2445-2604).
 //
-// Per activation: the 3x3x3 stencil (27 cell segments of the device cell list) is staged in LDS
-// as SoA; the draws of the cell's moves (written by k_draws) are read in chunks of 64.  Per move: every
+// Per activation: the stencil ((2R+1)^3 cell segments of the device cell list, R = 1 for 8
+// sublattices, 2 for 27) is staged in LDS; the draws of the cell's moves (written by k_draws) are read in chunks of 64.  Per move: every
 // thread forms the trial state; each wavefront tests its share of the stencil against the old
 // and trial positions (FP32 pre-filter), compacts the hits into a wavefront-private LDS list,
 // evaluates both GB energies of every listed pair (cutoff decided on the FP64 r^2), and reduces
@@ -115,9 +115,10 @@ struct PassLds {
     __host__ __device__ static inline int stage_cap(int cap) { return ((cap + 63) / 64) * 64; }
     // per-wavefront candidate list: list_cap entries and a slot that unselected lanes write
     __host__ __device__ static inline int list_stride(int cap) { return list_cap(cap) + 8; }
-    // byte offsets: stencil tables 0, positions 256, orientations, lists, draws (16-aligned);
-    // 12 * stage_cap is a multiple of 768, so every region keeps its alignment
-    __host__ __device__ static inline int off_su(int cap) { return 256 + 12 * stage_cap(cap); }
+    // byte offsets: stencil tables 0, positions kHeader, orientations, lists, draws
+    // (16-aligned); 12 * stage_cap is a multiple of 768, so every region keeps its alignment
+    static constexpr int kHeader = 1024;                 // s_start[128], s_off[128]
+    __host__ __device__ static inline int off_su(int cap) { return kHeader + 12 * stage_cap(cap); }
     __host__ __device__ static inline int off_list(int cap) {
         return off_su(cap) + cap * 3 * (int)sizeof(Real);
     }
@@ -283,9 +284,10 @@ __device__ inline MidFilter mid_filter(uint32_t qix, uint32_t qiy, uint32_t qiz,
     return M;
 }
 
-// Flat stencil index e -> global particle slot (segments of the 27 neighbour cells).
-__device__ inline int stencil_slot(const int* s_off, const int* s_start, int part_off, int e) {
-    int lo = 0, hi = 27;
+// Flat stencil index e -> global particle slot (segments of the ns neighbour cells).
+__device__ inline int stencil_slot(const int* s_off, const int* s_start, int part_off, int e,
+                                   int ns = 27) {
+    int lo = 0, hi = ns;
     while (hi - lo > 1) {
         int mid = (lo + hi) >> 1;
         if (s_off[mid] <= e) lo = mid; else hi = mid;
@@ -330,10 +332,10 @@ __global__ __launch_bounds__(64 * W) void k_pass(PassArgs P) {
     using L = PassLds<Real, W>;
     const int lcap = L::list_cap(cap);
 
-    int* s_start = (int*)smem;                                 // 32
-    int* s_off = s_start + 32;                                 // 32 (28 used)
+    int* s_start = (int*)smem;                                 // 128 (<= 125 used)
+    int* s_off = s_start + 128;                                // 128 (<= 126 used)
     using R3 = Vec3<Real>;
-    Pos3* sq = (Pos3*)(smem + 256);                            // stage_cap positions
+    Pos3* sq = (Pos3*)(smem + L::kHeader);                     // stage_cap positions
     R3* su = (R3*)(smem + L::off_su(cap));                     // cap orientations
     unsigned short* slist = (unsigned short*)(smem + L::off_list(cap)) + w * L::list_stride(cap);
     unsigned char* dp = smem + L::off_draws(cap);
@@ -360,42 +362,51 @@ __global__ __launch_bounds__(64 * W) void k_pass(PassArgs P) {
     const double R2 = C.R2;
     const double drot = (double)D.drot;
     const int nx = D.n[0], ny = D.n[1], nz = D.n[2];
-    const int hx = nx >> 1, hy = ny >> 1;
+    const int col = C.col;
+    const int hx = nx / col, hy = ny / col;
     const uint32_t sig = D.order[P.pass];
     const int ax = a % hx, ay = (a / hx) % hy, az = a / (hx * hy);
-    const int cx = 2 * ax + (int)(sig & 1u), cy = 2 * ay + (int)((sig >> 1) & 1u),
-              cz = 2 * az + (int)((sig >> 2) & 1u);
+    const int cx = col * ax + (int)(sig % (uint32_t)col),
+              cy = col * ay + (int)((sig / (uint32_t)col) % (uint32_t)col),
+              cz = col * az + (int)(sig / (uint32_t)(col * col));
     const int cell = cx + nx * (cy + ny * cz);
     const int* cs = P.cell_start + C.cs_off;
     const uint32_t gx = D.g[0], gy = D.g[1], gz = D.g[2];
 
-    // stencil table: 27 neighbour cells in (dz, dy, dx) order, centre = 13
+    // stencil table: NS = (2R+1)^3 neighbour cells in (dz, dy, dx) order, the active cell at
+    // NS/2; two cells per lane, one inclusive scan per half
+    const int RS = col - 1, S1 = 2 * RS + 1, NS = S1 * S1 * S1;
     if (w == 0) {
-        int cnt = 0;
-        if (lane < 27) {
-            int ex = cx + (lane % 3) - 1, ey = cy + ((lane / 3) % 3) - 1, ez = cz + lane / 9 - 1;
-            ex += ex < 0 ? nx : 0; ex -= ex >= nx ? nx : 0;
-            ey += ey < 0 ? ny : 0; ey -= ey >= ny ? ny : 0;
-            ez += ez < 0 ? nz : 0; ez -= ez >= nz ? nz : 0;
-            int nb = ex + nx * (ey + ny * ez);
-            int st = cs[nb];
-            cnt = cs[nb + 1] - st;
-            s_start[lane] = st;
+        int cnt[2] = {0, 0};
+        for (int h = 0; h < 2; ++h) {
+            const int c = lane + 64 * h;
+            if (c < NS) {
+                int ex = cx + (c % S1) - RS, ey = cy + ((c / S1) % S1) - RS, ez = cz + c / (S1 * S1) - RS;
+                ex += ex < 0 ? nx : 0; ex -= ex >= nx ? nx : 0;
+                ey += ey < 0 ? ny : 0; ey -= ey >= ny ? ny : 0;
+                ez += ez < 0 ? nz : 0; ez -= ez >= nz ? nz : 0;
+                const int nb = ex + nx * (ey + ny * ez);
+                const int st = cs[nb];
+                cnt[h] = cs[nb + 1] - st;
+                s_start[c] = st;
+            }
         }
-        int incl = cnt;
+        int i0 = cnt[0], i1 = cnt[1];
 #pragma unroll
-        for (int off = 1; off < 32; off <<= 1) {
-            int v = __shfl_up(incl, off);
-            if (lane >= off) incl += v;
+        for (int off = 1; off < 64; off <<= 1) {
+            const int v0 = __shfl_up(i0, off), v1 = __shfl_up(i1, off);
+            if (lane >= off) { i0 += v0; i1 += v1; }
         }
-        if (lane < 27) s_off[lane + 1] = incl;
+        const int tot0 = __shfl(i0, 63);
+        if (lane < NS) s_off[lane + 1] = i0;
+        if (lane + 64 < NS) s_off[lane + 65] = tot0 + i1;
         if (lane == 0) s_off[0] = 0;
     }
     __syncthreads();
-    const int S = s_off[27];
-    const int coff = s_off[13];
-    const int nc = s_off[14] - coff;
-    const int cstart = part_off + s_start[13];
+    const int S = s_off[NS];
+    const int coff = s_off[NS / 2];
+    const int nc = s_off[NS / 2 + 1] - coff;
+    const int cstart = part_off + s_start[NS / 2];
     constexpr bool staged = !GLOBAL;
     if (!GLOBAL && S > cap) {                  // uniform: leave this cell to the HBM kernel
         if (tid == 0) {
@@ -406,13 +417,15 @@ __global__ __launch_bounds__(64 * W) void k_pass(PassArgs P) {
         return;
     }
     // the stencil padded to whole wavefronts with a sentinel: the point opposite the centre of
-    // the active cell (half a box away on every axis).  With >= 4 cells per axis it lies
-    // >= 1.5 cells from any cell member per axis, beyond every move's filter radius
-    // (<= r_c + half a cell diagonal, trial positions being confined to the cell).
+    // the active cell (half a box away on every axis).  With >= 4 cells of >= r_c per axis (8
+    // sublattices) it lies >= 1.5 cells (>= 1.5 r_c) from any cell member per axis, with >= 6
+    // cells of >= r_c/2 (27 sublattices) >= 2.5 cells (>= 1.25 r_c): in both cases beyond every
+    // move's filter radius (<= r_c + half a cell diagonal, trial positions being confined to
+    // the cell).
     const int S_pad = (S + 63) & ~63;
     if (staged) {
         for (int e = tid; e < S; e += 64 * W) {
-            int gi = stencil_slot(s_off, s_start, part_off, e);
+            int gi = stencil_slot(s_off, s_start, part_off, e, NS);
             sq[e] = Pos3{P.qx[gi], P.qy[gi], P.qz[gi]};
             su[e] = R3{Ux[gi], Uy[gi], Uz[gi]};
         }
@@ -644,7 +657,7 @@ __global__ __launch_bounds__(64 * W) void k_pass(PassArgs P) {
                         // candidate, no compaction
                         for (int e = tid; e < S; e += 64 * W) {
                             if (e == li) continue;
-                            const int gj = stencil_slot(s_off, s_start, part_off, e);
+                            const int gj = stencil_slot(s_off, s_start, part_off, e, NS);
                             const uint32_t jx = P.qx[gj], jy = P.qy[gj], jz = P.qz[gj];
                             const Real ujx = Ux[gj], ujy = Uy[gj], ujz = Uz[gj];
                             for (int st = 0; st < 2; ++st) {

@@ -47,7 +47,7 @@ class Params(C.Structure):
         ("energy_check_interval", C.c_int), ("lattice_jitter", C.c_double),
         ("colloid_exclusion_radius", C.c_double), ("stage_capacity", C.c_int),
         ("waves_per_cell", C.c_int), ("pass_schedule", C.c_int), ("adapt_moves", C.c_int),
-        ("keys_read", C.c_uint32)]
+        ("sublattice_colors", C.c_int), ("keys_read", C.c_uint32)]
 
 
 class MoveTrace(C.Structure):
@@ -382,9 +382,9 @@ class Context:
         p = self.params[0]
         cap = p.num_particles + 4 * p.num_particles + 100000
         buf = (MoveTrace * cap)()
-        info = np.zeros(13, dtype=np.uint32)
+        info = np.zeros(36, dtype=np.uint32)
         check(lib().mcgb_sweep_traced(self.h, buf, cap, _u(info)))
-        nrec = int(info[11]) * int(info[12])
+        nrec = int(info[3]) * int(info[4])
         arr = np.frombuffer(buf, dtype=TRACE_DTYPE, count=nrec).copy()
         return arr, info
 
@@ -439,12 +439,13 @@ class Context:
         cs = np.empty(2_000_000 if n > 100000 else 200000, dtype=np.int32)
         ids = np.empty(n, dtype=np.int32)
         shift = np.empty(3, dtype=np.uint32)
-        order = np.empty(8, dtype=np.uint32)
+        order = np.empty(27, dtype=np.uint32)
         dims = np.empty(3, dtype=np.int32)
         check(lib().mcgb_cell_list(self.h, replica, t, _i(cs), cs.shape[0], _i(ids), n, _u(shift),
                                    _u(order), _i(dims)))
         nc = int(np.prod(dims))
-        return cs[:nc + 1].copy(), ids, shift, order, dims
+        npass = 27 if p.sublattice_colors == 3 else 8
+        return cs[:nc + 1].copy(), ids, shift, order[:npass], dims
 
     # ------------------------------------------------------------ learned first stage
 

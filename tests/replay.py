@@ -87,24 +87,27 @@ def replay_sweep(c, q, u, L, seed, replica, t, trace, info, check_tol=1e-9, fp32
     q = [a.astype(np.uint32).copy() for a in q]
     u = [a.astype(np.float64).copy() for a in u]
     N = q[0].shape[0]
-    n = [oc.cell_dim(L[d], c.cutoff) for d in range(3)]
+    col = 3 if c.colors == 3 else 2
+    P = col ** 3
+    n = [oc.grid_dim(L[d], c.cutoff, col) for d in range(3)]
     nc = n[0] * n[1] * n[2]
     k = -(-N // nc)
-    g, order, _ = oc.sweep_setup(seed, replica, t)
-    assert list(info[:3]) == list(g) and list(info[3:11]) == list(order)
-    assert info[11] == k and info[12] == nc
+    g, order, _ = oc.sweep_setup(seed, replica, t, col)
+    assert list(info[:3]) == list(g) and list(info[6:6 + P]) == list(order)
+    assert info[3] == k and info[4] == nc and info[5] == P
     cs, so = oc.cell_list(q, np.array(n, dtype=np.int32), g)
     key = [seed & 0xFFFFFFFF, ((seed >> 32) + replica) & 0xFFFFFFFF]
     K = [int(np.floor(c.max_displacement * TWO32 / L[d] + 0.5)) for d in range(3)]
     s = [L[d] / TWO32 for d in range(3)]
-    hx, hy, hz = n[0] // 2, n[1] // 2, n[2] // 2
+    hx, hy, hz = n[0] // col, n[1] // col, n[2] // col
     nact = hx * hy * hz
     stats = {"checked": 0, "accepted": 0}
-    for p in range(8):
+    for p in range(P):
         sig = int(order[p])
+        sx, sy, sz = sig % col, (sig // col) % col, sig // (col * col)
         for a in range(nact):
             ax, ay, az = a % hx, (a // hx) % hy, a // (hx * hy)
-            cx, cy, cz = 2 * ax + (sig & 1), 2 * ay + ((sig >> 1) & 1), 2 * az + ((sig >> 2) & 1)
+            cx, cy, cz = col * ax + sx, col * ay + sy, col * az + sz
             cell = cx + n[0] * (cy + n[1] * cz)
             members = so[cs[cell]:cs[cell + 1]]
             for m in range(k):

@@ -122,14 +122,15 @@ def test_colloid_matches_reference_vectors(name, R):
 
 
 def make_pair(N=1300, rho=0.25, R=0.0, T=1.0, precision=64, seed=5, resum=0, nthreads=4,
-              stage_capacity=0, start="layered"):
+              stage_capacity=0, start="layered", colors=2, **kw):
     if start == "layered":
         q, u, L = layered_system(N, rho, colloid_radius=R, seed=seed)
     else:
         q, u, L = lattice_system(N, rho, colloid_radius=R, seed=seed)
     p = m.gb_params(N, rho, T, colloid_radius=R, precision=precision, seed=900 + seed,
-                    energy_check_interval=resum, stage_capacity=stage_capacity)
-    c = oc.consts(temperature=T, colloid_radius=R, colloid_on=1 if R > 0 else 0)
+                    energy_check_interval=resum, stage_capacity=stage_capacity,
+                    sublattice_colors=colors, **kw)
+    c = oc.consts(temperature=T, colloid_radius=R, colloid_on=1 if R > 0 else 0, colors=colors)
     s = oc.System(c, q, u, L, seed=900 + seed, nthreads=nthreads, resum_interval=resum)
     ctx = m.Context(p)
     ctx.set_state_fixed(q, u, L)

@@ -161,6 +161,6 @@ def test_sweep_setup_order_is_permutation():
     seen = set()
     for t in range(200):
         g, order, vol = oc.sweep_setup(42, 0, t)
-        assert sorted(order.tolist()) == list(range(8))
+        assert sorted(order.tolist()) == list(range(8)) and len(order) == 8
         seen.add(tuple(order.tolist()))
     assert len(seen) > 150        # the shuffle really shuffles

@@ -8,15 +8,17 @@ from tests.replay import replay_sweep
 from tests.systems import lattice_system
 
 
-def make(colloid=0.0, N=1300, rho=0.3, T=1.0, seed=5, nthreads=1):
-    c = oc.consts(temperature=T, colloid_radius=colloid, colloid_on=1 if colloid > 0 else 0)
+def make(colloid=0.0, N=1300, rho=0.3, T=1.0, seed=5, nthreads=1, colors=2):
+    c = oc.consts(temperature=T, colloid_radius=colloid, colloid_on=1 if colloid > 0 else 0,
+                  colors=colors)
     q, u, L = lattice_system(N, rho, colloid_radius=colloid, seed=seed)
     return c, oc.System(c, q, u, L, seed=1234 + seed, nthreads=nthreads)
 
 
+@pytest.mark.parametrize("colors", [2, 3])
 @pytest.mark.parametrize("colloid", [0.0, 3.0])
-def test_sweep_replay_brute_force(colloid):
-    c, s = make(colloid)
+def test_sweep_replay_brute_force(colloid, colors):
+    c, s = make(colloid, colors=colors)
     for _ in range(2):                        # relax the lattice start a little
         s.sweep()
     q0 = [a.copy() for a in s.q]
@@ -34,9 +36,10 @@ def test_sweep_replay_brute_force(colloid):
         assert cnt[6] > 0
 
 
+@pytest.mark.parametrize("colors", [2, 3])
 @pytest.mark.parametrize("colloid", [0.0, 3.0])
-def test_running_energy_matches_recompute(colloid):
-    c, s = make(colloid)
+def test_running_energy_matches_recompute(colloid, colors):
+    c, s = make(colloid, colors=colors)
     for _ in range(4):
         s.sweep()
     e = s.total_energy()
@@ -51,9 +54,10 @@ def test_running_energy_matches_recompute(colloid):
     assert eb[2] == pytest.approx(e[2], rel=1e-10, abs=1e-9)
 
 
-def test_thread_count_does_not_change_results():
-    _, s1 = make(3.0, nthreads=1)
-    _, s4 = make(3.0, nthreads=4)
+@pytest.mark.parametrize("colors", [2, 3])
+def test_thread_count_does_not_change_results(colors):
+    _, s1 = make(3.0, nthreads=1, colors=colors)
+    _, s4 = make(3.0, nthreads=4, colors=colors)
     for _ in range(3):
         s1.sweep()
         s4.sweep()
@@ -85,3 +89,45 @@ def test_npt_ideal_gas_volume():
     err = V.std() * np.sqrt(tau / len(V))
     assert abs(V.mean() - target) < 4 * err + 0.005 * target
     assert s.counters[8] > 0.2 * s.counters[7]
+
+
+def test_grid_rules():
+    # 2 colours: largest even n with L/n >= r_c, n >= 4; 3 colours: largest multiple of 3
+    # with L/n >= r_c/2, n >= 6
+    assert oc.grid_dim(92.8, 4.0, 2) == 22 and oc.grid_dim(92.8, 4.0, 3) == 45
+    assert oc.grid_dim(16.0, 4.0, 2) == 0 and oc.grid_dim(16.1, 4.0, 2) == 4
+    assert oc.grid_dim(12.0, 4.0, 3) == 0 and oc.grid_dim(12.1, 4.0, 3) == 6
+    for L in np.linspace(16.2, 120.0, 57):
+        for col, w in ((2, 4.0), (3, 2.0)):
+            n = oc.grid_dim(L, 4.0, col)
+            assert n % col == 0 and L / n >= w and L / (n + col) < w * (1 + 1e-6)
+
+
+def test_27_colour_order_is_a_permutation_and_keeps_the_8_colour_draws():
+    seen = set()
+    for t in range(100):
+        g2, o2, v2 = oc.sweep_setup(42, 0, t, 2)
+        g3, o3, v3 = oc.sweep_setup(42, 0, t, 3)
+        assert sorted(o3.tolist()) == list(range(27))
+        assert np.array_equal(g2, g3) and np.array_equal(v2, v3)
+        seen.add(tuple(o3.tolist()))
+    assert len(seen) == 100
+
+
+def test_27_colour_sweeps_sample_the_same_energy():
+    """Both decompositions are valid Metropolis chains of the same Boltzmann distribution:
+    their mean energies agree within the statistical error (tau_int-corrected)."""
+    from parellel_mc_mpi_for_liquidcrystalsystems_amd import mcgb as m
+    means = []
+    for col in (2, 3):
+        c, s = make(0.0, N=1300, rho=0.25, T=1.5, seed=8, nthreads=4, colors=col)
+        E = []
+        for t in range(700):
+            s.sweep()
+            if t >= 150:
+                E.append(s.energy_value() / s.N)
+        E = np.array(E)
+        tau, _ = m.tau_int(E)
+        means.append((E.mean(), E.std() * np.sqrt(tau / len(E))))
+    (m2, e2), (m3, e3) = means
+    assert abs(m2 - m3) < 4.0 * np.hypot(e2, e3), means

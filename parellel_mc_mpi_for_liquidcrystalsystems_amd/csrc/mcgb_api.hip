@@ -75,6 +75,9 @@ struct mcgb_ctx {
     int cur = 0;
     int *cell_id = nullptr, *tmp = nullptr, *count = nullptr, *cell_start = nullptr;
     int *cell_start2 = nullptr, *list2 = nullptr;
+    int* tsum = nullptr;                 // [nrep][scan_tiles] tile sums of the count scan
+    EndPartial* endp = nullptr;          // [nrep][kEndParts] partial sweep totals
+    int scan_tiles = 1;
     CellAcc* acc = nullptr;
     MoveDraw* draws = nullptr;
     int rec_max = 0;
@@ -211,6 +214,15 @@ int check_params(const mcgb_params& p, int r) {
     return MCGB_OK;
 }
 
+// Exclusive scan of the cell counts (three launches over tiles of kScanTile cells).
+void enqueue_scan(mcgb_ctx* c, int* cell_start, int trial) {
+    const dim3 gt(c->scan_tiles, c->nrep);
+    k_scan_tiles<<<gt, 256, 0, c->st>>>(c->dC, c->dD, c->count, c->tsum, c->scan_tiles, trial);
+    k_scan_top<<<c->nrep, 64, 0, c->st>>>(c->dD, c->tsum, c->scan_tiles, trial);
+    k_scan_apply<<<gt, 256, 0, c->st>>>(c->dC, c->dD, c->count, c->tsum, c->scan_tiles, cell_start,
+                                        trial);
+}
+
 template <typename Real>
 int enqueue_rebuild(mcgb_ctx* c) {
     const int src = c->cur, dst = 1 - c->cur;
@@ -218,7 +230,7 @@ int enqueue_rebuild(mcgb_ctx* c) {
     k_zero<<<256, 256, 0, c->st>>>(c->count, c->ncell_tot);
     k_count<<<gp, 256, 0, c->st>>>(c->dC, c->dD, c->q[src][0], c->q[src][1], c->q[src][2],
                                    c->cell_id, c->count, 0);
-    k_scan<<<c->nrep, 1024, 0, c->st>>>(c->dC, c->dD, c->count, c->cell_start, 0);
+    enqueue_scan(c, c->cell_start, 0);
     k_scatter<<<gp, 256, 0, c->st>>>(c->dC, c->dD, c->cell_id, c->count, c->cell_start, c->tmp, 0);
     const dim3 gs((c->ncell_max + 3) / 4, c->nrep);
     k_sort_gather<Real><<<gs, 256, 0, c->st>>>(
@@ -253,7 +265,7 @@ int enqueue_npt(mcgb_ctx* c) {
     const dim3 gp((c->Nmax + 255) / 256, c->nrep);
     k_zero<<<256, 256, 0, c->st>>>(c->count, c->ncell_tot);
     k_count<<<gp, 256, 0, c->st>>>(c->dC, c->dD, c->q[b][0], c->q[b][1], c->q[b][2], c->cell_id, c->count, 1);
-    k_scan<<<c->nrep, 1024, 0, c->st>>>(c->dC, c->dD, c->count, c->cell_start2, 1);
+    enqueue_scan(c, c->cell_start2, 1);
     k_scatter<<<gp, 256, 0, c->st>>>(c->dC, c->dD, c->cell_id, c->count, c->cell_start2, c->tmp, 1);
     const dim3 gs((c->ncell_max + 3) / 4, c->nrep);
     k_sort_gather<Real><<<gs, 256, 0, c->st>>>(
@@ -302,12 +314,12 @@ void launch_pass_t(mcgb_ctx* c, const PassArgs& P) {
     default: k_pass<Real, 4, false, TR><<<c->nblk, 256, c->lds, c->st>>>(P); break;
     }
     }
-    // cells whose stencil exceeded the staging capacity: one single-wavefront workgroup per
-    // overflow-list entry reads the stencil from HBM (with an empty list every workgroup exits
-    // at once); only the stencil table, draws and partials live in LDS
+    // cells whose stencil exceeded the staging capacity: single-wavefront workgroups walk the
+    // overflow list and read the stencil from HBM (with an empty list every workgroup exits at
+    // once); only the stencil table, draws and partials live in LDS
     PassArgs Q = P;
     Q.cap = 32;
-    k_pass<Real, 1, true, TR><<<c->nblk, 64, PassLds<Real, 1>::bytes(32), c->st>>>(Q);
+    k_pass<Real, 1, true, TR><<<std::min(c->nblk, 1024), 64, PassLds<Real, 1>::bytes(32), c->st>>>(Q);
 }
 
 template <typename Real>
@@ -376,7 +388,8 @@ int enqueue_one(mcgb_ctx* c, mcgb_move_trace* trace, const hipEvent_t* ev) {
         if (ev) HIPCHK(hipEventRecord(ev[2 + p], c->st));
     }
     HIPCHK(hipGetLastError());
-    k_end_moves<<<c->nrep, 256, 0, c->st>>>(c->dC, c->dD, c->acc);
+    k_end_partial<<<dim3(kEndParts, c->nrep), 256, 0, c->st>>>(c->dC, c->dD, c->acc, c->endp);
+    k_end_moves<<<c->nrep, 64, 0, c->st>>>(c->dC, c->dD, c->endp);
     if ((rc = enqueue_energy<Real>(c, 1))) return rc;
     if (c->any_npt && (rc = enqueue_npt<Real>(c))) return rc;
     if ((rc = enqueue_observe<Real>(c, 1, 1))) return rc;
@@ -628,8 +641,8 @@ int mcgb_create(const mcgb_params* p, int n_replicas, mcgb_ctx** out) {
             mcgb_destroy(c);
             return MCGB_ERR_ARG;
         }
-    // wavefronts per active cell and LDS staging capacity (1.15x the mean stencil, 27 or 125
-    // cells, plus 64, within 64 KiB; a larger stencil takes the HBM path)
+    // wavefronts per active cell and LDS staging capacity (1.10x the mean stencil, 27 or 125
+    // cells, plus 32, within 64 KiB; a larger stencil takes the HBM path)
     c->waves = p[0].waves_per_cell;
     if (c->waves != 1 && c->waves != 2 && c->waves != 3 && c->waves != 4 && c->waves != 8) {
         mcgb_set_error("waves_per_cell must be 1, 2, 3, 4 or 8 (got %d)", c->waves);
@@ -637,7 +650,7 @@ int mcgb_create(const mcgb_params* p, int n_replicas, mcgb_ctx** out) {
         return MCGB_ERR_ARG;
     }
     const double ncells_stencil = c->col == 3 ? 125.0 : 27.0;
-    int cap = (int)(ncells_stencil * occ_max * 1.15 + 64.0);
+    int cap = (int)(ncells_stencil * occ_max * 1.10 + 32.0);
     cap = (cap + 31) & ~31;
     if (p[0].stage_capacity > 0) cap = (p[0].stage_capacity + 31) & ~31;
     cap = std::max(32, cap);
@@ -663,6 +676,9 @@ int mcgb_create(const mcgb_params* p, int n_replicas, mcgb_ctx** out) {
     if (!rc) rc = dalloc_t(c, &c->count, c->ncell_tot);
     if (!rc) rc = dalloc_t(c, &c->cell_start, c->ncs_tot);
     if (!rc) rc = dalloc_t(c, &c->cell_start2, c->ncs_tot);
+    c->scan_tiles = (c->ncell_max + kScanTile - 1) / kScanTile;
+    if (!rc) rc = dalloc_t(c, &c->tsum, (size_t)c->nrep * c->scan_tiles);
+    if (!rc) rc = dalloc_t(c, &c->endp, (size_t)c->nrep * kEndParts);
     if (!rc) rc = dalloc_t(c, &c->acc, c->ncell_tot);
     if (!rc) rc = dalloc_t(c, &c->draws, (size_t)c->Ntot + c->ncell_tot);
     if (!rc) rc = dalloc_t(c, &c->ovf_count, 27);

@@ -75,8 +75,11 @@ __device__ inline void adapt_steps(float& dmax, float& drot, unsigned long long 
 }
 
 // Draws of sweep t (grid shift, sublattice order, NPT draws), the cell grid and step sizes.
-__global__ void k_begin(const RepConst* __restrict__ RC, RepDyn* __restrict__ RD, int nrep,
-                        int resum_interval) {
+__global__ __launch_bounds__(64) void k_begin(const RepConst* __restrict__ RC,
+                                              RepDyn* __restrict__ RD, int nrep,
+                                              int resum_interval) {
+    __shared__ uint32_t sdr[64][26];                 // shuffle draws and order of each replica:
+    __shared__ uint32_t sord[64][27];                // data-dependent indices stay in LDS
     int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= nrep) return;
     const RepConst& C = RC[r];
@@ -84,19 +87,23 @@ __global__ void k_begin(const RepConst* __restrict__ RC, RepDyn* __restrict__ RD
     const uint32_t t = D.t;
     const int P = C.col * C.col * C.col;
     uint32_t s[8][4];
-    const int nblk = P == 27 ? 8 : 3;                   // set-up Philox blocks needed
-    for (int j = 0; j < nblk; ++j) philox4x32_10((uint32_t)j, 0u, t, kSetupTag, C.key0, C.key1, s[j]);
+#pragma unroll
+    for (int j = 0; j < 8; ++j) philox4x32_10((uint32_t)j, 0u, t, kSetupTag, C.key0, C.key1, s[j]);
     D.g[0] = s[0][0]; D.g[1] = s[0][1]; D.g[2] = s[0][2];
     D.vol[0] = s[0][3]; D.vol[1] = s[2][3];
     // shuffle_int_array (MCnptColloid.c:3721-3737) with Philox draws: S_1[0..3], S_2[0..2],
-    // then S_3..S_7 in order
-    uint32_t dr[26];
-    int nd = 0;
-    for (int w = 0; w < 4; ++w) dr[nd++] = s[1][w];
-    for (int w = 0; w < 3; ++w) dr[nd++] = s[2][w];
-    for (int j = 3; j < nblk; ++j)
-        for (int w = 0; w < 4 && nd < 26; ++w) dr[nd++] = s[j][w];
-    uint32_t ord[27];
+    // then S_3..S_7 in order (S_7[3] unused)
+    uint32_t* dr = sdr[threadIdx.x];
+    uint32_t* ord = sord[threadIdx.x];
+#pragma unroll
+    for (int w = 0; w < 4; ++w) dr[w] = s[1][w];
+#pragma unroll
+    for (int w = 0; w < 3; ++w) dr[4 + w] = s[2][w];
+#pragma unroll
+    for (int j = 3; j < 8; ++j)
+#pragma unroll
+        for (int w = 0; w < 4; ++w)
+            if (7 + 4 * (j - 3) + w < 26) dr[7 + 4 * (j - 3) + w] = s[j][w];
     for (int i = 0; i < P; ++i) ord[i] = (uint32_t)i;
     for (int i = P - 1, d = 0; i >= 1; --i, ++d) {
         uint32_t j = (uint32_t)(((uint64_t)dr[d] * (uint64_t)(i + 1)) >> 32);
@@ -174,41 +181,102 @@ __global__ void k_count(const RepConst* __restrict__ RC, const RepDyn* __restric
     atomicAdd(&count[C.cell_off + c], 1);
 }
 
-// Exclusive scan of the counts into cell_start (one workgroup per replica); the counts are
-// zeroed for reuse as fill pointers.
-__global__ __launch_bounds__(1024) void k_scan(const RepConst* __restrict__ RC,
-                                               RepDyn* __restrict__ RD, int* __restrict__ count,
-                                               int* __restrict__ cell_start, int trial) {
-    __shared__ int part[1024];
-    const int r = blockIdx.x;
-    const RepConst& C = RC[r];
-    RepDyn& D = RD[r];
+// Exclusive scan of the counts into cell_start in three steps over tiles of kScanTile cells:
+// tile sums (k_scan_tiles), an exclusive scan of the tile sums per replica (k_scan_top), and
+// the scan inside every tile from its base (k_scan_apply), which also zeroes the counts for
+// reuse as fill pointers.  trial != 0: the NPT trial grid.
+constexpr int kScanTile = 1024;
+
+__device__ inline int scan_ncell(const RepDyn& D, int trial) {
+    return trial ? D.nt[0] * D.nt[1] * D.nt[2] : D.ncell;
+}
+
+__global__ __launch_bounds__(256) void k_scan_tiles(const RepConst* __restrict__ RC,
+                                                    const RepDyn* __restrict__ RD,
+                                                    const int* __restrict__ count,
+                                                    int* __restrict__ tsum, int tmax, int trial) {
+    __shared__ int ws[4];
+    const int r = blockIdx.y;
+    const RepDyn& D = RD[r];
     if (trial && !D.trial_ok) return;
-    const int ncell = trial ? D.nt[0] * D.nt[1] * D.nt[2] : D.ncell;
-    const int tid = threadIdx.x;
-    const int chunk = (ncell + 1023) / 1024;
-    const int c0 = tid * chunk;
-    const int c1 = min(ncell, c0 + chunk);
-    int* cnt = count + C.cell_off;
-    int sum = 0;
-    for (int c = c0; c < c1; ++c) sum += cnt[c];
-    part[tid] = sum;
+    const int ncell = scan_ncell(D, trial);
+    const int t = blockIdx.x;
+    if (t * kScanTile >= ncell) return;                     // uniform over the block
+    const int* cnt = count + RC[r].cell_off;
+    int s = 0;
+    for (int j = 0; j < 4; ++j) {
+        const int c = t * kScanTile + threadIdx.x * 4 + j;
+        s += c < ncell ? cnt[c] : 0;
+    }
+    s = wave_sum(s);
+    if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = s;
     __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {       // Hillis-Steele inclusive scan
-        int v = tid >= off ? part[tid - off] : 0;
-        __syncthreads();
-        part[tid] += v;
-        __syncthreads();
+    if (threadIdx.x == 0) tsum[(size_t)r * tmax + t] = ws[0] + ws[1] + ws[2] + ws[3];
+}
+
+__global__ __launch_bounds__(64) void k_scan_top(const RepDyn* __restrict__ RD,
+                                                 int* __restrict__ tsum, int tmax, int trial) {
+    const int r = blockIdx.x;
+    const RepDyn& D = RD[r];
+    if (trial && !D.trial_ok) return;
+    const int nt = (scan_ncell(D, trial) + kScanTile - 1) / kScanTile;
+    int* ts = tsum + (size_t)r * tmax;
+    const int lane = threadIdx.x;
+    int carry = 0;
+    for (int t0 = 0; t0 < nt; t0 += 64) {                   // 64 tile sums per step
+        const int t = t0 + lane;
+        const int v = t < nt ? ts[t] : 0;
+        int incl = v;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const int u = __shfl_up(incl, off);
+            if (lane >= off) incl += u;
+        }
+        if (t < nt) ts[t] = carry + incl - v;
+        carry += __shfl(incl, 63);
     }
-    int run = part[tid] - sum;
+}
+
+__global__ __launch_bounds__(256) void k_scan_apply(const RepConst* __restrict__ RC,
+                                                    const RepDyn* __restrict__ RD,
+                                                    int* __restrict__ count,
+                                                    const int* __restrict__ tsum, int tmax,
+                                                    int* __restrict__ cell_start, int trial) {
+    __shared__ int ws[4];
+    const int r = blockIdx.y;
+    const RepConst& C = RC[r];
+    const RepDyn& D = RD[r];
+    if (trial && !D.trial_ok) return;
+    const int ncell = scan_ncell(D, trial);
+    const int t = blockIdx.x;
+    if (t * kScanTile >= ncell) return;
+    int* cnt = count + C.cell_off;
     int* cs = cell_start + C.cs_off;
-    for (int c = c0; c < c1; ++c) {
-        int v = cnt[c];
-        cs[c] = run;
-        run += v;
-        cnt[c] = 0;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int c0 = t * kScanTile + threadIdx.x * 4;
+    int v[4], s = 0;
+    for (int j = 0; j < 4; ++j) {
+        v[j] = c0 + j < ncell ? cnt[c0 + j] : 0;
+        s += v[j];
     }
-    if (tid == 0) cs[ncell] = C.N;
+    int incl = s;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const int u = __shfl_up(incl, off);
+        if (lane >= off) incl += u;
+    }
+    if (lane == 63) ws[w] = incl;
+    __syncthreads();
+    int run = tsum[(size_t)r * tmax + t] + incl - s;
+    for (int q = 0; q < w; ++q) run += ws[q];
+    for (int j = 0; j < 4; ++j) {
+        if (c0 + j < ncell) {
+            cs[c0 + j] = run;
+            cnt[c0 + j] = 0;
+        }
+        run += v[j];
+    }
+    if (t == 0 && threadIdx.x == 0) cs[ncell] = C.N;
 }
 
 // Unordered scatter of particle slots into their cell segments.
@@ -325,18 +393,29 @@ __global__ __launch_bounds__(256) void k_draws(const RepConst* __restrict__ RC,
 
 // Folds the per-cell accumulators of the sweep into the running energy and counters (fixed
 // reduction order: deterministic) and flags an FP64 re-summation when needed.
-__global__ __launch_bounds__(256) void k_end_moves(const RepConst* __restrict__ RC,
-                                                   RepDyn* __restrict__ RD,
-                                                   const CellAcc* __restrict__ acc) {
+// Per-cell accumulators of a sweep folded in two levels with a fixed partition and order
+// (deterministic): kEndParts chunks of cells per replica (k_end_partial), then the chunks.
+constexpr int kEndParts = 64;
+struct EndPartial {
+    long long v[11];
+    double f, mv;
+};
+
+__global__ __launch_bounds__(256) void k_end_partial(const RepConst* __restrict__ RC,
+                                                     const RepDyn* __restrict__ RD,
+                                                     const CellAcc* __restrict__ acc,
+                                                     EndPartial* __restrict__ part) {
     __shared__ long long si[256][11];
     __shared__ double sd[256][2];
-    const int r = blockIdx.x;
+    const int r = blockIdx.y;
     const RepConst& C = RC[r];
-    RepDyn& D = RD[r];
+    const int ncell = RD[r].ncell;
+    const int chunk = (ncell + kEndParts - 1) / kEndParts;
+    const int c0 = blockIdx.x * chunk, c1 = min(ncell, c0 + chunk);
     const int tid = threadIdx.x;
     long long v[11] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     double f = 0.0, mv = 0.0;
-    for (int c = tid; c < D.ncell; c += 256) {
+    for (int c = c0 + tid; c < c1; c += 256) {
         const CellAcc& A = acc[C.cell_off + c];
         v[0] += A.att; v[1] += A.acc; v[2] += A.left; v[3] += A.hard;
         v[4] += A.cand; v[5] += A.pairs; v[6] += A.coll; v[7] += A.dcap; v[8] += A.dccap;
@@ -357,6 +436,31 @@ __global__ __launch_bounds__(256) void k_end_moves(const RepConst* __restrict__ 
         __syncthreads();
     }
     if (tid == 0) {
+        EndPartial& P = part[(size_t)r * kEndParts + blockIdx.x];
+        for (int q = 0; q < 11; ++q) P.v[q] = si[0][q];
+        P.f = sd[0][0];
+        P.mv = sd[0][1];
+    }
+}
+
+__global__ __launch_bounds__(64) void k_end_moves(const RepConst* __restrict__ RC,
+                                                  RepDyn* __restrict__ RD,
+                                                  const EndPartial* __restrict__ part) {
+    const int r = blockIdx.x;
+    const RepConst& C = RC[r];
+    RepDyn& D = RD[r];
+    if (threadIdx.x == 0) {
+        long long si0[11] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+        double f = 0.0, mv = 0.0;
+        for (int j = 0; j < kEndParts; ++j) {
+            const EndPartial& P = part[(size_t)r * kEndParts + j];
+            for (int q = 0; q < 11; ++q) si0[q] += P.v[q];
+            f += P.f;
+            mv += P.mv;
+        }
+        long long si[1][11];
+        double sd[1][2] = {{f, mv}};
+        for (int q = 0; q < 11; ++q) si[0][q] = si0[q];
         D.attempted += si[0][0]; D.accepted += si[0][1]; D.left += si[0][2]; D.hard += si[0][3];
         D.cand += si[0][4]; D.pairs += si[0][5]; D.coll += si[0][6];
         D.cap_pairs += si[0][7];

@@ -106,7 +106,7 @@ struct WavePartial {
 
 template <typename Real, int W>
 struct PassLds {
-    static constexpr int kDraws = 32;
+    static constexpr int kDraws = 8;                 // moves whose draws are staged at a time
     __host__ __device__ static inline int list_cap(int cap) {
         return ((cap + 64 * W - 1) / (64 * W)) * 64;
     }
@@ -304,18 +304,11 @@ __device__ inline int stencil_slot(const int* s_off, const int* s_start, int par
 // then is the exact GB energy evaluated, and stage 2 accepts with min(1, exp(-beta (dE - dE~))).
 // The surrogate orders each pair by stencil slot (lower slot first), a fixed function of the
 // configuration for the whole move, so the composite kernel keeps detailed balance exactly.
-template <typename Real, int W, bool GLOBAL, bool TR, bool DA = false>
-__global__ __launch_bounds__(64 * W) void k_pass(PassArgs P) {
+// pass_cell makes the moves of active-cell block b (every early return is uniform over the
+// workgroup); k_pass runs one block per workgroup, or (GLOBAL) walks the overflow list.
+template <typename Real, int W, bool GLOBAL, bool TR, bool DA>
+__device__ __forceinline__ void pass_cell(const PassArgs& P, int b, unsigned char* smem) {
     static_assert(!(DA && (GLOBAL || TR)), "delayed acceptance runs in the staged, untraced pass");
-    extern __shared__ __align__(16) unsigned char smem[];
-    int b;
-    if (GLOBAL) {
-        const int nov = P.ovf_count[P.pass];
-        if ((int)blockIdx.x >= nov) return;
-        b = P.ovf_list[(size_t)P.pass * P.nblk + blockIdx.x];
-    } else {
-        b = blockIdx.x;
-    }
     const int r = P.blk_rep[b];
     const RepConst& C = P.RC[r];
     RepDyn& D = P.RD[r];
@@ -424,10 +417,31 @@ __global__ __launch_bounds__(64 * W) void k_pass(PassArgs P) {
     // the cell).
     const int S_pad = (S + 63) & ~63;
     if (staged) {
-        for (int e = tid; e < S; e += 64 * W) {
-            int gi = stencil_slot(s_off, s_start, part_off, e, NS);
-            sq[e] = Pos3{P.qx[gi], P.qy[gi], P.qz[gi]};
-            su[e] = R3{Ux[gi], Uy[gi], Uz[gi]};
+        // four records per thread per step: the slot searches, then every HBM load, then the
+        // LDS stores (24 loads in flight per thread instead of 6)
+        constexpr int kB = 4;
+        for (int e0 = tid; e0 < S; e0 += kB * 64 * W) {
+            int gi[kB];
+#pragma unroll
+            for (int j = 0; j < kB; ++j) {
+                const int e = e0 + j * 64 * W;
+                gi[j] = e < S ? stencil_slot(s_off, s_start, part_off, e, NS) : -1;
+            }
+            uint32_t x[kB], y[kB], z[kB];
+            Real a0[kB], a1[kB], a2[kB];
+#pragma unroll
+            for (int j = 0; j < kB; ++j)
+                if (gi[j] >= 0) {
+                    x[j] = P.qx[gi[j]]; y[j] = P.qy[gi[j]]; z[j] = P.qz[gi[j]];
+                    a0[j] = Ux[gi[j]]; a1[j] = Uy[gi[j]]; a2[j] = Uz[gi[j]];
+                }
+#pragma unroll
+            for (int j = 0; j < kB; ++j)
+                if (gi[j] >= 0) {
+                    const int e = e0 + j * 64 * W;
+                    sq[e] = Pos3{x[j], y[j], z[j]};
+                    su[e] = R3{a0[j], a1[j], a2[j]};
+                }
         }
         const uint32_t fx = gx + (uint32_t)(((uint64_t)(2 * cx + 1) << 31) / (uint64_t)nx) + 0x80000000u;
         const uint32_t fy = gy + (uint32_t)(((uint64_t)(2 * cy + 1) << 31) / (uint64_t)ny) + 0x80000000u;
@@ -760,6 +774,20 @@ __global__ __launch_bounds__(64 * W) void k_pass(PassArgs P) {
         ca.cand = n_cand; ca.dfin = dfin_sum; ca.moved_abs = moved;
         ca.s1rej = n_s1; ca.s2rej = n_s2;
         P.acc[C.cell_off + cell] = ca;
+    }
+}
+
+template <typename Real, int W, bool GLOBAL, bool TR, bool DA = false>
+__global__ __launch_bounds__(64 * W) void k_pass(PassArgs P) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    if (GLOBAL) {
+        const int nov = P.ovf_count[P.pass];
+        for (int j = blockIdx.x; j < nov; j += gridDim.x) {
+            pass_cell<Real, W, GLOBAL, TR, DA>(P, P.ovf_list[(size_t)P.pass * P.nblk + j], smem);
+            __syncthreads();                  // the next entry reuses the LDS tables
+        }
+    } else {
+        pass_cell<Real, W, GLOBAL, TR, DA>(P, blockIdx.x, smem);
     }
 }
 

@@ -237,6 +237,11 @@ int mcgb_autotune(mcgb_ctx *ctx, int sweeps, int *best_waves, int *best_schedule
  * sweep, measured with HIP events on the context's stream over the last mcgb_sweep() call. */
 int mcgb_kernel_times(mcgb_ctx *ctx, double *ms_pass, double *ms_rebuild, double *ms_sweep);
 
+/* Kernel timing on (default) or off.  On: the last two sweeps of every mcgb_sweep() call are
+ * launched kernel by kernel between HIP events (for mcgb_kernel_times); off: every sweep is a
+ * graph replay.  The sweeps' results do not depend on it. */
+int mcgb_set_kernel_timing(mcgb_ctx *ctx, int on);
+
 /* Cell list for sweep index t of replica r without moving anything: grid shift, sublattice
  * order, cell_start[n_cells+1] and the particle ids in cell order.  Replaces
  * get_owner_block_index()/get_owner_cell_index() (MCnptColloid.c:4371-4399) and

@@ -1296,6 +1296,12 @@ int mcgb_autotune(mcgb_ctx* c, int sweeps, int* best_waves, int* best_schedule,
     return MCGB_OK;
 }
 
+int mcgb_set_kernel_timing(mcgb_ctx* c, int on) {
+    if (!c) { mcgb_set_error("null context"); return MCGB_ERR_ARG; }
+    c->timing = on != 0;
+    return MCGB_OK;
+}
+
 int mcgb_kernel_times(mcgb_ctx* c, double* ms_pass, double* ms_rebuild, double* ms_sweep) {
     if (!c) { mcgb_set_error("null context"); return MCGB_ERR_ARG; }
     if (ms_pass) *ms_pass = c->ms_pass;

@@ -446,21 +446,28 @@ __global__ __launch_bounds__(256) void k_end_partial(const RepConst* __restrict_
 __global__ __launch_bounds__(64) void k_end_moves(const RepConst* __restrict__ RC,
                                                   RepDyn* __restrict__ RD,
                                                   const EndPartial* __restrict__ part) {
+    __shared__ long long si[kEndParts][11];
+    __shared__ double sd[kEndParts][2];
     const int r = blockIdx.x;
     const RepConst& C = RC[r];
     RepDyn& D = RD[r];
-    if (threadIdx.x == 0) {
-        long long si0[11] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-        double f = 0.0, mv = 0.0;
-        for (int j = 0; j < kEndParts; ++j) {
-            const EndPartial& P = part[(size_t)r * kEndParts + j];
-            for (int q = 0; q < 11; ++q) si0[q] += P.v[q];
-            f += P.f;
-            mv += P.mv;
+    const int tid = threadIdx.x;                     // one chunk per lane, then a fixed tree
+    {
+        const EndPartial& P = part[(size_t)r * kEndParts + tid];
+        for (int q = 0; q < 11; ++q) si[tid][q] = P.v[q];
+        sd[tid][0] = P.f;
+        sd[tid][1] = P.mv;
+    }
+    __syncthreads();
+    for (int off = kEndParts / 2; off >= 1; off >>= 1) {
+        if (tid < off) {
+            for (int q = 0; q < 11; ++q) si[tid][q] += si[tid + off][q];
+            sd[tid][0] += sd[tid + off][0];
+            sd[tid][1] += sd[tid + off][1];
         }
-        long long si[1][11];
-        double sd[1][2] = {{f, mv}};
-        for (int q = 0; q < 11; ++q) si[0][q] = si0[q];
+        __syncthreads();
+    }
+    if (tid == 0) {
         D.attempted += si[0][0]; D.accepted += si[0][1]; D.left += si[0][2]; D.hard += si[0][3];
         D.cand += si[0][4]; D.pairs += si[0][5]; D.coll += si[0][6];
         D.cap_pairs += si[0][7];

@@ -126,6 +126,7 @@ def lib():
         "mcgb_tau_int": (C.c_int, [dp, C.c_int, C.c_double, dp, ip]),
         "mcgb_get_step_sizes": (C.c_int, [ctx, C.c_int, fp, fp, u64p]),
         "mcgb_sweep_async": (C.c_int, [ctx, C.c_int]),
+        "mcgb_set_kernel_timing": (C.c_int, [ctx, C.c_int]),
         "mcgb_sync": (C.c_int, [ctx]),
         "mcgb_snapshot_begin": (C.c_int, [ctx, C.c_int]),
         "mcgb_snapshot_wait": (C.c_int, [ctx, C.c_int, C.c_int, dp, dp, dp, dp, dp, dp, dp]),
@@ -427,6 +428,9 @@ class Context:
         check(lib().mcgb_autotune(self.h, sweeps, C.byref(best), C.byref(sched), _d(ms)))
         keys = [f"{s}/{w}" for s in ("seq", "spec") for w in (1, 2, 3, 4, 8)]
         return best.value, sched.value, dict(zip(keys, ms.tolist()))
+
+    def set_kernel_timing(self, on=True):
+        check(lib().mcgb_set_kernel_timing(self.h, 1 if on else 0))
 
     def kernel_times(self):
         a, b, c = C.c_double(), C.c_double(), C.c_double()

@@ -38,12 +38,12 @@ F_CAND, F_PAIR, F_COLL = 17.0, 68.0, 45.0
 # HBM bytes per launch of the W=2 pass kernel on this workload: rocprofv3 FETCH_SIZE x 2.000 +
 # WRITE_SIZE x 1.000 (the factors a plain 256 MiB float32 copy reads), from the two counter
 # passes summarised in profiles/sweep_fp32_N256000_colloid_hbm_traffic.md
-TRAFFIC_PASS_BYTES = 28.25e6
+TRAFFIC_PASS_BYTES = 28.37e6
 
 
-def workload(N=256000, rho=0.32, T=1.0, R=5.0, seed=20261203):
+def workload(N=256000, rho=0.32, T=1.0, R=5.0, seed=20261203, colors=2):
     p = m.gb_params(N, rho, T, colloid_radius=R, precision=32, seed=seed,
-                    lattice_jitter=0.05, colloid_exclusion_radius=5.5)
+                    lattice_jitter=0.05, colloid_exclusion_radius=5.5, sublattice_colors=colors)
     pos, u, box = m.init_lattice(N, rho, colloid_radius=R, exclusion_radius=5.5, jitter=0.05, seed=seed)
     return p, pos, u, box
 
@@ -164,7 +164,7 @@ def npt_leg(args, waves, schedule):
     N, rho0, T, P = 64000, 0.28, 1.0, 2.0
     seed = 20261204
     p = m.gb_params(N, rho0, T, precision=32, seed=seed, npt=True, pressure=P,
-                    npt_max_dlnv=0.01, lattice_jitter=0.05)
+                    npt_max_dlnv=0.01, lattice_jitter=0.05, sublattice_colors=args.colors)
     p.waves_per_cell = waves
     p.pass_schedule = schedule
     pos, u, box = m.init_lattice(N, rho0, jitter=0.05, seed=seed)
@@ -173,6 +173,7 @@ def npt_leg(args, waves, schedule):
         ctx.sweep(2)
         ctx.reset_counters()
         ctx.series_clear()
+        ctx.set_kernel_timing(False)
         t0 = time.perf_counter()
         ctx.sweep(args.npt_sweeps)
         dt = time.perf_counter() - t0
@@ -180,6 +181,7 @@ def npt_leg(args, waves, schedule):
         e, p2, rho = ctx.series(max_n=max(1 << 16, args.npt_sweeps + 16))
     h = len(rho) // 2
     return {"workload": "NPT N=64000 P*=2 T*=1 from rho*=0.28, one ln V move per sweep",
+            "sublattices": args.colors ** 3,
             "sweeps": args.npt_sweeps, "moves_per_s": st["attempted"] / dt,
             "ms_per_sweep": 1e3 * dt / args.npt_sweeps,
             "volume_acceptance": st["volume_accepted"] / max(1, st["volume_attempted"]),
@@ -202,7 +204,7 @@ def replicas_leg(args):
         R = 3.0 if i % 4 == 0 else 0.0
         T = 0.8 + i * 0.8 / 31
         params.append(m.gb_params(int(Ns[i]), 0.30, T, colloid_radius=R, precision=32, seed=seed,
-                                  lattice_jitter=0.05,
+                                  lattice_jitter=0.05, sublattice_colors=args.replica_colors,
                                   colloid_exclusion_radius=R + 0.5 if R > 0 else 0.0))
         states.append(m.init_lattice(int(Ns[i]), 0.30, colloid_radius=R,
                                      exclusion_radius=R + 0.5 if R > 0 else -1.0, jitter=0.05,
@@ -213,12 +215,14 @@ def replicas_leg(args):
         ctx.sweep(args.relax // 2)
         best, sched, ms = ctx.autotune(4)
         ctx.sweep(5)
+        ctx.set_kernel_timing(False)
         a0 = sum(ctx.stats(replica=i)["attempted"] for i in range(32))
         t0 = time.perf_counter()
         ctx.sweep(args.steps)
         dt = time.perf_counter() - t0
         a1 = sum(ctx.stats(replica=i)["attempted"] for i in range(32))
     return {"workload": "32 NVT replicas, N log-uniform in [2000, 32000], T* 0.8..1.6, rho*=0.30",
+            "sublattices": args.replica_colors ** 3,
             "N_total": int(Ns.sum()), "N_min_max": [int(Ns.min()), int(Ns.max())],
             "sweeps": args.steps, "moves_per_s": (a1 - a0) / dt,
             "ms_per_sweep": 1e3 * dt / args.steps, "waves_per_cell": best,
@@ -245,6 +249,12 @@ def main():
     ap.add_argument("--no-npt", action="store_true", help="skip the NPT leg")
     ap.add_argument("--npt-sweeps", type=int, default=5000)
     ap.add_argument("--no-replicas", action="store_true", help="skip the 32-replica leg")
+    ap.add_argument("--colors", type=int, default=2, choices=(2, 3),
+                    help="sublattice colours per axis of the plain and NPT legs "
+                         "(2: 8 sublattices of cells >= r_c; 3: 27 of cells >= r_c/2)")
+    ap.add_argument("--replica-colors", type=int, default=3, choices=(2, 3),
+                    help="sublattice colours per axis of the replica leg (small boxes: 27 "
+                         "sublattices keep the per-cell move chains short)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the sweep is not "
@@ -254,7 +264,7 @@ def main():
         print("no GPU visible to HIP", file=sys.stderr)
         sys.exit(1)
     N, rho, T, R = args.n, 0.32, 1.0, 5.0
-    p, pos, u, box = workload(N, rho, T, R)
+    p, pos, u, box = workload(N, rho, T, R, colors=args.colors)
     seed = int(p.seed)
     if args.waves:
         p.waves_per_cell = args.waves
@@ -298,7 +308,7 @@ def main():
     moves = st["attempted"]
     value = moves / dt
     flops_sweep = (F_CAND * st["candidates"] + F_PAIR * st["pairs"] + F_COLL * st["colloid_evals"]) / args.steps
-    flops_pass = flops_sweep / 8.0
+    flops_pass = flops_sweep / float(args.colors ** 3)
     achieved = flops_pass / (kt["ms_pass"] * 1e-3) / 1e12 if kt["ms_pass"] > 0 else 0.0
     out = {
         "metric": METRIC,
@@ -316,10 +326,12 @@ def main():
         "config": {"workload": "plain FP32 NVT checkerboard sweep with a colloid",
                    "N": N, "density": rho, "T": T, "colloid_radius": R, "gb": [3, 5, 2, 1],
                    "cutoff": 4.0, "max_displacement": 0.1, "max_rotation": 0.1,
-                   "lattice_jitter": 0.05, "seed": seed, "step": "one sweep (8 sublattice passes)"},
+                   "lattice_jitter": 0.05, "seed": seed,
+                   "step": f"one sweep ({args.colors ** 3} sublattice passes)"},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP32_TFLOPS,
-                     "traffic": TRAFFIC_PASS_BYTES if (tune_w == 2 and tune_s == 0) else None,
+                     "traffic": (TRAFFIC_PASS_BYTES if (tune_w == 2 and tune_s == 0 and args.colors == 2)
+                                 else None),
                      "kernel": "k_pass<float>", "ms_per_launch": kt["ms_pass"],
                      "flop_per_launch": flops_pass},
         "acceptance": st["accepted"] / max(1, moves),

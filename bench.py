@@ -38,7 +38,7 @@ F_CAND, F_PAIR, F_COLL = 17.0, 68.0, 45.0
 # HBM bytes per launch of the W=2 pass kernel on this workload: rocprofv3 FETCH_SIZE x 2.000 +
 # WRITE_SIZE x 1.000 (the factors a plain 256 MiB float32 copy reads), from the two counter
 # passes summarised in profiles/sweep_fp32_N256000_colloid_hbm_traffic.md
-TRAFFIC_PASS_BYTES = 28.37e6
+TRAFFIC_PASS_BYTES = 28.35e6
 
 
 def workload(N=256000, rho=0.32, T=1.0, R=5.0, seed=20261203, colors=2):
@@ -297,6 +297,9 @@ def main():
         t_get = time.perf_counter()
         ctx.get_state()                # the whole state device -> host (PCIe), outside the timed loop
         t_get = time.perf_counter() - t_get
+        t_get2 = time.perf_counter()
+        ctx.get_state()                # again: the pinned record is allocated by the first call
+        t_get2 = time.perf_counter() - t_get2
         if not args.no_da:
             ctx.pairs_reserve(args.pairs // 2 + 2 * 10 ** 6)
             k = 0
@@ -338,6 +341,7 @@ def main():
         "ms_rebuild": kt["ms_rebuild"],
         "ms_sweep_device": kt["ms_sweep"],
         "ms_get_state": 1e3 * t_get,
+        "ms_get_state_repeat": 1e3 * t_get2,
         "ess_per_s": {"E/N": ess_per_s(e, dt), "P2": ess_per_s(p2, dt), "samples": int(len(e))},
         "max_fp64_drift": st["max_drift"],
         "resummations_timed": st["resummations"],

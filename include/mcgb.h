@@ -175,7 +175,8 @@ int mcgb_set_state_fixed(mcgb_ctx *ctx, int replica, int n,
                          const double *ux, const double *uy, const double *uz,
                          const double box[3]);
 
-/* Copies replica r's configuration back, in particle-id order. */
+/* Copies replica r's configuration back, in particle-id order: converted to FP64 on the device
+ * (x = q L / 2^32, the host's product) and copied through a pinned record of its own. */
 int mcgb_get_state(mcgb_ctx *ctx, int replica, int n,
                    double *x, double *y, double *z,
                    double *ux, double *uy, double *uz, double box[3]);

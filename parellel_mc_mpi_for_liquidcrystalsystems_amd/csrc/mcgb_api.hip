@@ -108,8 +108,8 @@ struct mcgb_ctx {
     int tr_nblk = 0, adam_t = 0;
     // asynchronous snapshots: one staging record and one pinned host record per replica
     double* snap_dev = nullptr;                // [6 Ntot + 3 nrep]
-    std::vector<double*> snap_host;            // hipHostMalloc, 6 N + 3 doubles each
-    std::vector<hipEvent_t> snap_ev;
+    std::vector<double*> snap_host;            // [replica][slot] hipHostMalloc, 6 N + 3 doubles
+    std::vector<hipEvent_t> snap_ev;           // [replica][slot]; slot 1 serves mcgb_get_state
     std::vector<int> snap_pending;
     bool use_graph = true;
     bool timing = true;
@@ -893,21 +893,31 @@ int mcgb_get_state_fixed(mcgb_ctx* c, int r, int n, uint32_t* qx, uint32_t* qy, 
     return MCGB_OK;
 }
 
+static int snapshot_slot(mcgb_ctx* c, int r, int slot, double** host, hipEvent_t* ev);
+static int snapshot_enqueue(mcgb_ctx* c, int r, double* host, hipEvent_t ev);
+static void snapshot_copy_out(const double* h, int N, double* x, double* y, double* z, double* ux,
+                              double* uy, double* uz, double box[3]);
+
+// The FP64 state in particle order: converted on the device (the same product q L / 2^32 as a
+// host conversion) and copied into a pinned record of its own, so a snapshot in flight is kept.
 int mcgb_get_state(mcgb_ctx* c, int r, int n, double* x, double* y, double* z, double* ux,
                    double* uy, double* uz, double box[3]) {
     int rc = check_ctx(c, r);
     if (rc) return rc;
-    std::vector<uint32_t> q[3];
-    for (int d = 0; d < 3; ++d) q[d].resize(n > 0 ? n : 1);
-    double L[3];
-    rc = mcgb_get_state_fixed(c, r, n, q[0].data(), q[1].data(), q[2].data(), ux, uy, uz, L);
-    if (rc) return rc;
-    double* xs[3] = {x, y, z};
-    for (int d = 0; d < 3; ++d)
-        if (xs[d])
-            for (int i = 0; i < n; ++i) xs[d][i] = (double)q[d][i] * (L[d] / kTwo32);
-    if (box)
-        for (int d = 0; d < 3; ++d) box[d] = L[d];
+    if (n != c->hC[r].N) {
+        mcgb_set_error("replica %d holds %d particles, %d requested", r, c->hC[r].N, n);
+        return MCGB_ERR_ARG;
+    }
+    if (!c->state_set[r]) {
+        mcgb_set_error("replica %d has no configuration", r);
+        return MCGB_ERR_STATE;
+    }
+    double* host;
+    hipEvent_t ev;
+    if ((rc = snapshot_slot(c, r, 1, &host, &ev))) return rc;
+    if ((rc = snapshot_enqueue(c, r, host, ev))) return rc;
+    HIPCHK(hipEventSynchronize(ev));
+    snapshot_copy_out(host, n, x, y, z, ux, uy, uz, box);
     return MCGB_OK;
 }
 
@@ -963,31 +973,14 @@ int mcgb_sync(mcgb_ctx* c) {
     return MCGB_OK;
 }
 
-int mcgb_snapshot_begin(mcgb_ctx* c, int r) {
-    int rc = check_ctx(c, r);
-    if (rc) return rc;
-    if (!c->state_set[r]) {
-        mcgb_set_error("replica %d has no configuration", r);
-        return MCGB_ERR_STATE;
-    }
-    if (c->snap_host.empty()) {
-        c->snap_host.assign(c->nrep, nullptr);
-        c->snap_ev.assign(c->nrep, nullptr);
-        c->snap_pending.assign(c->nrep, 0);
-    }
+// Device-side conversion of replica r to FP64 particle order and its copy into the pinned host
+// record `host` (6 N + 3 doubles), behind event `ev`, after the work already enqueued.
+static int snapshot_enqueue(mcgb_ctx* c, int r, double* host, hipEvent_t ev) {
     const RepConst& C = c->hC[r];
     const size_t rec = 6 * (size_t)C.N + 3;
+    int rc;
     if (!c->snap_dev && (rc = dalloc_t(c, &c->snap_dev, 6 * (size_t)c->Ntot + 3 * (size_t)c->nrep)))
         return rc;
-    if (!c->snap_ev[r]) HIPCHK(hipEventCreateWithFlags(&c->snap_ev[r], hipEventDisableTiming));
-    if (!c->snap_host[r]) {
-        hipError_t e = hipHostMalloc((void**)&c->snap_host[r], rec * sizeof(double), hipHostMallocDefault);
-        if (e != hipSuccess) {
-            c->snap_host[r] = nullptr;
-            mcgb_set_error("hipHostMalloc(%zu bytes) failed: %s", rec * sizeof(double), hipGetErrorString(e));
-            return MCGB_ERR_NOMEM;
-        }
-    }
     double* stage = c->snap_dev + 6 * (size_t)C.part_off + 3 * (size_t)r;
     const int b = c->cur;
     const dim3 grid((C.N + 255) / 256);
@@ -1000,8 +993,55 @@ int mcgb_snapshot_begin(mcgb_ctx* c, int r) {
             c->dC, c->dD, r, c->q[b][0], c->q[b][1], c->q[b][2], (const float*)c->u[b][0],
             (const float*)c->u[b][1], (const float*)c->u[b][2], c->pid[b], stage);
     HIPCHK(hipGetLastError());
-    HIPCHK(hipMemcpyAsync(c->snap_host[r], stage, rec * sizeof(double), hipMemcpyDeviceToHost, c->st));
-    HIPCHK(hipEventRecord(c->snap_ev[r], c->st));
+    HIPCHK(hipMemcpyAsync(host, stage, rec * sizeof(double), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipEventRecord(ev, c->st));
+    return MCGB_OK;
+}
+
+// Pinned host record and event of slot `slot` (0: user snapshots, 1: mcgb_get_state) of
+// replica r, allocated on first use.
+static int snapshot_slot(mcgb_ctx* c, int r, int slot, double** host, hipEvent_t* ev) {
+    if (c->snap_host.empty()) {
+        c->snap_host.assign(2 * (size_t)c->nrep, nullptr);
+        c->snap_ev.assign(2 * (size_t)c->nrep, nullptr);
+        c->snap_pending.assign(c->nrep, 0);
+    }
+    const size_t k = 2 * (size_t)r + slot;
+    const size_t rec = 6 * (size_t)c->hC[r].N + 3;
+    if (!c->snap_ev[k]) HIPCHK(hipEventCreateWithFlags(&c->snap_ev[k], hipEventDisableTiming));
+    if (!c->snap_host[k]) {
+        hipError_t e = hipHostMalloc((void**)&c->snap_host[k], rec * sizeof(double), hipHostMallocDefault);
+        if (e != hipSuccess) {
+            c->snap_host[k] = nullptr;
+            mcgb_set_error("hipHostMalloc(%zu bytes) failed: %s", rec * sizeof(double), hipGetErrorString(e));
+            return MCGB_ERR_NOMEM;
+        }
+    }
+    *host = c->snap_host[k];
+    *ev = c->snap_ev[k];
+    return MCGB_OK;
+}
+
+static void snapshot_copy_out(const double* h, int N, double* x, double* y, double* z, double* ux,
+                              double* uy, double* uz, double box[3]) {
+    double* dst[6] = {x, y, z, ux, uy, uz};
+    for (int k = 0; k < 6; ++k)
+        if (dst[k]) memcpy(dst[k], h + (size_t)k * N, (size_t)N * sizeof(double));
+    if (box)
+        for (int d = 0; d < 3; ++d) box[d] = h[6 * (size_t)N + d];
+}
+
+int mcgb_snapshot_begin(mcgb_ctx* c, int r) {
+    int rc = check_ctx(c, r);
+    if (rc) return rc;
+    if (!c->state_set[r]) {
+        mcgb_set_error("replica %d has no configuration", r);
+        return MCGB_ERR_STATE;
+    }
+    double* host;
+    hipEvent_t ev;
+    if ((rc = snapshot_slot(c, r, 0, &host, &ev))) return rc;
+    if ((rc = snapshot_enqueue(c, r, host, ev))) return rc;
     c->snap_pending[r] = 1;
     return MCGB_OK;
 }
@@ -1019,14 +1059,9 @@ int mcgb_snapshot_wait(mcgb_ctx* c, int r, int n, double* x, double* y, double* 
         mcgb_set_error("replica %d holds %d particles, %d requested", r, N, n);
         return MCGB_ERR_ARG;
     }
-    HIPCHK(hipEventSynchronize(c->snap_ev[r]));
+    HIPCHK(hipEventSynchronize(c->snap_ev[2 * (size_t)r]));
     c->snap_pending[r] = 0;
-    const double* h = c->snap_host[r];
-    double* dst[6] = {x, y, z, ux, uy, uz};
-    for (int k = 0; k < 6; ++k)
-        if (dst[k]) memcpy(dst[k], h + (size_t)k * N, (size_t)N * sizeof(double));
-    if (box)
-        for (int d = 0; d < 3; ++d) box[d] = h[6 * (size_t)N + d];
+    snapshot_copy_out(c->snap_host[2 * (size_t)r], N, x, y, z, ux, uy, uz, box);
     return MCGB_OK;
 }
 

@@ -44,6 +44,21 @@ def test_snapshot_equals_get_state(precision):
             ctx.snapshot_begin(replica=r)
             snap = ctx.snapshot_wait(replica=r)
             assert _same(snap, ctx.get_state(replica=r)), f"replica {r}"
+            # the device conversion is the host's: x = q * (L / 2^32) in IEEE double
+            q, u, L = ctx.get_state_fixed(replica=r)
+            host = ([q[d].astype(np.float64) * (L[d] / 2.0 ** 32) for d in range(3)], u, L)
+            assert _same(snap, host), f"replica {r}"
+
+
+def test_get_state_keeps_a_snapshot_in_flight():
+    with _ctx(32) as ctx:
+        ctx.sweep(2)
+        ctx.snapshot_begin()
+        ctx.sweep(3)
+        now = ctx.get_state()                  # its own pinned record
+        snap = ctx.snapshot_wait()
+        assert not np.array_equal(snap[0][0], now[0][0])
+        assert _same(now, ctx.get_state())
 
 
 def test_snapshot_is_taken_before_later_sweeps():

@@ -157,8 +157,6 @@ void fill_consts(const mcgb_params& p, RepConst* C) {
     C->temperature = p.system_temperature;
     C->beta = dv[0];
     C->pressure = p.system_pressure;
-    C->dmax = (double)p.max_displacement;
-    C->drot = (double)p.max_rotation;
     C->colloid_on = p.colloid_wall_option == 1 ? 1 : 0;
     C->R = p.colloid_radius;
     C->R2 = dv[10];
@@ -181,7 +179,6 @@ void fill_consts(const mcgb_params& p, RepConst* C) {
     C->f_mu = (float)C->mu;
     C->f_half_nu = (float)(0.5 * C->nu);
     C->f_rc2 = (float)C->rc2;
-    C->f_drot = (float)C->drot;
     C->f_g1c0 = (float)((1.0 - C->chi) * (1.0 - C->chi));
     C->f_g1c1 = (float)(C->chi * (1.0 - C->chi));
     C->f_g1d0 = (float)(1.0 - C->chi2);

@@ -32,7 +32,7 @@ struct RepConst {
     int rec_off;           // first per-move draw record (capacity N + ncell_cap)
     // physics, FP64 master copies (get_system_const, MCnptColloid.c:2752-2783)
     double sigma0, four_eps0, chi, chi2, chip, chip2, mu, nu, rc, rc2;
-    double temperature, beta, pressure, dmax, drot;
+    double temperature, beta, pressure;   // step sizes live in RepDyn (adapted on the device)
     double R, R2, colloid_cut, kappa_prime, one_minus_chi, one_minus_chip;
     double npt_max_dlnv;
     int adapt_moves;       // step-size adaptation threshold (attempted moves), 0 = off
@@ -40,7 +40,7 @@ struct RepConst {
     float min_thick;       // min(sigma_ee, sigma_ss) as the reference's float (:833-834)
     double vcoll;          // 4/3 pi R^3 (density convention, MCnptColloid.c:2799-2804)
     // FP32 copies for the pair math
-    float f_sigma0, f_four_eps0, f_chi, f_chi2, f_chip, f_chip2, f_mu, f_half_nu, f_rc2, f_drot;
+    float f_sigma0, f_four_eps0, f_chi, f_chi2, f_chip, f_chip2, f_mu, f_half_nu, f_rc2;
     // g(om) = (c0 + c1 (|p|^2+|q|^2)/r^2 + om^2 (r.w)^2/r^2) / (d0 + om^2 |w|^2)
     float f_g1c0, f_g1c1, f_g1d0, f_g2c0, f_g2c1, f_g2d0;
     float f_rc2_pre;       // candidate pre-filter, slightly above rc^2 (final test in FP64)

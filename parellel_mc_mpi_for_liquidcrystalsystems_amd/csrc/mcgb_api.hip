@@ -224,14 +224,14 @@ template <typename Real>
 int enqueue_rebuild(mcgb_ctx* c) {
     const int src = c->cur, dst = 1 - c->cur;
     const dim3 gp((c->Nmax + 255) / 256, c->nrep);
-    k_zero<<<256, 256, 0, c->st>>>(c->count, c->ncell_tot);
+    // counts start at zero: set once at creation, reset by k_sort_gather after every rebuild
     k_count<<<gp, 256, 0, c->st>>>(c->dC, c->dD, c->q[src][0], c->q[src][1], c->q[src][2],
                                    c->cell_id, c->count, 0);
     enqueue_scan(c, c->cell_start, 0);
     k_scatter<<<gp, 256, 0, c->st>>>(c->dC, c->dD, c->cell_id, c->count, c->cell_start, c->tmp, 0);
     const dim3 gs((c->ncell_max + 3) / 4, c->nrep);
     k_sort_gather<Real><<<gs, 256, 0, c->st>>>(
-        c->dC, c->dD, c->cell_start, c->tmp, c->pid[src], 0, 0, nullptr, c->q[src][0], c->q[src][1],
+        c->dC, c->dD, c->cell_start, c->tmp, c->pid[src], 0, 0, nullptr, c->count, c->q[src][0], c->q[src][1],
         c->q[src][2], (const Real*)c->u[src][0], (const Real*)c->u[src][1], (const Real*)c->u[src][2],
         c->q[dst][0], c->q[dst][1], c->q[dst][2], (Real*)c->u[dst][0], (Real*)c->u[dst][1],
         (Real*)c->u[dst][2], c->pid[dst]);
@@ -260,13 +260,12 @@ int enqueue_npt(mcgb_ctx* c) {
     const int b = c->cur;
     k_npt_trial<<<(c->nrep + 63) / 64, 64, 0, c->st>>>(c->dC, c->dD, c->nrep);
     const dim3 gp((c->Nmax + 255) / 256, c->nrep);
-    k_zero<<<256, 256, 0, c->st>>>(c->count, c->ncell_tot);
     k_count<<<gp, 256, 0, c->st>>>(c->dC, c->dD, c->q[b][0], c->q[b][1], c->q[b][2], c->cell_id, c->count, 1);
     enqueue_scan(c, c->cell_start2, 1);
     k_scatter<<<gp, 256, 0, c->st>>>(c->dC, c->dD, c->cell_id, c->count, c->cell_start2, c->tmp, 1);
     const dim3 gs((c->ncell_max + 3) / 4, c->nrep);
     k_sort_gather<Real><<<gs, 256, 0, c->st>>>(
-        c->dC, c->dD, c->cell_start2, c->tmp, c->pid[b], 1, 1, c->list2, nullptr, nullptr, nullptr,
+        c->dC, c->dD, c->cell_start2, c->tmp, c->pid[b], 1, 1, c->list2, c->count, nullptr, nullptr, nullptr,
         nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
     HIPCHK(hipGetLastError());
     return enqueue_energy<Real>(c, 2);
@@ -316,7 +315,7 @@ void launch_pass_t(mcgb_ctx* c, const PassArgs& P) {
     // once); only the stencil table, draws and partials live in LDS
     PassArgs Q = P;
     Q.cap = 32;
-    k_pass<Real, 1, true, TR><<<std::min(c->nblk, 1024), 64, PassLds<Real, 1>::bytes(32), c->st>>>(Q);
+    k_pass<Real, 1, true, TR><<<std::min(c->nblk, 256), 64, PassLds<Real, 1>::bytes(32), c->st>>>(Q);
 }
 
 template <typename Real>
@@ -357,9 +356,9 @@ template <typename Real>
 int enqueue_one(mcgb_ctx* c, mcgb_move_trace* trace, const hipEvent_t* ev) {
     int rc;
     if (ev) HIPCHK(hipEventRecord(ev[0], c->st));
-    k_begin<<<(c->nrep + 63) / 64, 64, 0, c->st>>>(c->dC, c->dD, c->nrep, c->resum_interval);
+    k_begin<<<(c->nrep + 63) / 64, 64, 0, c->st>>>(c->dC, c->dD, c->nrep, c->resum_interval,
+                                                  c->ovf_count);
     k_draws<<<dim3((c->rec_max + 255) / 256, c->nrep), 256, 0, c->st>>>(c->dC, c->dD, c->draws);
-    k_zero_passes<<<1, 64, 0, c->st>>>(c->ovf_count);
     if ((rc = enqueue_rebuild<Real>(c))) return rc;
     if (ev) HIPCHK(hipEventRecord(ev[1], c->st));
     PassArgs P;
@@ -690,6 +689,12 @@ int mcgb_create(const mcgb_params* p, int n_replicas, mcgb_ctx** out) {
     if (!rc) rc = dalloc_t(c, &c->dD, n_replicas);
     if (!rc) rc = dalloc_t(c, &c->dBlk, c->nblk);
     if (rc) { mcgb_destroy(c); return rc; }
+    // the cell counts are zero between rebuilds (each rebuild's k_sort_gather resets them)
+    if (hipMemset(c->count, 0, sizeof(int) * (size_t)c->ncell_tot) != hipSuccess) {
+        mcgb_set_error("hipMemset of the cell counts failed");
+        mcgb_destroy(c);
+        return MCGB_ERR_HIP;
+    }
     if (hipMemcpy(c->dC, c->hC.data(), sizeof(RepConst) * n_replicas, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(c->dD, D.data(), sizeof(RepDyn) * n_replicas, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(c->dBlk, c->blk_rep.data(), sizeof(int) * c->nblk, hipMemcpyHostToDevice) != hipSuccess) {
@@ -1353,7 +1358,7 @@ int mcgb_cell_list(mcgb_ctx* c, int r, uint32_t t, int* cell_start, int cell_sta
     const uint32_t t_saved = D[r].t;
     D[r].t = t;
     if ((rc = write_dyn(c, D))) return rc;
-    k_begin<<<(c->nrep + 63) / 64, 64, 0, c->st>>>(c->dC, c->dD, c->nrep, 0);
+    k_begin<<<(c->nrep + 63) / 64, 64, 0, c->st>>>(c->dC, c->dD, c->nrep, 0, nullptr);
     rc = c->prec == 64 ? enqueue_rebuild<double>(c) : enqueue_rebuild<float>(c);
     if (rc) return rc;
     std::vector<RepDyn> D2;

@@ -1,7 +1,7 @@
 // mcgb_kernels.hpp — HIP kernels of the checkerboard Gay-Berne sweep (gfx950, wave64).
 //
 // One sweep of every replica is the launch sequence
-//   k_begin -> k_zero -> k_count -> k_scan -> k_scatter -> k_sort_gather   (cell-list rebuild)
+//   k_begin -> k_count -> k_scan_* -> k_scatter -> k_sort_gather   (cell-list rebuild)
 //   -> k_pass x P (sublattice passes, P = 8 or 27) -> k_end_moves -> k_energy/k_energy_reduce (re-sum, only
 //   when flagged) -> [NPT: trial cell list, k_energy, k_volume] -> k_observe -> k_observe_reduce.
 // Nothing in it synchronises with the host.
@@ -77,9 +77,10 @@ __device__ inline void adapt_steps(float& dmax, float& drot, unsigned long long 
 // Draws of sweep t (grid shift, sublattice order, NPT draws), the cell grid and step sizes.
 __global__ __launch_bounds__(64) void k_begin(const RepConst* __restrict__ RC,
                                               RepDyn* __restrict__ RD, int nrep,
-                                              int resum_interval) {
+                                              int resum_interval, int* __restrict__ ovf_count) {
     __shared__ uint32_t sdr[64][26];                 // shuffle draws and order of each replica:
     __shared__ uint32_t sord[64][27];                // data-dependent indices stay in LDS
+    if (ovf_count && blockIdx.x == 0 && threadIdx.x < 27) ovf_count[threadIdx.x] = 0;
     int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= nrep) return;
     const RepConst& C = RC[r];
@@ -126,9 +127,6 @@ __global__ __launch_bounds__(64) void k_begin(const RepConst* __restrict__ RC,
     D.resum = (resum_interval > 0 && ((t + 1) % (uint32_t)resum_interval) == 0) ? 1 : 0;
 }
 
-__global__ void k_zero_passes(int* __restrict__ p) {
-    if (threadIdx.x < 27) p[threadIdx.x] = 0;
-}
 
 // Host-requested cell list with zero shift (set_state, explicit recompute): grid from D.L.
 __global__ void k_grid_g0(const RepConst* __restrict__ RC, RepDyn* __restrict__ RD, int nrep) {
@@ -151,9 +149,6 @@ __global__ void k_grid_g0(const RepConst* __restrict__ RC, RepDyn* __restrict__ 
     D.k = (C.N + nc - 1) / nc;
 }
 
-__global__ void k_zero(int* __restrict__ p, int n) {
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0;
-}
 
 // ------------------------------------------------------------------ cell-list rebuild
 
@@ -301,7 +296,7 @@ template <typename Real>
 __global__ __launch_bounds__(256) void k_sort_gather(
     const RepConst* __restrict__ RC, const RepDyn* __restrict__ RD, const int* __restrict__ cell_start,
     const int* __restrict__ tmp, const int* __restrict__ pid_src, int index_only, int trial,
-    int* __restrict__ list_out,
+    int* __restrict__ list_out, int* __restrict__ count,
     const uint32_t* __restrict__ sqx, const uint32_t* __restrict__ sqy, const uint32_t* __restrict__ sqz,
     const Real* __restrict__ sux, const Real* __restrict__ suy, const Real* __restrict__ suz,
     uint32_t* __restrict__ dqx, uint32_t* __restrict__ dqy, uint32_t* __restrict__ dqz,
@@ -317,6 +312,8 @@ __global__ __launch_bounds__(256) void k_sort_gather(
     const bool valid = c < ncell;
     const int s = valid ? cell_start[C.cs_off + c] : 0;
     const int nc = valid ? cell_start[C.cs_off + c + 1] - s : 0;
+    // the fill pointers of k_scatter are not read again: back to zero for the next rebuild
+    if (valid && lane == 0) count[C.cell_off + c] = 0;
     const int* seg = tmp + C.part_off + s;
     const int* pids = pid_src + C.part_off;
     const bool in_lds = nc <= 256;

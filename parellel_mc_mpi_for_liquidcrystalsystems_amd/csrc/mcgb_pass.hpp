@@ -125,12 +125,17 @@ struct PassLds {
     __host__ __device__ static inline int off_draws(int cap) {
         return (off_list(cap) + W * list_stride(cap) * 2 + 15) & ~15;
     }
+    // inside the draws region: the draws of kDraws moves, the wavefront partials, the
+    // surrogate partials, then the move preparation (trial orientations, new-state colloid
+    // terms, flags) and the colloid terms of up to kCache cell particles at their current states
+    static constexpr int kCache = 64;
+    static constexpr int kDrawBytes = kDraws * (32 + 3 * (int)sizeof(Real));   // a multiple of 16
+    static constexpr int kSurOff = (kDrawBytes + W * (int)sizeof(WavePartial) + 15) & ~15;
+    static constexpr int kPrepOff = (kSurOff + W * (int)sizeof(float) + 15) & ~15;
+    static constexpr int kPrepBytes = kDraws * 8 + kCache * 8 + kDraws * 3 * (int)sizeof(Real) +
+                                      kDraws * 4 + kCache;
     __host__ __device__ static inline size_t bytes(int cap) {
-        size_t b = (size_t)off_draws(cap);
-        b += kDraws * (8 + 8 + 3 * sizeof(Real) + 12 + 4);    // draws
-        b += W * sizeof(WavePartial) + 16;
-        b += W * sizeof(float) + 16;                          // surrogate partials
-        return b + 16;
+        return (size_t)off_draws(cap) + kPrepOff + kPrepBytes + 16;
     }
 };
 
@@ -341,10 +346,16 @@ __device__ __forceinline__ void pass_cell(const PassArgs& P, int b, unsigned cha
     int* sdqx = (int*)(svz + KD);                              // 3 x KD
     int* sdqy = sdqx + KD;
     int* sdqz = sdqy + KD;
-    uint32_t* sj = (uint32_t*)(sdqz + KD);                     // KD
-    constexpr int kDrawBytes = KD * (32 + 3 * (int)sizeof(Real));        // a multiple of 16
-    WavePartial* sred = (WavePartial*)(dp + kDrawBytes);
-    float* ssur = (float*)(dp + ((kDrawBytes + W * (int)sizeof(WavePartial) + 15) & ~15));
+    int* sjs = (int*)(sdqz + KD);                              // KD: cell-local particle index
+    WavePartial* sred = (WavePartial*)(dp + L::kDrawBytes);
+    float* ssur = (float*)(dp + L::kSurOff);
+    double* sen = (double*)(dp + L::kPrepOff);                 // KD: new-state colloid term
+    double* scache = sen + KD;                                 // kCache: current colloid terms
+    Real* sunx = (Real*)(scache + L::kCache);                  // 3 x KD: trial orientations
+    Real* suny = sunx + KD;
+    Real* sunz = suny + KD;
+    uint32_t* sflag = (uint32_t*)(sunz + KD);                  // KD: earlier moves of the particle | cap << 16
+    unsigned char* scc = (unsigned char*)(sflag + KD);         // kCache: cap flags of scache
     MlpRegs MR;
     if (DA) mlp_load(P.mlp, lane, MR);
 
@@ -477,6 +488,22 @@ __device__ __forceinline__ void pass_cell(const PassArgs& P, int b, unsigned cha
         const double hd = 0.5 * sqrt(wx * wx + wy * wy + wz * wz);
         cnear = dc - hd <= creach * (1.0 + 1e-9) + 1e-6;
     }
+    auto rel = [](uint32_t q, double s) { return (double)(int32_t)(q - 0x80000000u) * s; };
+    // Cells near the colloid keep the colloid term of each of their particles at its current
+    // state in LDS (set here, replaced on acceptance), so a move evaluates only its trial
+    // state's term, and that one is prepared with the chunk's draws, one move per lane.  The
+    // term is exactly 0 beyond the colloid cutoff, so evaluating it everywhere in such cells
+    // gives the same sums as testing the reach first.
+    const bool ccache = staged && cnear && nc <= L::kCache;
+    if (ccache)
+        for (int j = tid; j < nc; j += 64 * W) {
+            const Pos3 qj = sq[coff + j];
+            const R3 uj = su[coff + j];
+            bool cj;
+            scache[j] = colloid_f64(C, rel(qj.x, X.s0), rel(qj.y, X.s1), rel(qj.z, X.s2),
+                                    (double)uj.x, (double)uj.y, (double)uj.z, &cj);
+            scc[j] = cj ? 1 : 0;
+        }
 
     if (nc == 0) {
         if (tracing)
@@ -487,23 +514,62 @@ __device__ __forceinline__ void pass_cell(const PassArgs& P, int b, unsigned cha
     } else {
         for (int m0 = 0; m0 < k; m0 += KD) {
             __syncthreads();
+            int myj = -1;
+            uint32_t mycap = 0;
             if (tid < KD && m0 + tid < k) {   // the draws of up to KD moves (k_draws)
                 const MoveDraw& d = drw[m0 + tid];
-                sj[tid] = d.j;
+                const int js = (int)(((uint64_t)d.j * (uint32_t)nc) >> 32);
+                const Real vx = (Real)d.v[0], vy = (Real)d.v[1], vz = (Real)d.v[2];
+                myj = js;
+                sjs[tid] = js;
                 sdqx[tid] = d.dq[0];
                 sdqy[tid] = d.dq[1];
                 sdqz[tid] = d.dq[2];
-                svx[tid] = (Real)d.v[0];
-                svy[tid] = (Real)d.v[1];
-                svz[tid] = (Real)d.v[2];
+                svx[tid] = vx;
+                svy[tid] = vy;
+                svz[tid] = vz;
                 sthr[tid] = d.thr;
                 sthr1[tid] = d.thr1;
+                if (staged) {
+                    // the move's trial orientation (and trial colloid term) from the
+                    // particle's state at the start of the chunk
+                    const R3 uj = su[coff + js];
+                    Real ox, oy, oz;
+                    rotate(drot, uj.x, uj.y, uj.z, vx, vy, vz, ox, oy, oz);
+                    sunx[tid] = ox;
+                    suny[tid] = oy;
+                    sunz[tid] = oz;
+                    if (ccache) {
+                        const Pos3 qj = sq[coff + js];
+                        bool cn;
+                        sen[tid] = colloid_f64(C, rel(qj.x + (uint32_t)d.dq[0], X.s0),
+                                               rel(qj.y + (uint32_t)d.dq[1], X.s1),
+                                               rel(qj.z + (uint32_t)d.dq[2], X.s2), (double)ox,
+                                               (double)oy, (double)oz, &cn);
+                        mycap = cn ? 1u << 16 : 0u;
+                    }
+                }
+            }
+            if (staged && w == 0) {
+                // bit q: move q of the chunk (q earlier than this one) draws the same particle
+                uint32_t dup = 0;
+#pragma unroll
+                for (int q = 0; q < KD - 1; ++q) {
+                    const int jq = __shfl(myj, q);
+                    if (q < tid && jq == myj) dup |= 1u << q;
+                }
+                if (tid < KD) sflag[tid] = dup | mycap;
             }
             __syncthreads();
+            uint32_t acc_mask = 0;            // moves of this chunk accepted so far
             const int mend = min(k, m0 + KD);
             for (int m = m0; m < mend; ++m) {
                 const int ml = m - m0;
-                const int jsel = uni((int)(((uint64_t)sj[ml] * (uint32_t)nc) >> 32));
+                const int jsel = uni(sjs[ml]);
+                // the chunk's preparation of this move holds unless an earlier accepted move
+                // of the chunk moved the same particle
+                const uint32_t fl = staged ? uni(sflag[ml]) : 0u;
+                const bool fresh = staged && (fl & acc_mask & 0xffffu) == 0u;
                 const int li = coff + jsel;          // flat stencil index of the moved particle
                 const int gsel = cstart + jsel;      // its global slot
                 uint32_t qix, qiy, qiz;
@@ -537,8 +603,12 @@ __device__ __forceinline__ void pass_cell(const PassArgs& P, int b, unsigned cha
                     ++n_hard;
                 } else {
                     Real unx, uny, unz;
-                    rotate(drot, uix, uiy, uiz, svx[ml], svy[ml], svz[ml], unx, uny, unz);
-                    unx = uni(unx); uny = uni(uny); unz = uni(unz);
+                    if (fresh) {
+                        unx = uni(sunx[ml]); uny = uni(suny[ml]); unz = uni(sunz[ml]);
+                    } else {
+                        rotate(drot, uix, uiy, uiz, svx[ml], svy[ml], svz[ml], unx, uny, unz);
+                        unx = uni(unx); uny = uni(uny); unz = uni(unz);
+                    }
                     LaneAcc<Real> acc;
                     Real scl = 0;
                     int cpl = 0;
@@ -577,8 +647,23 @@ __device__ __forceinline__ void pass_cell(const PassArgs& P, int b, unsigned cha
                     }
                     // colloid-rod term of both states in FP64, only near the colloid
                     double cacc = 0.0, cscl = 0.0;
+                    double cen = 0.0;                 // the trial state's term (for the cache)
+                    bool ccn = false;
                     if (colloid_on) n_coll += 2;
-                    if (cnear) {
+                    if (ccache) {
+                        const double eo = scache[jsel];
+                        const bool co = scc[jsel] != 0;
+                        if (fresh) {
+                            cen = sen[ml];
+                            ccn = (fl >> 16) & 1u;
+                        } else {
+                            cen = colloid_f64(C, rnx, rny, rnz, (double)unx, (double)uny,
+                                              (double)unz, &ccn);
+                        }
+                        ccd = (ccn ? 1 : 0) - (co ? 1 : 0);
+                        cacc = (ccn ? 0.0 : cen) - (co ? 0.0 : eo);
+                        cscl = fabs(ccn ? 0.0 : cen) + fabs(co ? 0.0 : eo);
+                    } else if (cnear) {
                         const double rox = (double)(int32_t)(qix - 0x80000000u) * X.s0;
                         const double roy = (double)(int32_t)(qiy - 0x80000000u) * X.s1;
                         const double roz = (double)(int32_t)(qiz - 0x80000000u) * X.s2;
@@ -734,10 +819,15 @@ __device__ __forceinline__ void pass_cell(const PassArgs& P, int b, unsigned cha
                         dccap_sum += ccd;
                         dfin_sum += dfin;
                         moved += fabs(dfin);
+                        acc_mask |= 1u << ml;
                         if (tid == 0) {
                             if (staged) {
                                 sq[li] = Pos3{qnx, qny, qnz};
                                 su[li] = R3{unx, uny, unz};
+                            }
+                            if (ccache) {
+                                scache[jsel] = cen;
+                                scc[jsel] = ccn ? 1 : 0;
                             }
                             P.qx[gsel] = qnx; P.qy[gsel] = qny; P.qz[gsel] = qnz;
                             Ux[gsel] = unx; Uy[gsel] = uny; Uz[gsel] = unz;

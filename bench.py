@@ -39,6 +39,8 @@ F_CAND, F_PAIR, F_COLL = 17.0, 68.0, 45.0
 # WRITE_SIZE x 1.000 (the factors a plain 256 MiB float32 copy reads), from the two counter
 # passes summarised in profiles/sweep_fp32_N256000_colloid_hbm_traffic.md
 TRAFFIC_PASS_BYTES = 28.35e6
+# pass schedules (mcgb_params.pass_schedule)
+SCHEDULES = ("sequential", "speculative", "sequential_4waves")
 
 
 def workload(N=256000, rho=0.32, T=1.0, R=5.0, seed=20261203, colors=2):
@@ -226,7 +228,7 @@ def replicas_leg(args):
             "N_total": int(Ns.sum()), "N_min_max": [int(Ns.min()), int(Ns.max())],
             "sweeps": args.steps, "moves_per_s": (a1 - a0) / dt,
             "ms_per_sweep": 1e3 * dt / args.steps, "waves_per_cell": best,
-            "schedule": "speculative" if sched else "sequential", "autotune_ms_per_pass": ms}
+            "schedule": SCHEDULES[sched], "autotune_ms_per_pass": ms}
 
 
 def main():
@@ -242,7 +244,8 @@ def main():
                     help="untimed sweeps from the lattice start before tuning and warm-up")
     ap.add_argument("--waves", type=int, default=0, help="wavefronts per cell (0: autotune)")
     ap.add_argument("--schedule", type=int, default=1,
-                    help="pass schedule with --waves: 1 speculative, 0 sequential")
+                    help="pass schedule with --waves: 0 sequential, 1 speculative, "
+                         "2 sequential built for four wavefronts per SIMD")
     ap.add_argument("--da-sweeps", type=int, default=100)
     ap.add_argument("--pairs", type=int, default=10 ** 7, help="training pairs for the surrogate")
     ap.add_argument("--train-steps", type=int, default=20000)
@@ -281,8 +284,7 @@ def main():
         tune_s = args.schedule
         if not args.waves:
             best, sched, ms = ctx.autotune(4)
-            tune = {"waves_per_cell": best, "schedule": "speculative" if sched else "sequential",
-                    "ms_per_pass": ms}
+            tune = {"waves_per_cell": best, "schedule": SCHEDULES[sched], "ms_per_pass": ms}
             tune_w, tune_s = best, sched
         if args.warmup:
             ctx.sweep(args.warmup)

@@ -75,7 +75,9 @@ typedef struct mcgb_params {
     int      stage_capacity;                /* particles staged in LDS per stencil, 0: auto */
     int      waves_per_cell;                /* wavefronts per active cell: 1, 2, 3, 4 or 8  */
     int      pass_schedule;                 /* 1: speculative (moves of a cell evaluated in
-                                               parallel, decided in order), 0: sequential  */
+                                               parallel, decided in order), 0: sequential,
+                                               2: sequential built for four wavefronts per
+                                               SIMD (FP32; FP64 runs 0)                    */
     int      adapt_moves;                   /* step-size adaptation every >= adapt_moves
                                                attempted moves (the reference: 10000), 0 = off */
     int      sublattice_colors;             /* 2: 8 sublattices of cells >= r_c wide (default);
@@ -225,14 +227,15 @@ int mcgb_series(mcgb_ctx *ctx, int replica, int max,
                 double *energy_per_particle, double *p2, double *density);
 int mcgb_series_clear(mcgb_ctx *ctx);
 
-/* Autotuner: times `sweeps` sweeps with each pass schedule (sequential, speculative) and each
- * supported number of wavefronts per active cell (1, 2, 3, 4, 8; the LDS staging capacity
- * follows) and keeps the fastest.  The timed sweeps are ordinary sweeps of the simulation (they
- * advance it).  No counterpart in the reference.  ms_per_pass (may be NULL) receives the
- * measured pass times, index 5*schedule + {0,1,2,3,4} for 1, 2, 3, 4, 8 wavefronts (0 for the
- * speculative schedule with sublattice_colors 3, which it does not run). */
+/* Autotuner: times `sweeps` sweeps with each pass schedule (0 sequential, 1 speculative,
+ * 2 sequential built for four wavefronts per SIMD) and each supported number of wavefronts per
+ * active cell (1, 2, 3, 4, 8; the LDS staging capacity follows) and keeps the fastest.  The
+ * timed sweeps are ordinary sweeps of the simulation (they advance it).  No counterpart in the
+ * reference.  ms_per_pass (may be NULL) receives the measured pass times, index
+ * 5*schedule + {0,1,2,3,4} for 1, 2, 3, 4, 8 wavefronts (0 for a schedule it does not run: the
+ * speculative one with sublattice_colors 3, schedule 2 in FP64). */
 int mcgb_autotune(mcgb_ctx *ctx, int sweeps, int *best_waves, int *best_schedule,
-                  double ms_per_pass[10]);
+                  double ms_per_pass[15]);
 
 /* Average device time per launch of the sweep-pass kernel, the cell-list rebuild and the whole
  * sweep, measured with HIP events on the context's stream over the last mcgb_sweep() call. */

@@ -61,6 +61,7 @@ struct mcgb_ctx {
     size_t lds = 0;
     size_t lds_spec = 0;
     bool spec = true;          // speculative pass schedule for untraced plain sweeps
+    bool o4 = false;           // FP32 sequential pass built for four wavefronts per SIMD
     int resum_interval = 100;
     int series_cap = kSeriesCapDefault;
     bool any_npt = false;
@@ -300,6 +301,14 @@ void launch_pass_t(mcgb_ctx* c, const PassArgs& P) {
         case 3: k_pass_spec<Real, 3><<<c->nblk, 192, c->lds_spec, c->st>>>(P); break;
         case 8: k_pass_spec<Real, 8><<<c->nblk, 512, c->lds_spec, c->st>>>(P); break;
         default: k_pass_spec<Real, 4><<<c->nblk, 256, c->lds_spec, c->st>>>(P); break;
+        }
+    } else if (!TR && c->o4 && std::is_same<Real, float>::value) {
+        switch (c->waves) {
+        case 1: k_pass_o4<1><<<c->nblk, 64, c->lds, c->st>>>(P); break;
+        case 2: k_pass_o4<2><<<c->nblk, 128, c->lds, c->st>>>(P); break;
+        case 3: k_pass_o4<3><<<c->nblk, 192, c->lds, c->st>>>(P); break;
+        case 8: k_pass_o4<8><<<c->nblk, 512, c->lds, c->st>>>(P); break;
+        default: k_pass_o4<4><<<c->nblk, 256, c->lds, c->st>>>(P); break;
         }
     } else {
     switch (c->waves) {
@@ -656,7 +665,8 @@ int mcgb_create(const mcgb_params* p, int n_replicas, mcgb_ctx** out) {
     c->cap = cap;
     c->lds = c->rsize == 8 ? pass_lds_seq<double>(c->waves, cap) : pass_lds_seq<float>(c->waves, cap);
     c->lds_spec = c->rsize == 8 ? pass_lds_spec<double>(c->waves, cap) : pass_lds_spec<float>(c->waves, cap);
-    c->spec = p[0].pass_schedule != 0 && c->col == 2;    // the speculative pass: 8 sublattices
+    c->spec = p[0].pass_schedule == 1 && c->col == 2;    // the speculative pass: 8 sublattices
+    c->o4 = p[0].pass_schedule == 2 && c->rsize == 4;    // four wavefronts per SIMD: FP32
 
     // ---- device memory
     int rc = 0;
@@ -1301,7 +1311,7 @@ static int set_waves(mcgb_ctx* c, int waves) {
 }
 
 int mcgb_autotune(mcgb_ctx* c, int sweeps, int* best_waves, int* best_schedule,
-                  double ms_per_pass[10]) {
+                  double ms_per_pass[15]) {
     int rc = check_ctx(c, 0);
     if (rc) return rc;
     if (sweeps <= 0) { mcgb_set_error("mcgb_autotune: sweeps must be > 0"); return MCGB_ERR_ARG; }
@@ -1310,10 +1320,12 @@ int mcgb_autotune(mcgb_ctx* c, int sweeps, int* best_waves, int* best_schedule,
     c->timing = true;
     double best = 1e300;
     int bw = c->waves;
-    bool bs = c->spec;
-    for (int sched = 0; sched < 2; ++sched) {
+    int bs = c->spec ? 1 : (c->o4 ? 2 : 0);
+    for (int sched = 0; sched < 3; ++sched) {
         c->spec = sched == 1;
-        if (c->spec && c->col != 2) {           // the speculative pass runs 8 sublattices only
+        c->o4 = sched == 2;
+        // the speculative pass runs 8 sublattices only, the four-wavefront build FP32 only
+        if ((c->spec && c->col != 2) || (c->o4 && c->rsize != 4)) {
             if (ms_per_pass)
                 for (int k = 0; k < 5; ++k) ms_per_pass[5 * sched + k] = 0.0;
             continue;
@@ -1322,14 +1334,15 @@ int mcgb_autotune(mcgb_ctx* c, int sweeps, int* best_waves, int* best_schedule,
             set_waves(c, cand[k]);
             if ((rc = sweep_impl(c, sweeps, nullptr))) { c->timing = tm; return rc; }
             if (ms_per_pass) ms_per_pass[5 * sched + k] = c->ms_pass;
-            if (c->ms_pass < best) { best = c->ms_pass; bw = cand[k]; bs = c->spec; }
+            if (c->ms_pass < best) { best = c->ms_pass; bw = cand[k]; bs = sched; }
         }
     }
-    c->spec = bs;
+    c->spec = bs == 1;
+    c->o4 = bs == 2;
     set_waves(c, bw);
     c->timing = tm;
     if (best_waves) *best_waves = bw;
-    if (best_schedule) *best_schedule = bs ? 1 : 0;
+    if (best_schedule) *best_schedule = bs;
     return MCGB_OK;
 }
 

@@ -251,15 +251,20 @@ __device__ inline double gb_pair_f64(const GBd& C, double uix, double uiy, doubl
 // ------------------------------------------------------------------ colloid-rod, FP64
 
 // GBCollRod (MCnptColloid.c:1758-1854); rel = rod centre - colloid centre.  Evaluated in FP64
-// in both precisions (two calls per move, only near the colloid; kept out of line so that the
-// sweep kernel does not carry its registers).  *cap = the 1e8 cap.
-__device__ __attribute__((noinline)) double colloid_f64(const RepConst& C, double rx, double ry, double rz,
-                                     double ex, double ey, double ez, bool* cap) {
-    *cap = false;
+// in both precisions, only near the colloid, and kept out of line so that the sweep kernel does
+// not carry its registers.  cap = the 1e8 cap.  The result comes back by value, in registers
+// (an out-parameter of a call that is not inlined would live in scratch memory); colloid_f64
+// is the inline form with the cap as an out-parameter.
+struct ColloidTerm {
+    double e;
+    int cap;
+};
+__device__ __attribute__((noinline)) ColloidTerm colloid_term(const RepConst& C, double rx, double ry,
+                                                              double rz, double ex, double ey, double ez) {
     double d = sqrt(rx * rx + ry * ry + rz * rz);
     double dij = d - C.R;
-    if (d <= 1e-12) { *cap = true; return 0.0; }
-    if (dij > C.colloid_cut) return 0.0;
+    if (d <= 1e-12) return {0.0, 1};
+    if (dij > C.colloid_cut) return {0.0, 0};
     double cth = (rx / d) * ex + (ry / d) * ey + (rz / d) * ez;
     cth = cth > 1.0 ? 1.0 : (cth < -1.0 ? -1.0 : cth);
     double sin2 = 1.0 - cth * cth;
@@ -272,10 +277,16 @@ __device__ __attribute__((noinline)) double colloid_f64(const RepConst& C, doubl
     if (den_e <= 1e-12) den_e = 1e-12;
     double eps = C.kappa_prime * pow_int_fast(C.one_minus_chip / den_e, C.mu);
     double x = dij - scr + s0;
-    if (x <= 1e-10) { *cap = true; return 0.0; }
+    if (x <= 1e-10) return {0.0, 1};
     double s3 = s0 * s0 * s0, s9 = s3 * s3 * s3;
     double x2 = x * x, x3 = x2 * x, x8 = x3 * x3 * x2, x9 = x8 * x;
-    return eps * (s9 / (45.0 * x9) - s3 / (6.0 * x3) - s9 / (40.0 * d * x8) + s3 / (4.0 * d * x2));
+    return {eps * (s9 / (45.0 * x9) - s3 / (6.0 * x3) - s9 / (40.0 * d * x8) + s3 / (4.0 * d * x2)), 0};
+}
+__device__ inline double colloid_f64(const RepConst& C, double rx, double ry, double rz, double ex,
+                                     double ey, double ez, bool* cap) {
+    const ColloidTerm t = colloid_term(C, rx, ry, rz, ex, ey, ez);
+    *cap = t.cap != 0;
+    return t.e;
 }
 
 __host__ __device__ inline GBf make_gbf(const RepConst& C) {

@@ -104,6 +104,23 @@ struct WavePartial {
     double scale;     // sum |terms| (trace only)
 };
 
+// One trial move as prepared at the start of its chunk: its draws, and the particle's state at
+// that point with the trial orientation and trial colloid term derived from it (valid unless an
+// earlier accepted move of the chunk moved the same particle).  A move reads its record in one
+// batch of LDS loads.
+template <typename Real>
+struct alignas(16) MovePrep {
+    double thr, thr1;           // Metropolis thresholds: log U and (delayed acceptance) stage 1
+    double en;                  // trial-state colloid term (cells near the colloid)
+    uint32_t qx, qy, qz;        // position at the start of the chunk
+    int32_t dqx, dqy, dqz;      // displacement, fixed point
+    int32_t jsel;               // cell-local index of the particle
+    uint32_t flags;             // bit q: move q of the chunk draws the same particle; bit 16: capped trial term
+    Real ux, uy, uz;            // orientation at the start of the chunk
+    Real nx, ny, nz;            // trial orientation from it
+    Real vx, vy, vz;            // rotation draw
+};
+
 template <typename Real, int W>
 struct PassLds {
     static constexpr int kDraws = 8;                 // moves whose draws are staged at a time
@@ -125,17 +142,18 @@ struct PassLds {
     __host__ __device__ static inline int off_draws(int cap) {
         return (off_list(cap) + W * list_stride(cap) * 2 + 15) & ~15;
     }
-    // inside the draws region: the draws of kDraws moves, the wavefront partials, the
-    // surrogate partials, then the move preparation (trial orientations, new-state colloid
-    // terms, flags) and the colloid terms of up to kCache cell particles at their current states
+    // inside the draws region: the prepared records of kDraws moves, the wavefront partials,
+    // the surrogate partials, then the colloid terms of up to kCache cell particles at their
+    // current states and their cap flags
     static constexpr int kCache = 64;
-    static constexpr int kDrawBytes = kDraws * (32 + 3 * (int)sizeof(Real));   // a multiple of 16
+    static constexpr int kDrawBytes = kDraws * (int)sizeof(MovePrep<Real>);   // a multiple of 16
     static constexpr int kSurOff = (kDrawBytes + W * (int)sizeof(WavePartial) + 15) & ~15;
     static constexpr int kPrepOff = (kSurOff + W * (int)sizeof(float) + 15) & ~15;
-    static constexpr int kPrepBytes = kDraws * 8 + kCache * 8 + kDraws * 3 * (int)sizeof(Real) +
-                                      kDraws * 4 + kCache;
+    static constexpr int kPrepBytes = kCache * 8 + kCache;
+    static constexpr int kRawOff = (kPrepOff + kPrepBytes + 15) & ~15;   // raw draws of a chunk
+    static constexpr int kRawBytes = kDraws * (int)sizeof(MoveDraw);
     __host__ __device__ static inline size_t bytes(int cap) {
-        return (size_t)off_draws(cap) + kPrepOff + kPrepBytes + 16;
+        return (size_t)off_draws(cap) + kRawOff + kRawBytes + 16;
     }
 };
 
@@ -208,8 +226,11 @@ template <> struct LaneAcc<double> {
 };
 
 // Trial orientation normalize(u + d_rot v) (MCnptColloid.c:2526-2550; kept when |.|^2 <= 1e-12).
+// Products and sums are not fused into FMAs, so that every kernel, wherever it computes the
+// trial orientation, rounds it the same way (in FP64, as the oracle's plain double arithmetic).
 __device__ inline void rotate(double drot, float ux, float uy, float uz, float vx, float vy,
                               float vz, float& ox, float& oy, float& oz) {
+#pragma clang fp contract(off)
     const float d = (float)drot;
     const float tx = ux + d * vx, ty = uy + d * vy, tz = uz + d * vz;
     const float n2 = tx * tx + ty * ty + tz * tz;
@@ -220,6 +241,7 @@ __device__ inline void rotate(double drot, float ux, float uy, float uz, float v
 }
 __device__ inline void rotate(double drot, double ux, double uy, double uz, double vx, double vy,
                               double vz, double& ox, double& oy, double& oz) {
+#pragma clang fp contract(off)
     const double tx = ux + drot * vx, ty = uy + drot * vy, tz = uz + drot * vz;
     const double n2 = tx * tx + ty * ty + tz * tz;
     if (n2 <= 1e-12) { ox = ux; oy = uy; oz = uz; return; }
@@ -322,7 +344,7 @@ __device__ __forceinline__ void pass_cell(const PassArgs& P, int b, unsigned cha
     if (a >= nact) return;                    // uniform: whole block
     const int tid = threadIdx.x;
     const int lane = tid & 63;
-    const int w = tid >> 6;
+    const int w = uni(tid >> 6);             // wavefront index (uniform)
     const int cap = P.cap;
     Real* __restrict__ Ux = (Real*)P.ux;
     Real* __restrict__ Uy = (Real*)P.uy;
@@ -338,24 +360,12 @@ __device__ __forceinline__ void pass_cell(const PassArgs& P, int b, unsigned cha
     unsigned short* slist = (unsigned short*)(smem + L::off_list(cap)) + w * L::list_stride(cap);
     unsigned char* dp = smem + L::off_draws(cap);
     constexpr int KD = L::kDraws;                              // moves per draw chunk
-    double* sthr = (double*)dp;                                // KD
-    double* sthr1 = sthr + KD;                                 // KD
-    Real* svx = (Real*)(sthr1 + KD);                           // 3 x KD
-    Real* svy = svx + KD;
-    Real* svz = svy + KD;
-    int* sdqx = (int*)(svz + KD);                              // 3 x KD
-    int* sdqy = sdqx + KD;
-    int* sdqz = sdqy + KD;
-    int* sjs = (int*)(sdqz + KD);                              // KD: cell-local particle index
+    MovePrep<Real>* sprep = (MovePrep<Real>*)dp;               // KD prepared moves
     WavePartial* sred = (WavePartial*)(dp + L::kDrawBytes);
     float* ssur = (float*)(dp + L::kSurOff);
-    double* sen = (double*)(dp + L::kPrepOff);                 // KD: new-state colloid term
-    double* scache = sen + KD;                                 // kCache: current colloid terms
-    Real* sunx = (Real*)(scache + L::kCache);                  // 3 x KD: trial orientations
-    Real* suny = sunx + KD;
-    Real* sunz = suny + KD;
-    uint32_t* sflag = (uint32_t*)(sunz + KD);                  // KD: earlier moves of the particle | cap << 16
-    unsigned char* scc = (unsigned char*)(sflag + KD);         // kCache: cap flags of scache
+    double* scache = (double*)(dp + L::kPrepOff);              // kCache: current colloid terms
+    unsigned char* scc = (unsigned char*)(scache + L::kCache); // kCache: cap flags of scache
+    uint2* sdraw2 = (uint2*)(dp + L::kRawOff);                 // KD draws as read from HBM
     MlpRegs MR;
     if (DA) mlp_load(P.mlp, lane, MR);
 
@@ -376,6 +386,18 @@ __device__ __forceinline__ void pass_cell(const PassArgs& P, int b, unsigned cha
     const int cell = cx + nx * (cy + ny * cz);
     const int* cs = P.cell_start + C.cs_off;
     const uint32_t gx = D.g[0], gy = D.g[1], gz = D.g[2];
+
+    // the draws of the cell's moves (k_draws), KD moves (KD * 56 contiguous bytes) at a time,
+    // spread over wavefront 0 as 8-byte words: the first chunk's loads are issued here, ahead of
+    // the staging, and every later chunk's while the chunk before it runs
+    const int k = D.k;
+    const size_t tbase = ((size_t)P.pass * (size_t)nact + (size_t)a) * (size_t)k;
+    const MoveDraw* __restrict__ drw = P.draws + C.rec_off + tbase;
+    const uint2* __restrict__ drw2 = (const uint2*)drw;
+    constexpr int kW2 = (int)(sizeof(MoveDraw) / 8);          // 8-byte words per draw
+    static_assert(sizeof(MoveDraw) % 8 == 0 && KD * kW2 <= 64, "a chunk's draws fit one wavefront");
+    uint2 dpre = make_uint2(0u, 0u);
+    if (w == 0 && lane < min(k, KD) * kW2) dpre = drw2[lane];
 
     // stencil table: NS = (2R+1)^3 neighbour cells in (dz, dy, dx) order, the active cell at
     // NS/2; two cells per lane, one inclusive scan per half
@@ -407,7 +429,7 @@ __device__ __forceinline__ void pass_cell(const PassArgs& P, int b, unsigned cha
         if (lane == 0) s_off[0] = 0;
     }
     __syncthreads();
-    const int S = s_off[NS];
+    const int S = uni(s_off[NS]);
     const int coff = s_off[NS / 2];
     const int nc = s_off[NS / 2 + 1] - coff;
     const int cstart = part_off + s_start[NS / 2];
@@ -428,9 +450,9 @@ __device__ __forceinline__ void pass_cell(const PassArgs& P, int b, unsigned cha
     // the cell).
     const int S_pad = (S + 63) & ~63;
     if (staged) {
-        // four records per thread per step: the slot searches, then every HBM load, then the
-        // LDS stores (24 loads in flight per thread instead of 6)
-        constexpr int kB = 4;
+        // kB records per thread per step: the slot searches, then every HBM load, then the LDS
+        // stores (6 kB loads in flight per thread; with W = 2, 768 records in one step)
+        constexpr int kB = W == 2 ? 6 : 4;
         for (int e0 = tid; e0 < S; e0 += kB * 64 * W) {
             int gi[kB];
 #pragma unroll
@@ -460,9 +482,6 @@ __device__ __forceinline__ void pass_cell(const PassArgs& P, int b, unsigned cha
         for (int e = S + tid; e < S_pad; e += 64 * W) sq[e] = Pos3{fx, fy, fz};
     }
 
-    const int k = D.k;
-    const size_t tbase = ((size_t)P.pass * (size_t)nact + (size_t)a) * (size_t)k;
-    const MoveDraw* __restrict__ drw = P.draws + C.rec_off + tbase;
     constexpr bool tracing = TR;
     int n_att = 0, n_acc = 0, n_left = 0, n_hard = 0, n_coll = 0, n_s1 = 0, n_s2 = 0;
     int dcap_sum = 0, dccap_sum = 0;
@@ -515,42 +534,44 @@ __device__ __forceinline__ void pass_cell(const PassArgs& P, int b, unsigned cha
         for (int m0 = 0; m0 < k; m0 += KD) {
             __syncthreads();
             int myj = -1;
-            uint32_t mycap = 0;
-            if (tid < KD && m0 + tid < k) {   // the draws of up to KD moves (k_draws)
-                const MoveDraw& d = drw[m0 + tid];
+            MovePrep<Real> mp = {};
+            if (w == 0) {
+                // this chunk's draws into LDS (read back below by the same wavefront), the
+                // next chunk's loads issued
+                if (lane < min(k - m0, KD) * kW2) sdraw2[lane] = dpre;
+                if (lane < min(k - m0 - KD, KD) * kW2) dpre = drw2[(size_t)(m0 + KD) * kW2 + lane];
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            }
+            if (tid < KD && m0 + tid < k) {   // the draws of up to KD moves
+                const MoveDraw d = ((const MoveDraw*)sdraw2)[tid];
                 const int js = (int)(((uint64_t)d.j * (uint32_t)nc) >> 32);
-                const Real vx = (Real)d.v[0], vy = (Real)d.v[1], vz = (Real)d.v[2];
                 myj = js;
-                sjs[tid] = js;
-                sdqx[tid] = d.dq[0];
-                sdqy[tid] = d.dq[1];
-                sdqz[tid] = d.dq[2];
-                svx[tid] = vx;
-                svy[tid] = vy;
-                svz[tid] = vz;
-                sthr[tid] = d.thr;
-                sthr1[tid] = d.thr1;
+                mp.thr = d.thr;
+                mp.thr1 = d.thr1;
+                mp.dqx = d.dq[0]; mp.dqy = d.dq[1]; mp.dqz = d.dq[2];
+                mp.jsel = js;
+                mp.vx = (Real)d.v[0]; mp.vy = (Real)d.v[1]; mp.vz = (Real)d.v[2];
                 if (staged) {
-                    // the move's trial orientation (and trial colloid term) from the
-                    // particle's state at the start of the chunk
+                    // the particle's state at the start of the chunk, the move's trial
+                    // orientation (and trial colloid term) from it
+                    const Pos3 qj = sq[coff + js];
                     const R3 uj = su[coff + js];
-                    Real ox, oy, oz;
-                    rotate(drot, uj.x, uj.y, uj.z, vx, vy, vz, ox, oy, oz);
-                    sunx[tid] = ox;
-                    suny[tid] = oy;
-                    sunz[tid] = oz;
+                    mp.qx = qj.x; mp.qy = qj.y; mp.qz = qj.z;
+                    mp.ux = uj.x; mp.uy = uj.y; mp.uz = uj.z;
+                    rotate(drot, uj.x, uj.y, uj.z, mp.vx, mp.vy, mp.vz, mp.nx, mp.ny, mp.nz);
                     if (ccache) {
-                        const Pos3 qj = sq[coff + js];
                         bool cn;
-                        sen[tid] = colloid_f64(C, rel(qj.x + (uint32_t)d.dq[0], X.s0),
-                                               rel(qj.y + (uint32_t)d.dq[1], X.s1),
-                                               rel(qj.z + (uint32_t)d.dq[2], X.s2), (double)ox,
-                                               (double)oy, (double)oz, &cn);
-                        mycap = cn ? 1u << 16 : 0u;
+                        mp.en = colloid_f64(C, rel(qj.x + (uint32_t)d.dq[0], X.s0),
+                                            rel(qj.y + (uint32_t)d.dq[1], X.s1),
+                                            rel(qj.z + (uint32_t)d.dq[2], X.s2), (double)mp.nx,
+                                            (double)mp.ny, (double)mp.nz, &cn);
+                        mp.flags = cn ? 1u << 16 : 0u;
                     }
                 }
             }
-            if (staged && w == 0) {
+            if (w == 0) {
                 // bit q: move q of the chunk (q earlier than this one) draws the same particle
                 uint32_t dup = 0;
 #pragma unroll
@@ -558,23 +579,30 @@ __device__ __forceinline__ void pass_cell(const PassArgs& P, int b, unsigned cha
                     const int jq = __shfl(myj, q);
                     if (q < tid && jq == myj) dup |= 1u << q;
                 }
-                if (tid < KD) sflag[tid] = dup | mycap;
+                mp.flags |= dup;
+                if (tid < KD) sprep[tid] = mp;
             }
             __syncthreads();
             uint32_t acc_mask = 0;            // moves of this chunk accepted so far
             const int mend = min(k, m0 + KD);
             for (int m = m0; m < mend; ++m) {
                 const int ml = m - m0;
-                const int jsel = uni(sjs[ml]);
+                // the whole record in one batch of LDS loads (reading fields where they are
+                // used puts LDS latency on the move's critical path)
+                const MovePrep<Real> pm = sprep[ml];
+                const int jsel = uni(pm.jsel);
                 // the chunk's preparation of this move holds unless an earlier accepted move
                 // of the chunk moved the same particle
-                const uint32_t fl = staged ? uni(sflag[ml]) : 0u;
+                const uint32_t fl = uni(pm.flags);
                 const bool fresh = staged && (fl & acc_mask & 0xffffu) == 0u;
                 const int li = coff + jsel;          // flat stencil index of the moved particle
                 const int gsel = cstart + jsel;      // its global slot
                 uint32_t qix, qiy, qiz;
                 Real uix, uiy, uiz;
-                if (staged) {
+                if (fresh) {
+                    qix = uni(pm.qx); qiy = uni(pm.qy); qiz = uni(pm.qz);
+                    uix = uni(pm.ux); uiy = uni(pm.uy); uiz = uni(pm.uz);
+                } else if (staged) {
                     const Pos3 qi = sq[li];
                     const R3 ui = su[li];
                     qix = uni(qi.x); qiy = uni(qi.y); qiz = uni(qi.z);
@@ -583,7 +611,7 @@ __device__ __forceinline__ void pass_cell(const PassArgs& P, int b, unsigned cha
                     qix = P.qx[gsel]; qiy = P.qy[gsel]; qiz = P.qz[gsel];
                     uix = Ux[gsel]; uiy = Uy[gsel]; uiz = Uz[gsel];
                 }
-                const int dqx = uni(sdqx[ml]), dqy = uni(sdqy[ml]), dqz = uni(sdqz[ml]);
+                const int dqx = uni(pm.dqx), dqy = uni(pm.dqy), dqz = uni(pm.dqz);
                 const uint32_t qnx = qix + (uint32_t)dqx, qny = qiy + (uint32_t)dqy,
                                qnz = qiz + (uint32_t)dqz;
                 int outcome;
@@ -604,9 +632,9 @@ __device__ __forceinline__ void pass_cell(const PassArgs& P, int b, unsigned cha
                 } else {
                     Real unx, uny, unz;
                     if (fresh) {
-                        unx = uni(sunx[ml]); uny = uni(suny[ml]); unz = uni(sunz[ml]);
+                        unx = uni(pm.nx); uny = uni(pm.ny); unz = uni(pm.nz);
                     } else {
-                        rotate(drot, uix, uiy, uiz, svx[ml], svy[ml], svz[ml], unx, uny, unz);
+                        rotate(drot, uix, uiy, uiz, pm.vx, pm.vy, pm.vz, unx, uny, unz);
                         unx = uni(unx); uny = uni(uny); unz = uni(unz);
                     }
                     LaneAcc<Real> acc;
@@ -618,24 +646,29 @@ __device__ __forceinline__ void pass_cell(const PassArgs& P, int b, unsigned cha
                         // within the cutoff of either state lies within rc + |dq|/2 of their
                         // midpoint (triangle inequality), so one FP32 distance per candidate
                         const MidFilter M = mid_filter(qix, qiy, qiz, dqx, dqy, dqz, FC);
-                        // two candidates per lane per step, branch-free: the padded tail holds
-                        // sentinels the filter rejects; the second wavefront-wide group of a
-                        // step exists when the stencil reaches it (uniform), else lane a's
-                        // record is read again and dropped
+                        // up to four wavefront-wide groups of 64 records per step (this
+                        // wavefront's groups w, w + W, ...), their LDS loads issued before the
+                        // tests; the padded tail holds sentinels the filter rejects
                         auto scan = [&](auto iso) {
-                            for (int e0 = w * 64; e0 < S_pad; e0 += 128 * W) {
-                                const bool hasb = e0 + 64 * W < S_pad;
-                                const int ea = e0 + lane, eb = e0 + 64 * W + lane;
-                                const Pos3 pa = sq[ea], pb = sq[hasb ? eb : ea];
-                                const bool ka = M.near(pa, iso) & (ea != li);
-                                const bool kb = M.near(pb, iso) & (eb != li) & hasb;
-                                const unsigned long long ma = __ballot(ka);
-                                const unsigned long long mb = __ballot(kb);
-                                const int na = __popcll(ma);
-                                // every lane writes: unselected ones into the spare slot
-                                slist[ka ? base + lane_rank(ma) : lcap] = (unsigned short)ea;
-                                slist[kb ? base + na + lane_rank(mb) : lcap] = (unsigned short)eb;
-                                base += na + __popcll(mb);
+                            constexpr int kG = 4;
+                            const int ng = S_pad >> 6;
+                            for (int g0 = w; g0 < ng; g0 += kG * W) {
+                                const int nv = min(kG, (ng - g0 + W - 1) / W);   // uniform
+                                Pos3 p[kG];
+#pragma unroll
+                                for (int j = 0; j < kG; ++j)
+                                    if (j < nv) p[j] = sq[(g0 + j * W) * 64 + lane];
+#pragma unroll
+                                for (int j = 0; j < kG; ++j)
+                                    if (j < nv) {
+                                        const int e = (g0 + j * W) * 64 + lane;
+                                        const bool kk = M.near(p[j], iso) & (e != li);
+                                        const unsigned long long mk = __builtin_amdgcn_ballot_w64(kk);
+                                        // every lane writes: unselected ones into the spare slot
+                                        const int rank = lane_rank(mk);
+                                        slist[kk ? base + rank : lcap] = (unsigned short)e;
+                                        base += __popcll(mk);
+                                    }
                             }
                         };
                         if (M.iso) scan(std::true_type{});
@@ -654,7 +687,7 @@ __device__ __forceinline__ void pass_cell(const PassArgs& P, int b, unsigned cha
                         const double eo = scache[jsel];
                         const bool co = scc[jsel] != 0;
                         if (fresh) {
-                            cen = sen[ml];
+                            cen = pm.en;
                             ccn = (fl >> 16) & 1u;
                         } else {
                             cen = colloid_f64(C, rnx, rny, rnz, (double)unx, (double)uny,
@@ -727,7 +760,7 @@ __device__ __forceinline__ void pass_cell(const PassArgs& P, int b, unsigned cha
                         __syncthreads();
                         for (int q = 0; q < W; ++q) sur += (double)ssur[q];
                         const double dEs = (double)ccd * 1.0e8 + cacc + sur;
-                        stage1 = dEs <= sthr1[ml];
+                        stage1 = dEs <= pm.thr1;
                         exact = stage1 || P.da_judge;
                     }
                     if (staged && exact) {
@@ -799,7 +832,7 @@ __device__ __forceinline__ void pass_cell(const PassArgs& P, int b, unsigned cha
                     bool accept;
                     if (DA) {
                         // stage 2: the exact GB pair terms against the surrogate's
-                        accept = stage1 && (double)capd * 1.0e9 + (dpair - sur) <= sthr[ml];
+                        accept = stage1 && (double)capd * 1.0e9 + (dpair - sur) <= pm.thr;
                         if (exact && capd == 0 && tid == 0) {
                             const float x = (float)((sur - dpair) / C.temperature) / kDaRange;
                             int bin = (int)floorf((x + 1.0f) * (0.5f * kDaBins));
@@ -810,7 +843,7 @@ __device__ __forceinline__ void pass_cell(const PassArgs& P, int b, unsigned cha
                         else if (!accept) ++n_s2;
                     } else {
                         // log U <= -beta dE (MCnptColloid.c:2574)
-                        accept = (double)capd * 1.0e9 + (double)ccd * 1.0e8 + dfin <= sthr[ml];
+                        accept = (double)capd * 1.0e9 + (double)ccd * 1.0e8 + dfin <= pm.thr;
                     }
                     if (accept) {
                         outcome = MCGB_MOVE_ACCEPT;
@@ -879,6 +912,15 @@ __global__ __launch_bounds__(64 * W) void k_pass(PassArgs P) {
     } else {
         pass_cell<Real, W, GLOBAL, TR, DA>(P, blockIdx.x, smem);
     }
+}
+
+// The FP32 sequential pass compiled for four wavefronts per SIMD (at most 128 VGPRs): more
+// workgroups resident per CU, less latency hiding scheduled into each; which one is faster
+// depends on the number of active cells per pass (pass schedule 2, chosen by the autotuner).
+template <int W>
+__global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4))) void k_pass_o4(PassArgs P) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    pass_cell<float, W, false, false, false>(P, blockIdx.x, smem);
 }
 
 }  // namespace mcgb

@@ -345,7 +345,11 @@ def main():
         "ms_get_state": 1e3 * t_get,
         "ms_get_state_repeat": 1e3 * t_get2,
         "ess_per_s": {"E/N": ess_per_s(e, dt), "P2": ess_per_s(p2, dt), "samples": int(len(e))},
+        # |running total - FP64 re-summation| at the re-summations (recorded after sweeps whose
+        # accepted moves changed the finite energy by <= 1e10 in total), beside the total it is
+        # a difference of (caps included)
         "max_fp64_drift": st["max_drift"],
+        "energy_total": st["energy"],
         "resummations_timed": st["resummations"],
         "relax_sweeps": args.relax,
         "relax_s": t_relax,

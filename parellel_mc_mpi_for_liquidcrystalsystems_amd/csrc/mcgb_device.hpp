@@ -170,39 +170,81 @@ __host__ __device__ inline int grid_dim(double L, double rc, int col) {
 // exp2(mu (log2 N' - log2 D') - (nu/2) log2 D), D = 1 - chi^2 c^2.  The distance r comes from
 // the exact FP64 r^2 (r2d) with one FP64 Newton step on the hardware rsqrt, and
 // r - sigma + sigma0 is formed in FP64: the remaining error is that of sigma (a few FP32 ulps).
-// Transcendentals: v_rsq x2, v_rcp x2, v_log x3, v_exp x1.  *cap = the 1e9 cap.
+// Transcendentals: v_rsq x2, v_rcp x2, v_log x3, v_exp x1.  cap = the 1e9 cap.
+// Written for Q pairs that share e_i, each step done for every pair before the next step, so
+// that Q independent dependency chains interleave; Q = 1 is the single-pair function.  Every
+// pair goes through the same operations whatever Q is.
+#define MCGB_EACH for (int q = 0; q < Q; ++q)
+template <int Q>
+__device__ inline void gb_pair_q_f32(const GBf& C, float uix, float uiy, float uiz,
+                                     const float* ujx, const float* ujy, const float* ujz,
+                                     const float* dx, const float* dy, const float* dz,
+                                     const double* r2d, float* u, bool* cap) {
+    float rinv[Q], ir2[Q], wx[Q], wy[Q], wz[Q], w2[Q], tau[Q], ab[Q], tt[Q];
+    float n1[Q], d1[Q], n2[Q], d2[Q], eps[Q], sigma[Q];
+    double dist[Q];
+#pragma unroll
+    MCGB_EACH {
+        const float r2 = (float)r2d[q];
+        rinv[q] = __builtin_amdgcn_rsqf(r2);
+        ir2[q] = rinv[q] * rinv[q];
+    }
+    // w = e_i x e_j, p = e_i x dr, q = e_j x dr
+#pragma unroll
+    MCGB_EACH {
+        wx[q] = uiy * ujz[q] - uiz * ujy[q];
+        wy[q] = uiz * ujx[q] - uix * ujz[q];
+        wz[q] = uix * ujy[q] - uiy * ujx[q];
+    }
+#pragma unroll
+    MCGB_EACH {
+        const float px = uiy * dz[q] - uiz * dy[q], py = uiz * dx[q] - uix * dz[q],
+                    pz = uix * dy[q] - uiy * dx[q];
+        const float qx = ujy[q] * dz[q] - ujz[q] * dy[q], qy = ujz[q] * dx[q] - ujx[q] * dz[q],
+                    qz = ujx[q] * dy[q] - ujy[q] * dx[q];
+        w2[q] = wx[q] * wx[q] + wy[q] * wy[q] + wz[q] * wz[q];
+        tau[q] = dx[q] * wx[q] + dy[q] * wy[q] + dz[q] * wz[q];
+        ab[q] = (px * px + py * py + pz * pz + qx * qx + qy * qy + qz * qz) * ir2[q];
+        tt[q] = tau[q] * tau[q] * ir2[q];
+    }
+    // shape g(chi) and energy g(chi') as N / D
+#pragma unroll
+    MCGB_EACH {
+        n1[q] = C.g1c0 + C.g1c1 * ab[q] + C.chi2 * tt[q];
+        d1[q] = C.g1d0 + C.chi2 * w2[q];
+        n2[q] = C.g2c0 + C.g2c1 * ab[q] + C.chip2 * tt[q];
+        d2[q] = C.g2d0 + C.chip2 * w2[q];
+    }
+#pragma unroll
+    MCGB_EACH {
+        eps[q] = __builtin_amdgcn_exp2f(
+            C.mu * (__builtin_amdgcn_logf(n2[q]) - __builtin_amdgcn_logf(d2[q])) -
+            C.half_nu * __builtin_amdgcn_logf(d1[q]));
+        sigma[q] = C.sigma0 * __builtin_amdgcn_rsqf(n1[q] * __builtin_amdgcn_rcpf(d1[q]));
+    }
+    // r in FP64: one Newton step on the FP32 reciprocal square root
+#pragma unroll
+    MCGB_EACH {
+        const double y = (double)rinv[q];
+        const double y1 = y * (1.5 - 0.5 * r2d[q] * y * y);
+        dist[q] = (r2d[q] * y1 - (double)sigma[q]) + C.sigma0d;
+        cap[q] = dist[q] < 0.001;
+    }
+#pragma unroll
+    MCGB_EACH {
+        const float rho = C.sigma0 * __builtin_amdgcn_rcpf((float)dist[q]);
+        const float rho2 = rho * rho;
+        const float rho6 = rho2 * rho2 * rho2;
+        u[q] = C.four_eps0 * eps[q] * rho6 * (rho6 - 1.0f);
+    }
+}
+#undef MCGB_EACH
 __device__ inline float gb_pair_f32(const GBf& C, float uix, float uiy, float uiz,
                                     float ujx, float ujy, float ujz, float dx, float dy, float dz,
                                     double r2d, bool* cap) {
-    const float r2 = (float)r2d;
-    const float rinv = __builtin_amdgcn_rsqf(r2);
-    const float ir2 = rinv * rinv;
-    // w = e_i x e_j, p = e_i x dr, q = e_j x dr
-    const float wx = uiy * ujz - uiz * ujy, wy = uiz * ujx - uix * ujz, wz = uix * ujy - uiy * ujx;
-    const float px = uiy * dz - uiz * dy, py = uiz * dx - uix * dz, pz = uix * dy - uiy * dx;
-    const float qx = ujy * dz - ujz * dy, qy = ujz * dx - ujx * dz, qz = ujx * dy - ujy * dx;
-    const float w2 = wx * wx + wy * wy + wz * wz;
-    const float tau = dx * wx + dy * wy + dz * wz;
-    const float ab = (px * px + py * py + pz * pz + qx * qx + qy * qy + qz * qz) * ir2;
-    const float tt = tau * tau * ir2;
-    // shape g(chi) and energy g(chi') as N / D
-    const float n1 = C.g1c0 + C.g1c1 * ab + C.chi2 * tt;
-    const float d1 = C.g1d0 + C.chi2 * w2;
-    const float n2 = C.g2c0 + C.g2c1 * ab + C.chip2 * tt;
-    const float d2 = C.g2d0 + C.chip2 * w2;
-    const float eps = __builtin_amdgcn_exp2f(
-        C.mu * (__builtin_amdgcn_logf(n2) - __builtin_amdgcn_logf(d2)) -
-        C.half_nu * __builtin_amdgcn_logf(d1));
-    const float sigma = C.sigma0 * __builtin_amdgcn_rsqf(n1 * __builtin_amdgcn_rcpf(d1));
-    // r in FP64: one Newton step on the FP32 reciprocal square root
-    const double y = (double)rinv;
-    const double y1 = y * (1.5 - 0.5 * r2d * y * y);
-    const double dist = (r2d * y1 - (double)sigma) + C.sigma0d;
-    *cap = dist < 0.001;
-    const float rho = C.sigma0 * __builtin_amdgcn_rcpf((float)dist);
-    const float rho2 = rho * rho;
-    const float rho6 = rho2 * rho2 * rho2;
-    return C.four_eps0 * eps * rho6 * (rho6 - 1.0f);
+    float u;
+    gb_pair_q_f32<1>(C, uix, uiy, uiz, &ujx, &ujy, &ujz, &dx, &dy, &dz, &r2d, &u, cap);
+    return u;
 }
 
 // ------------------------------------------------------------------ GB pair, FP64

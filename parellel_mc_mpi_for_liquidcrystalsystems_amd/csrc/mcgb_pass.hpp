@@ -191,6 +191,51 @@ __device__ inline bool pair_energy(const GBd& G, const PairGeom& X, double uix, 
     return true;
 }
 
+// The same without the branch: the term is computed for every pair, the result says whether it
+// counts (within the cutoff).  Lets two pairs' dependency chains interleave.
+__device__ inline bool pair_energy_all(const GBf& G, const PairGeom& X, float uix, float uiy,
+                                       float uiz, float ujx, float ujy, float ujz, int ox, int oy,
+                                       int oz, float* u, bool* cap) {
+    const double dx = (double)ox * X.s0, dy = (double)oy * X.s1, dz = (double)oz * X.s2;
+    const double r2 = dx * dx + dy * dy + dz * dz;
+    *u = gb_pair_f32(G, uix, uiy, uiz, ujx, ujy, ujz, (float)dx, (float)dy, (float)dz, r2, cap);
+    return r2 < X.rc2;
+}
+__device__ inline bool pair_energy_all(const GBd& G, const PairGeom& X, double uix, double uiy,
+                                       double uiz, double ujx, double ujy, double ujz, int ox,
+                                       int oy, int oz, double* u, bool* cap) {
+    const double dx = (double)ox * X.s0, dy = (double)oy * X.s1, dz = (double)oz * X.s2;
+    const double r2 = dx * dx + dy * dy + dz * dz;
+    *u = gb_pair_f64(G, uix, uiy, uiz, ujx, ujy, ujz, dx, dy, dz, r2, cap);
+    return r2 < X.rc2;
+}
+
+// Two pairs of the same moved particle (orientation ui) at once: in FP32 their dependency
+// chains interleave (gb_pair_q_f32<2>); in FP64 they are evaluated one after the other.
+__device__ inline void pair_energy_two(const GBf& G, const PairGeom& X, float uix, float uiy,
+                                       float uiz, const float* ujx, const float* ujy,
+                                       const float* ujz, const int* ox, const int* oy,
+                                       const int* oz, float* u, bool* cap, bool* in) {
+    float fx[2], fy[2], fz[2];
+    double r2[2];
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+        const double dx = (double)ox[q] * X.s0, dy = (double)oy[q] * X.s1, dz = (double)oz[q] * X.s2;
+        r2[q] = dx * dx + dy * dy + dz * dz;
+        in[q] = r2[q] < X.rc2;
+        fx[q] = (float)dx; fy[q] = (float)dy; fz[q] = (float)dz;
+    }
+    gb_pair_q_f32<2>(G, uix, uiy, uiz, ujx, ujy, ujz, fx, fy, fz, r2, u, cap);
+}
+__device__ inline void pair_energy_two(const GBd& G, const PairGeom& X, double uix, double uiy,
+                                       double uiz, const double* ujx, const double* ujy,
+                                       const double* ujz, const int* ox, const int* oy,
+                                       const int* oz, double* u, bool* cap, bool* in) {
+    for (int q = 0; q < 2; ++q)
+        in[q] = pair_energy_all(G, X, uix, uiy, uiz, ujx[q], ujy[q], ujz[q], ox[q], oy[q], oz[q],
+                                &u[q], &cap[q]);
+}
+
 // Per-lane accumulator of a finite energy difference: the FP32 pair terms are summed in FP64
 // (each lane adds only a few terms per move; FP64 adds and DPP moves are full rate), reduced
 // across lanes in a fixed DPP order and across wavefronts in a fixed order: deterministic.
@@ -764,25 +809,37 @@ __device__ __forceinline__ void pass_cell(const PassArgs& P, int b, unsigned cha
                         exact = stage1 || P.da_judge;
                     }
                     if (staged && exact) {
+                        // items it and it + 64 per lane and step (the same state: 64 is even),
+                        // both evaluated without a branch so that their dependency chains
+                        // interleave; each lane adds its terms in item order, as one item per
+                        // step would.  A missing second item repeats the first and is dropped.
                         const int items = 2 * base;
-                        for (int it = lane; it < items; it += 64) {
-                            const int e = slist[it >> 1];
+                        for (int it = lane; it < items; it += 128) {
+                            const bool hb = it + 64 < items;
+                            const int ea = slist[it >> 1], eb = slist[(hb ? it + 64 : it) >> 1];
                             const bool nw = it & 1;
                             const uint32_t px = nw ? qnx : qix, py = nw ? qny : qiy, pz = nw ? qnz : qiz;
-                            const Pos3 pe = sq[e];
-                            const R3 ue = su[e];
-                            bool cp;
-                            Real u;
-                            if (pair_energy(G, X, nw ? unx : uix, nw ? uny : uiy, nw ? unz : uiz,
-                                            ue.x, ue.y, ue.z, (int)(pe.x - px),
-                                            (int)(pe.y - py), (int)(pe.z - pz), &u, &cp)) {
-                                ++my_pairs;
-                                if (cp) cpl += nw ? 1 : -1;
-                                else {
-                                    acc.add(nw ? u : -u);
-                                    if (TR) scl += u < 0 ? -u : u;
-                                }
-                            }
+                            const Real ox = nw ? unx : uix, oy = nw ? uny : uiy, oz = nw ? unz : uiz;
+                            const Pos3 pa = sq[ea], pb = sq[eb];
+                            const R3 ua = su[ea], ub = su[eb];
+                            const Real jx[2] = {ua.x, ub.x}, jy[2] = {ua.y, ub.y}, jz[2] = {ua.z, ub.z};
+                            const int dxs[2] = {(int)(pa.x - px), (int)(pb.x - px)};
+                            const int dys[2] = {(int)(pa.y - py), (int)(pb.y - py)};
+                            const int dzs[2] = {(int)(pa.z - pz), (int)(pb.z - pz)};
+                            Real v2[2];
+                            bool c2[2], in2[2];
+                            pair_energy_two(G, X, ox, oy, oz, jx, jy, jz, dxs, dys, dzs, v2, c2, in2);
+                            // selects, not branches: a branch would let the compiler sink
+                            // each pair's chain into its own block (adding 0 leaves a sum as is)
+                            const bool ia = in2[0], ib = in2[1] & hb;
+                            const bool fa = ia & !c2[0], fb = ib & !c2[1];
+                            const Real ta = fa ? (nw ? v2[0] : -v2[0]) : (Real)0;
+                            const Real tb = fb ? (nw ? v2[1] : -v2[1]) : (Real)0;
+                            my_pairs += (int)ia + (int)ib;
+                            cpl += ((int)(ia & c2[0]) + (int)(ib & c2[1])) * (nw ? 1 : -1);
+                            acc.add(ta);
+                            acc.add(tb);
+                            if (TR) scl += (ta < 0 ? -ta : ta) + (tb < 0 ? -tb : tb);
                         }
                     } else if (!staged) {
                         // HBM path (stencil beyond the staging capacity): both states per
@@ -900,8 +957,11 @@ __device__ __forceinline__ void pass_cell(const PassArgs& P, int b, unsigned cha
     }
 }
 
+// At most three wavefronts per SIMD: with more allowed, the 1331 workgroups of a pass at
+// N = 256k (W = 2) run 10% slower (measured); schedule 2 (k_pass_o4) is the four-wavefront
+// build for workloads that gain from it.
 template <typename Real, int W, bool GLOBAL, bool TR, bool DA = false>
-__global__ __launch_bounds__(64 * W) void k_pass(PassArgs P) {
+__global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(1, 3))) void k_pass(PassArgs P) {
     extern __shared__ __align__(16) unsigned char smem[];
     if (GLOBAL) {
         const int nov = P.ovf_count[P.pass];
@@ -915,8 +975,8 @@ __global__ __launch_bounds__(64 * W) void k_pass(PassArgs P) {
 }
 
 // The FP32 sequential pass compiled for four wavefronts per SIMD (at most 128 VGPRs): more
-// workgroups resident per CU, less latency hiding scheduled into each; which one is faster
-// depends on the number of active cells per pass (pass schedule 2, chosen by the autotuner).
+// workgroups resident per CU; which one is faster depends on the number of active cells per
+// pass (pass schedule 2, chosen by the autotuner).
 template <int W>
 __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4))) void k_pass_o4(PassArgs P) {
     extern __shared__ __align__(16) unsigned char smem[];

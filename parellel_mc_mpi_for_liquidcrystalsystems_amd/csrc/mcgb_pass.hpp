@@ -902,6 +902,9 @@ __device__ __forceinline__ void pass_cell(const PassArgs& P, int b, unsigned cha
                         // log U <= -beta dE (MCnptColloid.c:2574)
                         accept = (double)capd * 1.0e9 + (double)ccd * 1.0e8 + dfin <= pm.thr;
                     }
+                    // every lane computed the same decision: held as a scalar, the branches
+                    // on it (and on what it feeds, the chunk's accepted-move mask) are uniform
+                    accept = uni((int)accept) != 0;
                     if (accept) {
                         outcome = MCGB_MOVE_ACCEPT;
                         ++n_acc;
@@ -909,7 +912,7 @@ __device__ __forceinline__ void pass_cell(const PassArgs& P, int b, unsigned cha
                         dccap_sum += ccd;
                         dfin_sum += dfin;
                         moved += fabs(dfin);
-                        acc_mask |= 1u << ml;
+                        acc_mask = uni(acc_mask | (1u << ml));
                         if (tid == 0) {
                             if (staged) {
                                 sq[li] = Pos3{qnx, qny, qnz};

@@ -122,15 +122,16 @@ def test_colloid_matches_reference_vectors(name, R):
 
 
 def make_pair(N=1300, rho=0.25, R=0.0, T=1.0, precision=64, seed=5, resum=0, nthreads=4,
-              stage_capacity=0, start="layered", colors=2, **kw):
+              stage_capacity=0, start="layered", colors=2, cutoff=4.0, **kw):
     if start == "layered":
         q, u, L = layered_system(N, rho, colloid_radius=R, seed=seed)
     else:
         q, u, L = lattice_system(N, rho, colloid_radius=R, seed=seed)
     p = m.gb_params(N, rho, T, colloid_radius=R, precision=precision, seed=900 + seed,
                     energy_check_interval=resum, stage_capacity=stage_capacity,
-                    sublattice_colors=colors, **kw)
-    c = oc.consts(temperature=T, colloid_radius=R, colloid_on=1 if R > 0 else 0, colors=colors)
+                    sublattice_colors=colors, cutoff=cutoff, **kw)
+    c = oc.consts(temperature=T, colloid_radius=R, colloid_on=1 if R > 0 else 0, colors=colors,
+                  cutoff=cutoff)
     s = oc.System(c, q, u, L, seed=900 + seed, nthreads=nthreads, resum_interval=resum)
     ctx = m.Context(p)
     ctx.set_state_fixed(q, u, L)
@@ -189,6 +190,40 @@ def test_fp64_sweeps_match_oracle_move_by_move(R):
         assert st["candidates"] == s.counters[4] and st["pairs"] == s.counters[5]
         assert st["cap_pairs"] == s.energy[0] and st["cap_colloid"] == s.energy[1]
         assert st["energy_finite"] == pytest.approx(s.energy[2], rel=1e-10, abs=1e-6)
+
+
+def test_fp64_dense_cells_near_colloid_match_oracle():
+    # cutoff 6: a 4^3 grid of cells 6.8 wide holding ~90 particles each.  Cells near the
+    # colloid with more than 64 particles evaluate both colloid terms per move, the others keep
+    # them in the LDS cache; stencils beyond the staging capacity go to the HBM pass.  Decisions
+    # and states stay exact against the oracle.
+    ctx, s, c = make_pair(N=6000, rho=0.3, R=3.0, precision=64, seed=31, cutoff=6.0)
+    with ctx:
+        cs, _, _, _, dims = ctx.cell_list(0)
+        counts = np.diff(cs)
+        assert tuple(dims) == (4, 4, 4) and counts.max() > 64 and counts.min() <= 64
+        for sweep in range(2):
+            print(f"dense cells: sweep {sweep}", flush=True)
+            tr_g, info_g = ctx.sweep_traced()
+            tr_o, info_o, _ = s.sweep(trace=True)
+            assert np.array_equal(info_g, info_o)
+            assert np.array_equal(tr_g["outcome"], tr_o["outcome"]), f"sweep {sweep}"
+            assert np.array_equal(tr_g["dcap"], tr_o["dcap"]) and np.array_equal(tr_g["dccap"], tr_o["dccap"])
+            err = np.abs(tr_g["dfin"] - tr_o["dfin"]) / np.maximum(tr_o["scale"], 1.0)
+            assert err.max() <= 1e-12
+        q, u, L = ctx.get_state_fixed()
+        for d in range(3):
+            assert np.array_equal(q[d], s.q[d])
+            np.testing.assert_allclose(u[d], s.u[d], rtol=0, atol=1e-12)
+        # the same moves untraced (the kernel the sweeps run) land on the same state
+        ctx.sweep(2)
+        for _ in range(2):
+            s.sweep()
+        q, u, L = ctx.get_state_fixed()
+        for d in range(3):
+            assert np.array_equal(q[d], s.q[d])
+            np.testing.assert_allclose(u[d], s.u[d], rtol=0, atol=1e-12)
+        assert ctx.stats()["accepted"] == s.counters[1]
 
 
 @pytest.mark.parametrize("R", [0.0, 3.0])

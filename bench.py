@@ -125,6 +125,11 @@ def da_leg(args, pairs_plain, waves, schedule):
         for mode in ("plain", "da"):
             ctx.set_state_fixed(q0, u0, L0)
             ctx.da_enable(mode == "da")
+            if mode == "da":
+                # the delayed-acceptance kernel has its own register budget: tune W for it
+                # (every schedule runs that kernel while it is on), then restart from q0
+                da_w, _, _ = ctx.autotune(2)
+                ctx.set_state_fixed(q0, u0, L0)
             ctx.sweep(2)                                   # graph capture outside the timing
             dt, st, e, p2 = timed(ctx, args.da_sweeps)
             evaluated = st["attempted"] - st["rejected_left_cell"] - st["rejected_hard_core"]
@@ -133,6 +138,7 @@ def da_leg(args, pairs_plain, waves, schedule):
                  "acceptance": st["accepted"] / max(1, evaluated),
                  "ess_per_s": {"E/N": ee, "P2": ep},
                  "mean": {"E/N": float(np.mean(e)), "P2": float(np.mean(p2))}}
+            r["waves_per_cell"] = da_w if mode == "da" else waves
             if mode == "da":
                 s1, s2 = st["da_stage1_rejected"], st["da_stage2_rejected"]
                 r["stage1_rejection"] = s1 / max(1, evaluated)

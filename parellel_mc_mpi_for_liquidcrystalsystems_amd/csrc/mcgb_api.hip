@@ -288,11 +288,11 @@ template <typename Real, bool TR>
 void launch_pass_t(mcgb_ctx* c, const PassArgs& P) {
     if (!TR && c->da_on) {
         switch (c->waves) {
-        case 1: k_pass<Real, 1, false, false, true><<<c->nblk, 64, c->lds, c->st>>>(P); break;
-        case 2: k_pass<Real, 2, false, false, true><<<c->nblk, 128, c->lds, c->st>>>(P); break;
-        case 3: k_pass<Real, 3, false, false, true><<<c->nblk, 192, c->lds, c->st>>>(P); break;
-        case 8: k_pass<Real, 8, false, false, true><<<c->nblk, 512, c->lds, c->st>>>(P); break;
-        default: k_pass<Real, 4, false, false, true><<<c->nblk, 256, c->lds, c->st>>>(P); break;
+        case 1: k_pass_da<Real, 1><<<c->nblk, 64, c->lds, c->st>>>(P); break;
+        case 2: k_pass_da<Real, 2><<<c->nblk, 128, c->lds, c->st>>>(P); break;
+        case 3: k_pass_da<Real, 3><<<c->nblk, 192, c->lds, c->st>>>(P); break;
+        case 8: k_pass_da<Real, 8><<<c->nblk, 512, c->lds, c->st>>>(P); break;
+        default: k_pass_da<Real, 4><<<c->nblk, 256, c->lds, c->st>>>(P); break;
         }
     } else if (!TR && c->spec) {
         switch (c->waves) {
@@ -1506,8 +1506,8 @@ void pack_mlp(const float* P, MlpPacked* M) {
                 }
             for (int r = 0; r < 16; ++r) {
                 const int row = 32 * t + mfma_row(r, g);
-                M->b2[t][r][lane] = P[kOffB2 + row];
-                M->w3[t][r][lane] = P[kOffW3 + row];
+                M->b2[t][lane][r] = P[kOffB2 + row];
+                M->w3[t][lane][r] = P[kOffW3 + row];
             }
         }
     }

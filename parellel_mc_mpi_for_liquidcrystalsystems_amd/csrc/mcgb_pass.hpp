@@ -977,6 +977,15 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(1, 3))) 
     }
 }
 
+// The delayed-acceptance pass compiled for at least two wavefronts per SIMD (at most 256
+// registers, VGPRs and matrix-core accumulators together): with W = 1 every active cell of
+// the 256k grid is then resident at once.
+template <typename Real, int W>
+__global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(2))) void k_pass_da(PassArgs P) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    pass_cell<Real, W, false, false, true>(P, blockIdx.x, smem);
+}
+
 // The FP32 sequential pass compiled for four wavefronts per SIMD (at most 128 VGPRs): more
 // workgroups resident per CU; which one is faster depends on the number of active cells per
 // pass (pass schedule 2, chosen by the autotuner).

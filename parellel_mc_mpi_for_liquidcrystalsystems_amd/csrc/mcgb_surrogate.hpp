@@ -51,8 +51,8 @@ __host__ __device__ inline int mfma_row(int r, int g) { return 8 * (r >> 2) + 4 
 struct MlpPacked {
     uint16_t w1[2][8][64];       // [hidden tile][elem][lane]   (BF16 bits)
     uint16_t w2[2][4][8][64];    // [out tile][K step][elem][lane]
-    float b2[2][16][64];         // [out tile][C row slot][lane]
-    float w3[2][16][64];
+    float b2[2][64][16];         // [out tile][lane][C row slot]: a lane's 16 values contiguous
+    float w3[2][64][16];
     float b3;
     float pad[15];
 };
@@ -67,12 +67,15 @@ __device__ inline float silu_grad(float z) {
     return s * (1.0f + z * (1.0f - s));
 }
 
-// Per-lane MFMA operands of the network, held in registers by the kernels that use them.
+// Per-lane MFMA operands of the network: the matrix operands held in registers by the kernels
+// that use them; the output layer's bias and weights (64 more VGPRs) are read per tile from
+// the packed record (L1/L2-resident), which keeps the delayed-acceptance sweep under 256
+// registers.
 struct MlpRegs {
     bf16x8 w1[2];
     bf16x8 w2[2][4];
-    floatx16 b2[2];
-    floatx16 w3[2];
+    const floatx16* b2;          // this lane's [2] output-tile biases
+    const floatx16* w3;          // this lane's [2] output-layer weights
     float b3;
 };
 
@@ -89,12 +92,9 @@ __device__ inline void mlp_load(const MlpPacked* __restrict__ M, int lane, MlpRe
         R.w1[t] = load_bf16x8(&M->w1[t][0][0], lane);
 #pragma unroll
         for (int m = 0; m < 4; ++m) R.w2[t][m] = load_bf16x8(&M->w2[t][m][0][0], lane);
-#pragma unroll
-        for (int r = 0; r < 16; ++r) {
-            R.b2[t][r] = M->b2[t][r][lane];
-            R.w3[t][r] = M->w3[t][r][lane];
-        }
     }
+    R.b2 = (const floatx16*)&M->b2[0][lane][0];        // tile t at [t * 64]
+    R.w3 = (const floatx16*)&M->w3[0][lane][0];
     R.b3 = M->b3;
 }
 
@@ -122,6 +122,8 @@ __device__ inline bf16x8 mlp_input(float r, float a, float b, float c, bool on, 
 // Outputs of the 32 pairs whose inputs lanes 0..31 supplied: lanes j and j + 32 both return
 // pair j's value.  Must be called by the whole wavefront (wavefront-uniform control flow).
 __device__ inline float mlp_tile32(const MlpRegs& R, bf16x8 X) {
+    // the output layer's operands, loaded first so that the loads overlap layer 1
+    const floatx16 b2a = R.b2[0], b2b = R.b2[64], w3a = R.w3[0], w3b = R.w3[64];
     floatx16 z1a = {}, z1b = {};
     z1a = __builtin_amdgcn_mfma_f32_32x32x16_bf16(R.w1[0], X, z1a, 0, 0, 0);
     z1b = __builtin_amdgcn_mfma_f32_32x32x16_bf16(R.w1[1], X, z1b, 0, 0, 0);
@@ -133,7 +135,7 @@ __device__ inline float mlp_tile32(const MlpRegs& R, bf16x8 X) {
         h[2][e] = (__bf16)silu_f(z1b[e]);
         h[3][e] = (__bf16)silu_f(z1b[8 + e]);
     }
-    floatx16 z2a = R.b2[0], z2b = R.b2[1];
+    floatx16 z2a = b2a, z2b = b2b;
 #pragma unroll
     for (int m = 0; m < 4; ++m) {
         z2a = __builtin_amdgcn_mfma_f32_32x32x16_bf16(R.w2[0][m], h[m], z2a, 0, 0, 0);
@@ -141,7 +143,7 @@ __device__ inline float mlp_tile32(const MlpRegs& R, bf16x8 X) {
     }
     float y = 0.0f;
 #pragma unroll
-    for (int r = 0; r < 16; ++r) y += R.w3[0][r] * silu_f(z2a[r]) + R.w3[1][r] * silu_f(z2b[r]);
+    for (int r = 0; r < 16; ++r) y += w3a[r] * silu_f(z2a[r]) + w3b[r] * silu_f(z2b[r]);
     y += __shfl_xor(y, 32);
     return y + R.b3;
 }

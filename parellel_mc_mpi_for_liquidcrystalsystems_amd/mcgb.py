@@ -422,7 +422,9 @@ class Context:
         check(lib().mcgb_series_clear(self.h))
 
     def autotune(self, sweeps=4):
-        """Returns (waves_per_cell, schedule, {"seq/W", "spec/W" or "seq4/W": ms per pass})."""
+        """Returns (waves_per_cell, schedule, {"seq/W", "spec/W" or "seq4/W": ms per pass}).
+        With delayed acceptance on, every schedule runs the delayed-acceptance kernel, so it
+        tunes W for that kernel."""
         best, sched = C.c_int(), C.c_int()
         ms = np.empty(15)
         check(lib().mcgb_autotune(self.h, sweeps, C.byref(best), C.byref(sched), _d(ms)))

@@ -172,7 +172,7 @@ def npt_leg(args, waves, schedule):
     N, rho0, T, P = 64000, 0.28, 1.0, 2.0
     seed = 20261204
     p = m.gb_params(N, rho0, T, precision=32, seed=seed, npt=True, pressure=P,
-                    npt_max_dlnv=0.01, lattice_jitter=0.05, sublattice_colors=args.colors)
+                    npt_max_dlnv=0.01, lattice_jitter=0.05, sublattice_colors=args.npt_colors)
     p.waves_per_cell = waves
     p.pass_schedule = schedule
     pos, u, box = m.init_lattice(N, rho0, jitter=0.05, seed=seed)
@@ -189,7 +189,7 @@ def npt_leg(args, waves, schedule):
         e, p2, rho = ctx.series(max_n=max(1 << 16, args.npt_sweeps + 16))
     h = len(rho) // 2
     return {"workload": "NPT N=64000 P*=2 T*=1 from rho*=0.28, one ln V move per sweep",
-            "sublattices": args.colors ** 3,
+            "sublattices": args.npt_colors ** 3,
             "sweeps": args.npt_sweeps, "moves_per_s": st["attempted"] / dt,
             "ms_per_sweep": 1e3 * dt / args.npt_sweeps,
             "volume_acceptance": st["volume_accepted"] / max(1, st["volume_attempted"]),
@@ -259,8 +259,10 @@ def main():
     ap.add_argument("--npt-sweeps", type=int, default=5000)
     ap.add_argument("--no-replicas", action="store_true", help="skip the 32-replica leg")
     ap.add_argument("--colors", type=int, default=2, choices=(2, 3),
-                    help="sublattice colours per axis of the plain and NPT legs "
+                    help="sublattice colours per axis of the plain leg "
                          "(2: 8 sublattices of cells >= r_c; 3: 27 of cells >= r_c/2)")
+    ap.add_argument("--npt-colors", type=int, default=2, choices=(2, 3),
+                    help="sublattice colours per axis of the NPT leg")
     ap.add_argument("--replica-colors", type=int, default=3, choices=(2, 3),
                     help="sublattice colours per axis of the replica leg (small boxes: 27 "
                          "sublattices keep the per-cell move chains short)")

@@ -3,12 +3,14 @@
 // attempt_monte_carlo_move, :2445-2604).
 //
 // Per activation: the stencil ((2R+1)^3 cell segments of the device cell list, R = 1 for 8
-// sublattices, 2 for 27) is staged in LDS; the draws of the cell's moves (written by k_draws) are read in chunks of 64.  Per move: every
-// thread forms the trial state; each wavefront tests its share of the stencil against the old
-// and trial positions (FP32 pre-filter), compacts the hits into a wavefront-private LDS list,
-// evaluates both GB energies of every listed pair (cutoff decided on the FP64 r^2), and reduces
-// (DPP) a compensated energy difference; the wavefront partials meet in LDS, every thread takes
-// the same Metropolis decision, thread 0 commits an accepted move to LDS and HBM.
+// sublattices, 2 for 27) is staged in LDS.  The moves run in chunks of 8: the chunk's draws
+// (written by k_draws) arrive from HBM one chunk ahead, and 8 lanes prepare one record per
+// move (draws, the particle's state, trial orientation, trial colloid term).  Per move: each
+// wavefront tests its share of the stencil against the old and trial positions (FP32
+// midpoint pre-filter), compacts the hits into a wavefront-private LDS list, evaluates both GB
+// energies of every listed pair (cutoff decided on the FP64 r^2) two items per lane at a time,
+// and reduces (DPP) the energy difference; the wavefront partials meet in LDS, every thread
+// takes the same Metropolis decision, thread 0 commits an accepted move to LDS and HBM.
 // Two workgroup barriers per move.  Stencils beyond the staging capacity read HBM directly.
 #pragma once
 

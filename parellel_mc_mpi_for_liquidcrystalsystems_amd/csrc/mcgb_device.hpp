@@ -303,26 +303,31 @@ struct ColloidTerm {
 };
 __device__ __attribute__((noinline)) ColloidTerm colloid_term(const RepConst& C, double rx, double ry,
                                                               double rz, double ex, double ey, double ez) {
-    double d = sqrt(rx * rx + ry * ry + rz * rz);
-    double dij = d - C.R;
+    // the reference's expression with four divisions instead of nine: 1/d, 1/x and the two
+    // ratios; the rest as products (rounding within a few FP64 ulps of the literal form)
+    const double d = sqrt(rx * rx + ry * ry + rz * rz);
+    const double dij = d - C.R;
     if (d <= 1e-12) return {0.0, 1};
     if (dij > C.colloid_cut) return {0.0, 0};
-    double cth = (rx / d) * ex + (ry / d) * ey + (rz / d) * ez;
+    const double id = 1.0 / d;
+    double cth = (rx * ex + ry * ey + rz * ez) * id;
     cth = cth > 1.0 ? 1.0 : (cth < -1.0 ? -1.0 : cth);
-    double sin2 = 1.0 - cth * cth;
-    double s0 = C.sigma0;
+    const double sin2 = 1.0 - cth * cth;
+    const double s0 = C.sigma0;
     double den = C.one_minus_chi, num = 1.0 - C.chi * sin2;
     if (den <= 0.0) den = 1e-12;
     if (num <= 1e-12) num = 1e-12;
-    double scr = s0 * sqrt(num / den);
+    const double scr = s0 * sqrt(num / den);
     double den_e = 1.0 - C.chip * sin2;
     if (den_e <= 1e-12) den_e = 1e-12;
-    double eps = C.kappa_prime * pow_int_fast(C.one_minus_chip / den_e, C.mu);
-    double x = dij - scr + s0;
+    const double eps = C.kappa_prime * pow_int_fast(C.one_minus_chip / den_e, C.mu);
+    const double x = dij - scr + s0;
     if (x <= 1e-10) return {0.0, 1};
-    double s3 = s0 * s0 * s0, s9 = s3 * s3 * s3;
-    double x2 = x * x, x3 = x2 * x, x8 = x3 * x3 * x2, x9 = x8 * x;
-    return {eps * (s9 / (45.0 * x9) - s3 / (6.0 * x3) - s9 / (40.0 * d * x8) + s3 / (4.0 * d * x2)), 0};
+    const double s3 = s0 * s0 * s0, s9 = s3 * s3 * s3;
+    const double ix = 1.0 / x;
+    const double ix2 = ix * ix, ix3 = ix2 * ix, ix8 = ix3 * ix3 * ix2, ix9 = ix8 * ix;
+    return {eps * (s9 * ix9 * (1.0 / 45.0) - s3 * ix3 * (1.0 / 6.0) - s9 * ix8 * id * (1.0 / 40.0) +
+                   s3 * ix2 * id * 0.25), 0};
 }
 __device__ inline double colloid_f64(const RepConst& C, double rx, double ry, double rz, double ex,
                                      double ey, double ez, bool* cap) {

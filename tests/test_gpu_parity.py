@@ -394,6 +394,22 @@ def test_autotune_keeps_results_and_picks_a_supported_width():
             assert np.array_equal(q[d], s.q[d])
 
 
+def test_autotune_fp32_times_every_schedule_and_keeps_the_energy():
+    # FP32: the four-wavefront build (schedule 2) is timed too; whichever schedule and width
+    # win, the running energy after the tuning sweeps and four more equals the FP64 recompute
+    ctx, s, c = make_pair(N=9000, R=5.0, precision=32, seed=19)
+    with ctx:
+        best, sched, ms = ctx.autotune(2)
+        assert best in (1, 2, 3, 4, 8) and sched in (0, 1, 2)
+        assert all(v > 0 for v in ms.values())
+        ctx.sweep(4)
+        st = ctx.stats()
+        tot = ctx.total_energy()
+        assert st["cap_pairs"] == tot[1] and st["cap_colloid"] == tot[2]
+        assert abs(st["energy_finite"] - tot[3]) <= 1e-6 * (abs(tot[3]) + 9000)
+        assert st["attempted"] > 34 * 9000            # 30 tuning sweeps and 4 more
+
+
 def test_graph_replay_matches_single_sweeps():
     # mcgb_sweep(6) replays a captured two-sweep HIP graph; six mcgb_sweep(1) calls launch
     # kernels one by one: identical FP64 trajectories

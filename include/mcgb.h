@@ -359,6 +359,33 @@ int mcgb_write_fort1(const char *path, int n, const double *x, const double *y, 
 int mcgb_read_fort1(const char *path, int n, double *x, double *y, double *z,
                     double *ux, double *uy, double *uz, double box[3], double *colloid_radius);
 
+/* Bit-reproducible resume (no counterpart in the reference, whose rand() stream cannot be
+ * resumed; fortSphere.1 above stays the reference's format).  The record holds what a run
+ * needs to continue bit for bit besides its state: the Philox key (seed), the sweep index (the
+ * Philox counter of the next sweep's draws), the step sizes, the step-size adaptation counters
+ * and the running energy (integer counts of capped pairs and colloid overlaps and the finite
+ * part, which the NPT volume test compares its trial sum against).  mcgb_resume_state uploads a state verbatim (fixed-point positions; orientations as
+ * stored, not renormalised) together with the record, whose seed must be the replica's.
+ * mcgb_write_resume / mcgb_read_resume keep both in a text file (positions as integers,
+ * floating-point values as exact hexadecimal literals). */
+typedef struct {
+    uint64_t seed;
+    uint64_t sweep_index;
+    float max_displacement, max_rotation;
+    uint64_t adapt_attempted, adapt_accepted, adaptations;
+    int64_t capped_pairs, capped_colloid;
+    double energy_finite;
+} mcgb_resume;
+int mcgb_get_resume(mcgb_ctx *ctx, int replica, mcgb_resume *out);
+int mcgb_resume_state(mcgb_ctx *ctx, int replica, int n, const uint32_t *qx, const uint32_t *qy,
+                      const uint32_t *qz, const double *ux, const double *uy, const double *uz,
+                      const double box[3], const mcgb_resume *record);
+int mcgb_write_resume(const char *path, int n, const uint32_t *qx, const uint32_t *qy,
+                      const uint32_t *qz, const double *ux, const double *uy, const double *uz,
+                      const double box[3], const mcgb_resume *record);
+int mcgb_read_resume(const char *path, int n, uint32_t *qx, uint32_t *qy, uint32_t *qz,
+                     double *ux, double *uy, double *uz, double box[3], mcgb_resume *record);
+
 /* ---------------------------------------------------------------- learned first stage
  *
  * No counterpart in the reference (its MC has no surrogate).  A per-pair MLP 4 -> 64 -> 64 -> 1

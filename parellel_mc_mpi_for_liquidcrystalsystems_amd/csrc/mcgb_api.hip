@@ -745,9 +745,12 @@ int mcgb_create(const mcgb_params* p, int n_replicas, mcgb_ctx** out) {
     return MCGB_OK;
 }
 
+// Uploads replica r's state.  rec = null: orientations are normalised, the sweep index, step
+// sizes and adaptation count carry over and the other counters restart.  rec given (resume):
+// orientations go up as they are and the record's counters are restored.
 static int set_state_impl(mcgb_ctx* c, int r, int n, const uint32_t* qx, const uint32_t* qy,
                           const uint32_t* qz, const double* ux, const double* uy, const double* uz,
-                          const double box[3]) {
+                          const double box[3], const mcgb_resume* rec = nullptr) {
     const RepConst& C = c->hC[r];
     if (n != C.N) {
         mcgb_set_error("replica %d holds %d particles, %d given", r, C.N, n);
@@ -779,6 +782,7 @@ static int set_state_impl(mcgb_ctx* c, int r, int n, const uint32_t* qx, const u
             mcgb_set_error("replica %d: orientation %d has zero or non-finite norm", r, i);
             return MCGB_ERR_STATE;
         }
+        if (rec) nn = 1.0;                      // a resumed state goes up as it was stored
         un[0][i] = ux[i] / nn; un[1][i] = uy[i] / nn; un[2][i] = uz[i] / nn;
     }
     std::vector<float> uf(c->rsize == 4 ? n : 0);
@@ -798,8 +802,8 @@ static int set_state_impl(mcgb_ctx* c, int r, int n, const uint32_t* qx, const u
     RepDyn& Dr = D[r];
     uint32_t t = Dr.t;
     int sc = Dr.series_count;
-    const float dmax = Dr.dmax, drot = Dr.drot;
-    const unsigned long long adapts = Dr.adapts;
+    const float dmax = rec ? rec->max_displacement : Dr.dmax, drot = rec ? rec->max_rotation : Dr.drot;
+    const unsigned long long adapts = rec ? rec->adaptations : Dr.adapts;
     for (int d = 0; d < 3; ++d)
         if ((double)dmax >= 0.5 * box[d]) {
             mcgb_set_error("replica %d: the current max_displacement %g must be below half the box",
@@ -807,16 +811,74 @@ static int set_state_impl(mcgb_ctx* c, int r, int n, const uint32_t* qx, const u
             return MCGB_ERR_GEOMETRY;
         }
     memset(&Dr, 0, sizeof(Dr));
-    Dr.t = c->state_set[r] ? t : 0u;
+    Dr.t = rec ? (uint32_t)rec->sweep_index : (c->state_set[r] ? t : 0u);
     Dr.series_count = sc;
     Dr.dmax = dmax;
     Dr.drot = drot;
     Dr.adapts = adapts;
+    if (rec) {
+        Dr.ad_att = rec->adapt_attempted;
+        Dr.ad_acc = rec->adapt_accepted;
+    }
     for (int d = 0; d < 3; ++d) Dr.L[d] = box[d];
     if ((rc = write_dyn(c, D))) return rc;
-    if ((rc = recompute(c, 0))) return rc;
+    // Full sum of every replica into e_res only; the running energy of replica r alone is set
+    // (from the sum, or from the record), the other replicas' chains are left as they were.
+    if ((rc = recompute(c, 3))) return rc;
+    if ((rc = read_dyn(c, D))) return rc;
+    RepDyn& Ds = D[r];
+    Ds.cap_pairs = rec ? rec->capped_pairs : (long long)Ds.e_res[0];
+    Ds.cap_coll = rec ? rec->capped_colloid : (long long)Ds.e_res[1];
+    Ds.e_fin = rec ? rec->energy_finite : Ds.e_res[2];
+    if ((rc = write_dyn(c, D))) return rc;
     c->state_set[r] = 1;
     return MCGB_OK;
+}
+
+int mcgb_get_resume(mcgb_ctx* c, int r, mcgb_resume* out) {
+    int rc = check_ctx(c, r);
+    if (rc) return rc;
+    if (!out) {
+        mcgb_set_error("mcgb_get_resume: null record");
+        return MCGB_ERR_ARG;
+    }
+    std::vector<RepDyn> D;
+    if ((rc = read_dyn(c, D))) return rc;
+    const RepDyn& d = D[r];
+    out->seed = c->P[r].seed;
+    out->sweep_index = d.t;
+    out->max_displacement = d.dmax;
+    out->max_rotation = d.drot;
+    out->adapt_attempted = d.ad_att;
+    out->adapt_accepted = d.ad_acc;
+    out->adaptations = d.adapts;
+    out->capped_pairs = d.cap_pairs;
+    out->capped_colloid = d.cap_coll;
+    out->energy_finite = d.e_fin;
+    return MCGB_OK;
+}
+
+int mcgb_resume_state(mcgb_ctx* c, int r, int n, const uint32_t* qx, const uint32_t* qy,
+                      const uint32_t* qz, const double* ux, const double* uy, const double* uz,
+                      const double box[3], const mcgb_resume* rec) {
+    int rc = check_ctx(c, r);
+    if (rc) return rc;
+    if (!qx || !qy || !qz || !ux || !uy || !uz || !box || !rec) {
+        mcgb_set_error("mcgb_resume_state: null array or record");
+        return MCGB_ERR_ARG;
+    }
+    if (rec->seed != c->P[r].seed) {
+        mcgb_set_error("mcgb_resume_state: the record's seed %llu is not replica %d's (%llu)",
+                       (unsigned long long)rec->seed, r, (unsigned long long)c->P[r].seed);
+        return MCGB_ERR_ARG;
+    }
+    if (rec->sweep_index > 0xffffffffull || !(rec->max_displacement > 0.0f) ||
+        !(rec->max_rotation >= 0.0f) || rec->capped_pairs < 0 || rec->capped_colloid < 0 ||
+        !std::isfinite(rec->energy_finite)) {
+        mcgb_set_error("mcgb_resume_state: sweep index, step sizes or energy out of range");
+        return MCGB_ERR_ARG;
+    }
+    return set_state_impl(c, r, n, qx, qy, qz, ux, uy, uz, box, rec);
 }
 
 int mcgb_set_state_fixed(mcgb_ctx* c, int r, int n, const uint32_t* qx, const uint32_t* qy,

@@ -494,6 +494,76 @@ static int close_checked(FILE *f, const char *path)
     return MCGB_OK;
 }
 
+/* ------------------------------------------------------------------ resume record */
+
+/* Layout (text): a header line "mcgb-resume 1 N", a line with the seed, sweep index, the
+ * three adaptation counters and the two capped-term counts, a line with d_max, d_rot, the
+ * finite running energy and the box as hexadecimal floating-point literals (%a, exact), then
+ * one line per particle: qx qy qz as unsigned integers and ux uy uz as hexadecimal literals. */
+int mcgb_write_resume(const char *path, int n, const uint32_t *qx, const uint32_t *qy,
+                      const uint32_t *qz, const double *ux, const double *uy, const double *uz,
+                      const double box[3], const mcgb_resume *rec)
+{
+    if (!path || n < 0 || !qx || !qy || !qz || !ux || !uy || !uz || !box || !rec) {
+        mcgb_set_error("mcgb_write_resume: null argument or negative count");
+        return MCGB_ERR_ARG;
+    }
+    FILE *f = open_for(path, 0);
+    if (!f) return MCGB_ERR_IO;
+    fprintf(f, "mcgb-resume 1 %d\n", n);
+    fprintf(f, "%llu %llu %llu %llu %llu %lld %lld\n", (unsigned long long)rec->seed,
+            (unsigned long long)rec->sweep_index, (unsigned long long)rec->adapt_attempted,
+            (unsigned long long)rec->adapt_accepted, (unsigned long long)rec->adaptations,
+            (long long)rec->capped_pairs, (long long)rec->capped_colloid);
+    fprintf(f, "%a %a %a %a %a %a\n", (double)rec->max_displacement, (double)rec->max_rotation,
+            rec->energy_finite, box[0], box[1], box[2]);
+    for (int i = 0; i < n; ++i)
+        fprintf(f, "%u %u %u %a %a %a\n", qx[i], qy[i], qz[i], ux[i], uy[i], uz[i]);
+    return close_checked(f, path);
+}
+
+int mcgb_read_resume(const char *path, int n, uint32_t *qx, uint32_t *qy, uint32_t *qz,
+                     double *ux, double *uy, double *uz, double box[3], mcgb_resume *rec)
+{
+    if (!path || n < 0 || !qx || !qy || !qz || !ux || !uy || !uz || !box || !rec) {
+        mcgb_set_error("mcgb_read_resume: null argument or negative count");
+        return MCGB_ERR_ARG;
+    }
+    FILE *f = fopen(path, "r");
+    if (!f) { mcgb_set_error("cannot open %s", path); return MCGB_ERR_IO; }
+    int version = 0, nf = -1, ok = 1;
+    unsigned long long v[5];
+    long long cp, cc;
+    double dm, dr, ef;
+    ok = fscanf(f, "mcgb-resume %d %d", &version, &nf) == 2 && version == 1;
+    if (ok && nf != n) {
+        fclose(f);
+        mcgb_set_error("%s holds %d particles, %d requested", path, nf, n);
+        return MCGB_ERR_IO;
+    }
+    ok = ok && fscanf(f, "%llu %llu %llu %llu %llu %lld %lld", &v[0], &v[1], &v[2], &v[3], &v[4],
+                      &cp, &cc) == 7;
+    ok = ok && fscanf(f, "%la %la %la %la %la %la", &dm, &dr, &ef, &box[0], &box[1], &box[2]) == 6;
+    for (int i = 0; ok && i < n; ++i) {
+        unsigned a, b, c;
+        ok = fscanf(f, "%u %u %u %la %la %la", &a, &b, &c, &ux[i], &uy[i], &uz[i]) == 6;
+        qx[i] = a; qy[i] = b; qz[i] = c;
+    }
+    fclose(f);
+    if (!ok) { mcgb_set_error("error reading %s", path); return MCGB_ERR_IO; }
+    rec->seed = v[0];
+    rec->sweep_index = v[1];
+    rec->adapt_attempted = v[2];
+    rec->adapt_accepted = v[3];
+    rec->adaptations = v[4];
+    rec->max_displacement = (float)dm;
+    rec->max_rotation = (float)dr;
+    rec->capped_pairs = cp;
+    rec->capped_colloid = cc;
+    rec->energy_finite = ef;
+    return MCGB_OK;
+}
+
 /* print_Trayectoria, MCnptColloid.c:3081-3107 (same layout, caller's path). */
 int mcgb_write_trajectory(const char *path, int append, int n, const double *x, const double *y,
                           const double *z, const double *ux, const double *uy, const double *uz,

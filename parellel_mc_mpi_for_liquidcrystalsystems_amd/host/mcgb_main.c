@@ -13,7 +13,14 @@
  * fortSphere.1.BAK (:877, :958-960), and qmga_mcnvtGB.cnf, fortSphere.1 and DiscosEsfera.xyz at
  * the end (:1048-1061).  Every error prints its message and exits with status 1, as the
  * reference does.
+ *
+ * Beside the final fortSphere.1 it writes fortSphere.1.mcgb, the exact state with the resume
+ * record (sweep index, step sizes, adaptation counters, running energy).  A later run with
+ * initial_configuration_type 0 and the same seed continues from it bit for bit when it matches
+ * fortSphere.1 (same box, orientations and positions to the 8 printed decimals); otherwise it
+ * starts from fortSphere.1 alone, as the reference does.
  */
+#include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <time.h>
@@ -46,6 +53,30 @@ static void save_point(mcgb_ctx *ctx, int N, double *x, double *y, double *z, do
         die("cannot write fortSphere.1.BAK");
 }
 
+/* 1 when the resume file's state is the configuration read from fortSphere.1 (printed with 8
+ * decimals: positions, orientations and box agree within 1e-6, positions up to a period). */
+static int resume_matches(int N, const uint32_t *qx, const uint32_t *qy, const uint32_t *qz,
+                          const double *rux, const double *ruy, const double *ruz,
+                          const double *rbox, const double *x, const double *y, const double *z,
+                          const double *ux, const double *uy, const double *uz,
+                          const double *box)
+{
+    const double tol = 1e-6;
+    for (int d = 0; d < 3; ++d)
+        if (fabs(rbox[d] - box[d]) > tol) return 0;
+    const uint32_t *q[3] = {qx, qy, qz};
+    const double *c[3] = {x, y, z};
+    for (int i = 0; i < N; ++i) {
+        if (fabs(rux[i] - ux[i]) > tol || fabs(ruy[i] - uy[i]) > tol || fabs(ruz[i] - uz[i]) > tol)
+            return 0;
+        for (int d = 0; d < 3; ++d) {
+            double e = fabs(ldexp((double)q[d][i], -32) * rbox[d] - c[d][i]);
+            if (fmin(e, fabs(e - rbox[d])) > tol) return 0;
+        }
+    }
+    return 1;
+}
+
 int main(int argc, char **argv)
 {
     const char *path = argc > 1 ? argv[1] : "parametersFile.txt";
@@ -55,7 +86,15 @@ int main(int argc, char **argv)
     double *x = malloc(sizeof(double) * N), *y = malloc(sizeof(double) * N),
            *z = malloc(sizeof(double) * N), *ux = malloc(sizeof(double) * N),
            *uy = malloc(sizeof(double) * N), *uz = malloc(sizeof(double) * N);
-    if (!x || !y || !z || !ux || !uy || !uz) die("out of memory");
+    uint32_t *qx = malloc(sizeof(uint32_t) * N), *qy = malloc(sizeof(uint32_t) * N),
+             *qz = malloc(sizeof(uint32_t) * N);
+    double *rux = malloc(sizeof(double) * N), *ruy = malloc(sizeof(double) * N),
+           *ruz = malloc(sizeof(double) * N);
+    if (!x || !y || !z || !ux || !uy || !uz || !qx || !qy || !qz || !rux || !ruy || !ruz)
+        die("out of memory");
+    double rbox[3];
+    mcgb_resume rec;
+    int resume = 0;
     double box[3], R = p.colloid_radius;
     const double Rc = p.colloid_wall_option == 1 ? p.colloid_radius : 0.0;
     int ok = MCGB_OK;
@@ -72,11 +111,32 @@ int main(int argc, char **argv)
                                     p.lattice_jitter, p.seed, x, y, z, ux, uy, uz, box); break;
     }
     if (ok != MCGB_OK) die("cannot build the start configuration");
-    if (mcgb_write_trajectory("Configuraciones.dat", 0, N, x, y, z, ux, uy, uz, box) != MCGB_OK)
+    if (p.initial_configuration_type == 0 &&
+        mcgb_read_resume("fortSphere.1.mcgb", N, qx, qy, qz, rux, ruy, ruz, rbox, &rec) == MCGB_OK) {
+        resume = rec.seed == p.seed &&
+                 resume_matches(N, qx, qy, qz, rux, ruy, ruz, rbox, x, y, z, ux, uy, uz, box);
+        if (!resume)
+            printf("fortSphere.1.mcgb belongs to another configuration or seed: "
+                   "starting from fortSphere.1\n");
+    }
+    /* the first frame is the start configuration; a resumed one is the exact state, known once
+     * it is on the GPU */
+    if (!resume &&
+        mcgb_write_trajectory("Configuraciones.dat", 0, N, x, y, z, ux, uy, uz, box) != MCGB_OK)
         die("cannot write Configuraciones.dat");
     mcgb_ctx *ctx;
     if (mcgb_create(&p, 1, &ctx) != MCGB_OK) die("cannot create the GPU context");
-    if (mcgb_set_state(ctx, 0, N, x, y, z, ux, uy, uz, box) != MCGB_OK) die("cannot upload the state");
+    if (resume) {
+        if (mcgb_resume_state(ctx, 0, N, qx, qy, qz, rux, ruy, ruz, rbox, &rec) != MCGB_OK ||
+            mcgb_get_state(ctx, 0, N, x, y, z, ux, uy, uz, box) != MCGB_OK)
+            die("cannot resume from fortSphere.1.mcgb");
+        printf("resuming from fortSphere.1.mcgb after %llu sweeps\n",
+               (unsigned long long)rec.sweep_index);
+        if (mcgb_write_trajectory("Configuraciones.dat", 0, N, x, y, z, ux, uy, uz, box) != MCGB_OK)
+            die("cannot write Configuraciones.dat");
+    } else if (mcgb_set_state(ctx, 0, N, x, y, z, ux, uy, uz, box) != MCGB_OK) {
+        die("cannot upload the state");
+    }
 
     printf("================================================\n");
     printf("N = %d  box = %.5f x %.5f x %.5f  T* = %.4f  %s  precision FP%d\n", N, box[0], box[1],
@@ -121,6 +181,10 @@ int main(int argc, char **argv)
         die("cannot write qmga_mcnvtGB.cnf");
     if (mcgb_write_fort1("fortSphere.1", N, x, y, z, ux, uy, uz, box, R) != MCGB_OK)
         die("cannot write fortSphere.1");
+    if (mcgb_get_state_fixed(ctx, 0, N, qx, qy, qz, rux, ruy, ruz, rbox) != MCGB_OK ||
+        mcgb_get_resume(ctx, 0, &rec) != MCGB_OK ||
+        mcgb_write_resume("fortSphere.1.mcgb", N, qx, qy, qz, rux, ruy, ruz, rbox, &rec) != MCGB_OK)
+        die("cannot write fortSphere.1.mcgb");
     if (mcgb_write_xyz("DiscosEsfera.xyz", 0, N, x, y, z, box) != MCGB_OK ||
         mcgb_write_xyz("DiscosEsfera.xyz", 1, N, x, y, z, box) != MCGB_OK)
         die("cannot write DiscosEsfera.xyz");
@@ -131,5 +195,6 @@ int main(int argc, char **argv)
            s.attempted ? (double)s.accepted / s.attempted : 0.0, s.max_drift);
     mcgb_destroy(ctx);
     free(x); free(y); free(z); free(ux); free(uy); free(uz);
+    free(qx); free(qy); free(qz); free(rux); free(ruy); free(ruz);
     return 0;
 }

@@ -74,6 +74,20 @@ class Stats(C.Structure):
         return d
 
 
+class Resume(C.Structure):
+    """Everything beyond the configuration that a replica's chain depends on: the Philox key
+    (seed), the Philox counter (sweeps done), the step sizes, the adaptation counters and the
+    running energy (capped-term counts and finite part)."""
+    _fields_ = [("seed", C.c_uint64), ("sweep_index", C.c_uint64),
+                ("max_displacement", C.c_float), ("max_rotation", C.c_float),
+                ("adapt_attempted", C.c_uint64), ("adapt_accepted", C.c_uint64),
+                ("adaptations", C.c_uint64), ("capped_pairs", C.c_int64),
+                ("capped_colloid", C.c_int64), ("energy_finite", C.c_double)]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
 _lib = None
 dp = C.POINTER(C.c_double)
 up = C.POINTER(C.c_uint32)
@@ -141,6 +155,13 @@ def lib():
         "mcgb_eigen_sym3": (C.c_int, [dp, dp, dp]),
         "mcgb_write_fort1": (C.c_int, [C.c_char_p, C.c_int, dp, dp, dp, dp, dp, dp, dp, C.c_double]),
         "mcgb_read_fort1": (C.c_int, [C.c_char_p, C.c_int, dp, dp, dp, dp, dp, dp, dp, dp]),
+        "mcgb_get_resume": (C.c_int, [ctx, C.c_int, C.POINTER(Resume)]),
+        "mcgb_resume_state": (C.c_int, [ctx, C.c_int, C.c_int, up, up, up, dp, dp, dp, dp,
+                                        C.POINTER(Resume)]),
+        "mcgb_write_resume": (C.c_int, [C.c_char_p, C.c_int, up, up, up, dp, dp, dp, dp,
+                                        C.POINTER(Resume)]),
+        "mcgb_read_resume": (C.c_int, [C.c_char_p, C.c_int, up, up, up, dp, dp, dp, dp,
+                                       C.POINTER(Resume)]),
         "mcgb_surrogate_set": (C.c_int, [ctx, fp]),
         "mcgb_surrogate_get": (C.c_int, [ctx, fp]),
         "mcgb_surrogate_init": (C.c_int, [ctx, C.c_uint32]),
@@ -271,6 +292,27 @@ def write_xyz(path, pos, box, append=False):
     b = np.ascontiguousarray(box, dtype=np.float64)
     check(lib().mcgb_write_xyz(os.fsencode(path), 1 if append else 0, a[0].shape[0],
                                *(_d(v) for v in a), _d(b)))
+
+
+def write_resume(path, q, u, box, record):
+    """Write a resume file: fixed-point positions, orientations and box as exact values, and
+    the record (a Resume)."""
+    q = [np.ascontiguousarray(a, dtype=np.uint32) for a in q]
+    u = [np.ascontiguousarray(a, dtype=np.float64) for a in u]
+    box = np.ascontiguousarray(box, dtype=np.float64)
+    check(lib().mcgb_write_resume(os.fsencode(path), q[0].shape[0], *(_u(a) for a in q),
+                                  *(_d(a) for a in u), _d(box), C.byref(record)))
+
+
+def read_resume(path, n):
+    """Read a resume file of n particles: (q, u, box, record)."""
+    q = [np.empty(n, dtype=np.uint32) for _ in range(3)]
+    u = [np.empty(n) for _ in range(3)]
+    box = np.empty(3)
+    rec = Resume()
+    check(lib().mcgb_read_resume(os.fsencode(path), n, *(_u(a) for a in q), *(_d(a) for a in u),
+                                 _d(box), C.byref(rec)))
+    return q, u, box, rec
 
 
 def tau_int(x, c=5.0):
@@ -546,6 +588,22 @@ class Context:
         d, r, n = C.c_float(), C.c_float(), C.c_uint64()
         check(lib().mcgb_get_step_sizes(self.h, replica, C.byref(d), C.byref(r), C.byref(n)))
         return np.float32(d.value), np.float32(r.value), n.value
+
+    def get_resume(self, replica=0):
+        """The replica's resume record (a Resume): seed, sweeps done, step sizes, adaptation
+        counters.  Together with get_state_fixed it fixes every later sweep."""
+        rec = Resume()
+        check(lib().mcgb_get_resume(self.h, replica, C.byref(rec)))
+        return rec
+
+    def resume_state(self, q, u, box, record, replica=0):
+        """Upload a state verbatim with its resume record; the chain then continues exactly as
+        the one the record was taken from.  The record's seed must equal the replica's."""
+        q = [np.ascontiguousarray(a, dtype=np.uint32) for a in q]
+        u = [np.ascontiguousarray(a, dtype=np.float64) for a in u]
+        box = np.ascontiguousarray(box, dtype=np.float64)
+        check(lib().mcgb_resume_state(self.h, replica, q[0].shape[0], *(_u(a) for a in q),
+                                      *(_d(a) for a in u), _d(box), C.byref(record)))
 
     def set_step_sizes(self, max_displacement, max_rotation, replica=0):
         check(lib().mcgb_set_step_sizes(self.h, replica, float(max_displacement),

@@ -17,10 +17,11 @@ GOLD = os.path.join(REPO, "tests", "golden")
 pytestmark = pytest.mark.gpu
 
 
-def _params(tmp_path, **over):
+def _params(tmp_path, extra="", **over):
     txt = open(os.path.join(GOLD, "params_cfg0.txt")).read()
     for k, v in over.items():
         txt = re.sub(r"(?m)^%s\s+\S+" % k, "%s %s" % (k, v), txt)
+    txt += extra
     path = tmp_path / "parametersFile.txt"
     path.write_text(txt)
     return path
@@ -77,3 +78,39 @@ def test_driver_runs_and_writes_reference_outputs(tmp_path, ctype):
     m.check(m.lib().mcgb_read_fort1(str(tmp_path / "fortSphere.1").encode(), n,
                                     *(m._d(a) for a in arrs), m._d(b2), m._d(R)))
     assert np.allclose(arrs[0], frames[2][1][:, 0], atol=2e-6)
+
+
+def test_driver_restart_from_fortsphere_continues_bit_for_bit(tmp_path):
+    """20 sweeps in one run end in the same fortSphere.1.mcgb (exact state and resume record)
+    as 8 sweeps, a restart from fortSphere.1 (initial_configuration_type 0) and 12 more; a
+    fortSphere.1 that no longer matches the resume file starts afresh from fortSphere.1."""
+    if not os.path.exists(DRIVER):
+        pytest.fail("driver not built")
+    n = 3000
+    extra = "seed 991\nadapt_moves 2000\nprecision 32\n"
+
+    def run(d, ctype, sweeps):
+        d.mkdir(exist_ok=True)
+        path = _params(d, extra=extra, num_particles=n, system_density=0.1,
+                       initial_configuration_type=ctype, total_mc_cycles=sweeps,
+                       configuration_save_interval=0, statistics_print_interval=4)
+        print(f"driver restart: {d.name} type {ctype} sweeps {sweeps}", flush=True)
+        p = subprocess.run([DRIVER, str(path)], capture_output=True, text=True, cwd=d, timeout=120)
+        assert p.returncode == 0, p.stderr
+        return p.stdout
+
+    straight, split = tmp_path / "straight", tmp_path / "split"
+    run(straight, 2, 20)
+    assert "resuming" not in run(split, 2, 8)
+    out = run(split, 0, 12)
+    assert "resuming from fortSphere.1.mcgb after 8 sweeps" in out
+    want = (straight / "fortSphere.1.mcgb").read_text()
+    assert (split / "fortSphere.1.mcgb").read_text() == want
+    assert want.splitlines()[1].split()[:2] == ["991", "20"]
+    assert (split / "fortSphere.1").read_text() == (straight / "fortSphere.1").read_text()
+    # a replaced fortSphere.1 (here: the first orientation flipped) is not resumed
+    lines = (split / "fortSphere.1").read_text().splitlines()
+    lines[3] = " ".join("%15.8f" % -float(v) for v in lines[3].split())
+    (split / "fortSphere.1").write_text("\n".join(lines) + "\n")
+    out = run(split, 0, 1)
+    assert "starting from fortSphere.1" in out and "resuming" not in out

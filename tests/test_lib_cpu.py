@@ -149,6 +149,37 @@ def test_fort1_round_trip(tmp_path):
     assert R[0] == 5.0 and np.allclose(b2, box, atol=1e-8)
 
 
+def test_resume_file_round_trip_is_exact(tmp_path):
+    rng = np.random.default_rng(3)
+    n = 257
+    q = [rng.integers(0, 2**32, n, dtype=np.uint64).astype(np.uint32) for _ in range(3)]
+    q[0][:2] = (0, 2**32 - 1)
+    u = [rng.standard_normal(n) for _ in range(3)]
+    u[0][:3] = (-0.0, 5e-324, np.nextafter(1.0, 2.0))
+    box = np.array([31.25, np.nextafter(40.0, 0.0), 1e-3 / 3])
+    rec = m.Resume(seed=2**64 - 1, sweep_index=2**32 - 1, max_displacement=np.float32(0.1),
+                   max_rotation=np.float32(1e-7), adapt_attempted=12345678901,
+                   adapt_accepted=7, adaptations=2**40, capped_pairs=3,
+                   capped_colloid=2**40 + 1, energy_finite=-np.nextafter(-1234.5, 0.0))
+    path = str(tmp_path / "fortSphere.1.mcgb")
+    m.write_resume(path, q, u, box, rec)
+    q2, u2, b2, r2 = m.read_resume(path, n)
+    for a, b in zip(q + u + [box], q2 + u2 + [b2]):
+        assert a.tobytes() == b.tobytes()
+    assert r2.as_dict() == rec.as_dict()
+    with pytest.raises(m.McgbError, match="holds 257 particles, 256 requested"):
+        m.read_resume(path, 256)
+    bad = str(tmp_path / "bad")
+    with open(bad, "w") as f:
+        f.write(open(path).read().replace("mcgb-resume 1", "mcgb-resume 2", 1))
+    with pytest.raises(m.McgbError, match="error reading"):
+        m.read_resume(bad, n)
+    with open(bad, "w") as f:
+        f.write("\n".join(open(path).read().splitlines()[:-1]))
+    with pytest.raises(m.McgbError, match="error reading"):
+        m.read_resume(bad, n)
+
+
 def test_create_without_gpu_fails_loudly():
     if m.device_count() > 0:
         pytest.skip("a GPU is visible")

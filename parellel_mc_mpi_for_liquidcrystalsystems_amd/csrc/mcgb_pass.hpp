@@ -164,6 +164,18 @@ struct PairGeom {
     double s0, s1, s2, rc2;
 };
 
+// The pair constants moved into vector registers (one copy per lane): as scalars, together
+// with the move's uniform values, they overflow the scalar register file, and the compiler
+// then parks scalars in vector-register lanes, one v_writelane/v_readlane per spill and use.
+template <typename T> __device__ inline void to_vgpr(T& x) { asm volatile("" : "+v"(x)); }
+__device__ inline void pin_vgpr(GBf& G, PairGeom& X) {
+    to_vgpr(G.sigma0); to_vgpr(G.four_eps0); to_vgpr(G.chi2); to_vgpr(G.chip2);
+    to_vgpr(G.mu); to_vgpr(G.half_nu); to_vgpr(G.g1c0); to_vgpr(G.g1c1); to_vgpr(G.g1d0);
+    to_vgpr(G.g2c0); to_vgpr(G.g2c1); to_vgpr(G.g2d0); to_vgpr(G.sigma0d);
+    to_vgpr(X.s0); to_vgpr(X.s1); to_vgpr(X.s2); to_vgpr(X.rc2);
+}
+__device__ inline void pin_vgpr(GBd&, PairGeom&) {}
+
 template <typename Real> struct GBConst;
 template <> struct GBConst<float> { using type = GBf; };
 template <> struct GBConst<double> { using type = GBd; };
@@ -380,7 +392,7 @@ __device__ inline int stencil_slot(const int* s_off, const int* s_start, int par
 // configuration for the whole move, so the composite kernel keeps detailed balance exactly.
 // pass_cell makes the moves of active-cell block b (every early return is uniform over the
 // workgroup); k_pass runs one block per workgroup, or (GLOBAL) walks the overflow list.
-template <typename Real, int W, bool GLOBAL, bool TR, bool DA>
+template <typename Real, int W, bool GLOBAL, bool TR, bool DA, bool O4 = false>
 __device__ __forceinline__ void pass_cell(const PassArgs& P, int b, unsigned char* smem) {
     static_assert(!(DA && (GLOBAL || TR)), "delayed acceptance runs in the staged, untraced pass");
     const int r = P.blk_rep[b];
@@ -416,8 +428,9 @@ __device__ __forceinline__ void pass_cell(const PassArgs& P, int b, unsigned cha
     MlpRegs MR;
     if (DA) mlp_load(P.mlp, lane, MR);
 
-    const typename GBConst<Real>::type G = gb_const(C, (Real*)nullptr);
-    const PairGeom X = {D.s[0], D.s[1], D.s[2], C.rc2};
+    typename GBConst<Real>::type G = gb_const(C, (Real*)nullptr);
+    PairGeom X = {D.s[0], D.s[1], D.s[2], C.rc2};
+    if constexpr (!O4 && !DA) pin_vgpr(G, X);   // not in the 128- and 256-register builds
     const int part_off = C.part_off;
     const int colloid_on = C.colloid_on;
     const double R2 = C.R2;
@@ -994,7 +1007,7 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(2))) voi
 template <int W>
 __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4))) void k_pass_o4(PassArgs P) {
     extern __shared__ __align__(16) unsigned char smem[];
-    pass_cell<float, W, false, false, false>(P, blockIdx.x, smem);
+    pass_cell<float, W, false, false, false, true>(P, blockIdx.x, smem);
 }
 
 }  // namespace mcgb

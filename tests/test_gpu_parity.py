@@ -320,6 +320,34 @@ def test_stencil_beyond_lds_capacity_uses_global_path(precision):
             assert st["checked"] > 500
 
 
+@pytest.mark.parametrize("R", [0.0, 3.0])
+def test_fp32_overflow_cells_untraced_match_traced(R):
+    """Cells beyond the LDS capacity in untraced FP32 sequential sweeps (graph replay): three
+    sweeps end bit for bit where three traced sweeps end (one wavefront per cell), and with two
+    wavefronts per cell the running energy equals the FP64 recompute."""
+    print(f"overflow untraced R={R}", flush=True)
+    kw = dict(N=1300, precision=32, seed=12, R=R, stage_capacity=64, pass_schedule=0)
+    a, _, _ = make_pair(waves_per_cell=1, **kw)
+    b, _, _ = make_pair(waves_per_cell=1, **kw)
+    with a, b:
+        a.sweep(3)
+        for _ in range(3):
+            b.sweep_traced()
+        qa, ua, _ = a.get_state_fixed()
+        qb, ub, _ = b.get_state_fixed()
+        for d in range(3):
+            assert np.array_equal(qa[d], qb[d]) and np.array_equal(ua[d], ub[d])
+        assert a.stats()["accepted"] == b.stats()["accepted"] > 0
+    c, _, _ = make_pair(waves_per_cell=2, **kw)
+    with c:
+        c.sweep(3)
+        st = c.stats()
+        assert st["attempted"] > 3000 and 0 < st["accepted"] < st["attempted"]
+        tot = c.total_energy()
+        assert st["cap_pairs"] == tot[1] and st["cap_colloid"] == tot[2]
+        assert abs(st["energy_finite"] - tot[3]) <= 1e-6 * (abs(tot[3]) + 1300)
+
+
 def test_waves_per_cell_do_not_change_fp64_results():
     # the wavefront split only reorders FP64 sums: identical decisions and states
     states = []

@@ -431,6 +431,12 @@ __device__ __forceinline__ void pass_cell(const PassArgs& P, int b, unsigned cha
     typename GBConst<Real>::type G = gb_const(C, (Real*)nullptr);
     PairGeom X = {D.s[0], D.s[1], D.s[2], C.rc2};
     if constexpr (!O4 && !DA) pin_vgpr(G, X);   // not in the 128- and 256-register builds
+    // the state arrays as the accepted moves write them (addresses in vector registers in the
+    // sequential build, for the same reason)
+    uint32_t* wq[3] = {P.qx, P.qy, P.qz};
+    Real* wu[3] = {Ux, Uy, Uz};
+    if constexpr (!O4 && !DA)
+        for (int d = 0; d < 3; ++d) { to_vgpr(wq[d]); to_vgpr(wu[d]); }
     const int part_off = C.part_off;
     const int colloid_on = C.colloid_on;
     const double R2 = C.R2;
@@ -937,8 +943,8 @@ __device__ __forceinline__ void pass_cell(const PassArgs& P, int b, unsigned cha
                                 scache[jsel] = cen;
                                 scc[jsel] = ccn ? 1 : 0;
                             }
-                            P.qx[gsel] = qnx; P.qy[gsel] = qny; P.qz[gsel] = qnz;
-                            Ux[gsel] = unx; Uy[gsel] = uny; Uz[gsel] = unz;
+                            wq[0][gsel] = qnx; wq[1][gsel] = qny; wq[2][gsel] = qnz;
+                            wu[0][gsel] = unx; wu[1][gsel] = uny; wu[2][gsel] = unz;
                         }
                     } else {
                         outcome = MCGB_MOVE_REJECT;

@@ -430,12 +430,14 @@ __device__ __forceinline__ void pass_cell(const PassArgs& P, int b, unsigned cha
 
     typename GBConst<Real>::type G = gb_const(C, (Real*)nullptr);
     PairGeom X = {D.s[0], D.s[1], D.s[2], C.rc2};
-    if constexpr (!O4 && !DA) pin_vgpr(G, X);   // not in the 128- and 256-register builds
+    // not in the 128- and 256-register builds; with eight wavefronts per cell the registers
+    // cost the second resident workgroup per CU (measured)
+    if constexpr (!O4 && !DA && W <= 4) pin_vgpr(G, X);
     // the state arrays as the accepted moves write them (addresses in vector registers in the
     // sequential builds, for the same reason; they fit the four-wavefront budget too)
     uint32_t* wq[3] = {P.qx, P.qy, P.qz};
     Real* wu[3] = {Ux, Uy, Uz};
-    if constexpr (!DA && std::is_same<Real, float>::value)
+    if constexpr (!DA && W <= 4 && std::is_same<Real, float>::value)
         for (int d = 0; d < 3; ++d) { to_vgpr(wq[d]); to_vgpr(wu[d]); }
     const int part_off = C.part_off;
     const int colloid_on = C.colloid_on;

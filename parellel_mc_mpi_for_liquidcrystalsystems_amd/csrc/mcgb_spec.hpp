@@ -178,8 +178,11 @@ __global__ __launch_bounds__(64 * W) void k_pass_spec(PassArgs P) {
     unsigned short* smv = (unsigned short*)(sres + KD);        // KD moved cell members
     int* sred = (int*)(smv + KD + 8);                          // W pair counts
 
-    const typename GBConst<Real>::type G = gb_const(C, (Real*)nullptr);
-    const PairGeom X = {D.s[0], D.s[1], D.s[2], C.rc2};
+    typename GBConst<Real>::type G = gb_const(C, (Real*)nullptr);
+    PairGeom X = {D.s[0], D.s[1], D.s[2], C.rc2};
+    // FP32: the pair constants in vector registers (as in the sequential pass); with more
+    // than two wavefronts per cell the extra registers cost a workgroup per CU (measured)
+    if constexpr (W <= 2) pin_vgpr(G, X);
     const int part_off = C.part_off;
     const int colloid_on = C.colloid_on;
     const double R2 = C.R2;

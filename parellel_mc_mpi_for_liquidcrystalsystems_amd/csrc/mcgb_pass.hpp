@@ -433,6 +433,10 @@ __device__ __forceinline__ void pass_cell(const PassArgs& P, int b, unsigned cha
     // not in the 128- and 256-register builds; with eight wavefronts per cell the registers
     // cost the second resident workgroup per CU (measured)
     if constexpr (!O4 && !DA && W <= 4) pin_vgpr(G, X);
+    // the delayed-acceptance build has room for the box scales only (250 of 256 registers)
+    if constexpr (DA && std::is_same<Real, float>::value) {
+        to_vgpr(X.s0); to_vgpr(X.s1); to_vgpr(X.s2); to_vgpr(X.rc2);
+    }
     // the state arrays as the accepted moves write them (addresses in vector registers in the
     // sequential builds, for the same reason; they fit the four-wavefront budget too)
     uint32_t* wq[3] = {P.qx, P.qy, P.qz};

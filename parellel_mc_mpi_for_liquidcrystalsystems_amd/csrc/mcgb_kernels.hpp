@@ -155,6 +155,29 @@ __global__ void k_grid_g0(const RepConst* __restrict__ RC, RepDyn* __restrict__ 
 // Cell of every particle (get_owner_block_index/get_owner_cell_index, MCnptColloid.c:4371-4399,
 // as one exact integer multiply-shift per axis) and cell occupancy counts.
 // trial != 0: the NPT trial grid (D.nt, zero shift).
+// One atomic per distinct cell of a wavefront: lane counts of equal cells are summed first.
+// The particles arrive in the previous sweep's cell order, so a wavefront spans a few cells.
+// Returns the lane's slot: the counter's old value plus its rank among the lanes of its cell
+// (the slot order within a cell is arbitrary; k_sort_gather orders every cell by id).
+__device__ inline int wave_cell_add(int* __restrict__ ctr, int c, bool valid) {
+    const int lane = threadIdx.x & 63;
+    int slot = 0;
+    unsigned long long pending = __builtin_amdgcn_ballot_w64(valid);
+    while (pending) {                                  // uniform: a ballot
+        const int leader = __builtin_ctzll(pending);
+        const int lc = __builtin_amdgcn_readlane(c, leader);
+        const unsigned long long m = __builtin_amdgcn_ballot_w64(valid && c == lc);
+        int old = 0;
+        if (lane == leader) old = atomicAdd(&ctr[lc], __popcll(m));
+        old = __builtin_amdgcn_readlane(old, leader);
+        if ((m >> lane) & 1ull)
+            slot = old + (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                                         __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+        pending &= ~m;
+    }
+    return slot;
+}
+
 __global__ void k_count(const RepConst* __restrict__ RC, const RepDyn* __restrict__ RD,
                         const uint32_t* __restrict__ qx, const uint32_t* __restrict__ qy,
                         const uint32_t* __restrict__ qz, int* __restrict__ cell_id,
@@ -162,18 +185,21 @@ __global__ void k_count(const RepConst* __restrict__ RC, const RepDyn* __restric
     const int r = blockIdx.y;
     const RepConst& C = RC[r];
     const RepDyn& D = RD[r];
-    if (trial && !D.trial_ok) return;
+    if (trial && !D.trial_ok) return;                  // uniform: the whole grid row
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= C.N) return;
-    const int gi = C.part_off + i;
-    const uint32_t nx = trial ? D.nt[0] : D.n[0], ny = trial ? D.nt[1] : D.n[1],
-                   nz = trial ? D.nt[2] : D.n[2];
-    const uint32_t gx = trial ? 0u : D.g[0], gy = trial ? 0u : D.g[1], gz = trial ? 0u : D.g[2];
-    const uint32_t cx = cell_coord(qx[gi], gx, nx), cy = cell_coord(qy[gi], gy, ny),
-                   cz = cell_coord(qz[gi], gz, nz);
-    const int c = (int)(cx + nx * (cy + ny * cz));
-    cell_id[gi] = c;
-    atomicAdd(&count[C.cell_off + c], 1);
+    const bool valid = i < C.N;
+    int c = 0;
+    if (valid) {
+        const int gi = C.part_off + i;
+        const uint32_t nx = trial ? D.nt[0] : D.n[0], ny = trial ? D.nt[1] : D.n[1],
+                       nz = trial ? D.nt[2] : D.n[2];
+        const uint32_t gx = trial ? 0u : D.g[0], gy = trial ? 0u : D.g[1], gz = trial ? 0u : D.g[2];
+        const uint32_t cx = cell_coord(qx[gi], gx, nx), cy = cell_coord(qy[gi], gy, ny),
+                       cz = cell_coord(qz[gi], gz, nz);
+        c = (int)(cx + nx * (cy + ny * cz));
+        cell_id[gi] = c;
+    }
+    wave_cell_add(count + C.cell_off, c, valid);
 }
 
 // Exclusive scan of the counts into cell_start in three steps over tiles of kScanTile cells:
@@ -280,12 +306,12 @@ __global__ void k_scatter(const RepConst* __restrict__ RC, const RepDyn* __restr
                           const int* __restrict__ cell_start, int* __restrict__ tmp, int trial) {
     const int r = blockIdx.y;
     const RepConst& C = RC[r];
-    if (trial && !RD[r].trial_ok) return;
+    if (trial && !RD[r].trial_ok) return;              // uniform: the whole grid row
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= C.N) return;
-    const int c = cell_id[C.part_off + i];
-    const int slot = atomicAdd(&fill[C.cell_off + c], 1);
-    tmp[C.part_off + cell_start[C.cs_off + c] + slot] = i;
+    const bool valid = i < C.N;
+    const int c = valid ? cell_id[C.part_off + i] : 0;
+    const int slot = wave_cell_add(fill + C.cell_off, c, valid);
+    if (valid) tmp[C.part_off + cell_start[C.cs_off + c] + slot] = i;
 }
 
 // Orders every cell segment by particle id (a deterministic cell list: dist_batch_to_cells,

@@ -271,6 +271,17 @@ int mcgb_colloid_energy_batch(const mcgb_params *p, int n, const double *rel, co
 /* Philox4x32-10 on the GPU: ctr n*4, key n*2, out n*4 (known-answer tests). */
 int mcgb_philox_batch(int n, const uint32_t *ctr, const uint32_t *key, uint32_t *out);
 
+/* Parity hook for the move proposal: copies the draws the device made for the last sweep of
+ * replica r, in record order pass*(nact*k) + active_cell*k + move: j = raw particle selector
+ * (the move picks cell member (j * n_c) >> 32), dq = fixed-point step (n*3; the reference's
+ * d_max (U - 1/2) per axis, MCnptColloid.c:2502-2509), v = rotation vector on the unit sphere
+ * (n*3; replaces the Marsaglia table of pto_ransphr/Marsaglia4MC, :3036-3078, read at
+ * :2526-2550), thr = Metropolis threshold -log(U) T (:2566-2574).  Any array may be NULL.
+ * info = {sweep index of the draws, k, active cells per pass, passes}.  Returns the number of
+ * records (MCGB_ERR_CAPACITY when above capacity). */
+int mcgb_sweep_draws(mcgb_ctx *ctx, int replica, int capacity, uint32_t *j, int32_t *dq,
+                     double *v, double *thr, uint32_t info[4]);
+
 /* ---------------------------------------------------------------- host utilities */
 
 /* Seeded lattice start: n_s^3 sites, n_s the smallest side that leaves >= N sites outside the

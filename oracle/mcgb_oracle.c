@@ -267,6 +267,72 @@ static inline int32_t step_q(uint32_t r, int64_t K)
     return (int32_t)(p >= 0 ? (p >> 33) : -((-p) >> 33));
 }
 
+/* Planted mistakes for the tests that show the statistical checks detect them (0: none). */
+static int g_mutation = 0;
+
+void orc_set_mutation(int m) { g_mutation = m; }
+
+/* The draws of move m of active cell `cell` in pass p of sweep t: particle selector, fixed-point
+ * step (MCnptColloid.c:2502-2509), rotation vector on the unit sphere (the distribution of
+ * pto_ransphr/Marsaglia4MC, :3036-3078, used at :2526-2550) and Metropolis threshold
+ * -log(U) T (:2566-2574). */
+typedef struct {
+    uint32_t j;
+    int32_t dq[3];
+    double v[3];
+    double thr;
+} orc_draw;
+
+static void move_draw(const uint32_t key[2], uint32_t t, int p, uint32_t cell, int m,
+                      const int64_t K[3], double temperature, orc_draw *d)
+{
+    uint32_t ca[4] = {(uint32_t)m, cell, t, ((uint32_t)p << 8)};
+    uint32_t cb[4] = {(uint32_t)m, cell, t, ((uint32_t)p << 8) | 1u};
+    uint32_t A[4], B[4];
+    orc_philox(ca, key, A);
+    orc_philox(cb, key, B);
+    d->j = A[0];
+    for (int k = 0; k < 3; ++k) {
+        d->dq[k] = step_q(A[1 + k], K[k]);
+        if (g_mutation == ORC_MUT_STEP_DRIFT) d->dq[k] += (int32_t)(K[k] / 8);
+    }
+    double zz = 1.0 - 2.0 * u01(B[0]);
+    if (g_mutation == ORC_MUT_Z_HALF) zz = 1.0 - u01(B[0]);
+    double phi = 2.0 * M_PI * u01(B[1]);
+    if (g_mutation == ORC_MUT_PHI_HALF) phi = M_PI * u01(B[1]);
+    double sxy = sqrt(fmax(0.0, 1.0 - zz * zz));
+    d->v[0] = sxy * cos(phi);
+    d->v[1] = sxy * sin(phi);
+    d->v[2] = zz;
+    d->thr = g_mutation == ORC_MUT_BETA ? -log(u01(B[2])) / temperature
+                                        : -log(u01(B[2])) * temperature;
+}
+
+static void step_amplitudes(double max_displacement, const double L[3], int64_t K[3])
+{
+    for (int d = 0; d < 3; ++d) K[d] = llround(max_displacement * TWO32 / L[d]);
+}
+
+void orc_move_draws(uint64_t seed, uint32_t replica, uint32_t t, int pass, uint32_t cell, int m0,
+                    int n, const double L[3], double max_displacement, double temperature,
+                    uint32_t *j, int32_t *dq, double *v, double *thr)
+{
+    uint32_t key[2];
+    int64_t K[3];
+    make_key(seed, replica, key);
+    step_amplitudes((double)(float)max_displacement, L, K);
+    for (int k = 0; k < n; ++k) {
+        orc_draw d;
+        move_draw(key, t, pass, cell, m0 + k, K, temperature, &d);
+        if (j) j[k] = d.j;
+        for (int a = 0; a < 3; ++a) {
+            if (dq) dq[3 * (size_t)k + a] = d.dq[a];
+            if (v) v[3 * (size_t)k + a] = d.v[a];
+        }
+        if (thr) thr[k] = d.thr;
+    }
+}
+
 typedef struct {
     const orc_consts *c;
     const uint32_t *qx, *qy, *qz;
@@ -316,7 +382,7 @@ static void move_delta(const energy_view *v, int i, const uint32_t qn[3], const 
                         *npair += 1;
                         if (cap) capn_n++; else unw = e;
                     }
-                    acc += unw - uo;
+                    acc += g_mutation == ORC_MUT_DROP_OLD ? unw : unw - uo;
                     sc += fabs(unw) + fabs(uo);
                 }
             }
@@ -472,7 +538,7 @@ int orc_sweep(const orc_consts *c, int N, uint32_t *qx, uint32_t *qy, uint32_t *
     for (int d = 0; d < 3; ++d) { v.s[d] = L[d] / TWO32; v.n[d] = n[d]; }
     v.cell_start = cs; v.sorted = so;
     int64_t K[3];
-    for (int d = 0; d < 3; ++d) K[d] = llround((double)c->max_displacement * TWO32 / L[d]);
+    step_amplitudes(c->max_displacement, L, K);
 
     /* per-active-cell accumulators: deterministic totals whatever the thread count */
     int64_t *acc_i = (int64_t *)calloc((size_t)nact * 9, sizeof(int64_t));
@@ -499,18 +565,14 @@ int orc_sweep(const orc_consts *c, int N, uint32_t *qx, uint32_t *qy, uint32_t *
                     if (trace) trace[(size_t)p * nact * k + (size_t)a * k + m] = rec;
                     continue;
                 }
-                uint32_t ca[4] = {(uint32_t)m, (uint32_t)cell, t, ((uint32_t)p << 8)};
-                uint32_t cb[4] = {(uint32_t)m, (uint32_t)cell, t, ((uint32_t)p << 8) | 1u};
-                uint32_t A[4], B[4];
-                orc_philox(ca, key, A);
-                orc_philox(cb, key, B);
-                int jsel = (int)(((uint64_t)A[0] * (uint64_t)ncnt) >> 32);
+                orc_draw dr;
+                move_draw(key, t, p, (uint32_t)cell, m, K, c->temperature, &dr);
+                int jsel = (int)(((uint64_t)dr.j * (uint64_t)ncnt) >> 32);
                 int i = so[c0 + jsel];
                 rec.pid = i;
                 ai[0]++;
-                uint32_t qn[3] = {qx[i] + (uint32_t)step_q(A[1], K[0]),
-                                  qy[i] + (uint32_t)step_q(A[2], K[1]),
-                                  qz[i] + (uint32_t)step_q(A[3], K[2])};
+                uint32_t qn[3] = {qx[i] + (uint32_t)dr.dq[0], qy[i] + (uint32_t)dr.dq[1],
+                                  qz[i] + (uint32_t)dr.dq[2]};
                 if (cell_coord(qn[0], g[0], (uint32_t)n[0]) != (uint32_t)cx ||
                     cell_coord(qn[1], g[1], (uint32_t)n[1]) != (uint32_t)cy ||
                     cell_coord(qn[2], g[2], (uint32_t)n[2]) != (uint32_t)cz) {
@@ -528,12 +590,9 @@ int orc_sweep(const orc_consts *c, int N, uint32_t *qx, uint32_t *qy, uint32_t *
                     if (trace) trace[(size_t)p * nact * k + (size_t)a * k + m] = rec;
                     continue;
                 }
-                double zz = 1.0 - 2.0 * u01(B[0]);
-                double phi = 2.0 * M_PI * u01(B[1]);
-                double sxy = sqrt(fmax(0.0, 1.0 - zz * zz));
-                double un[3] = {ux[i] + c->max_rotation * (sxy * cos(phi)),
-                                uy[i] + c->max_rotation * (sxy * sin(phi)),
-                                uz[i] + c->max_rotation * zz};
+                double un[3] = {ux[i] + c->max_rotation * dr.v[0],
+                                uy[i] + c->max_rotation * dr.v[1],
+                                uz[i] + c->max_rotation * dr.v[2]};
                 double n2 = dot3(un, un);
                 if (n2 <= 1e-12) { un[0] = ux[i]; un[1] = uy[i]; un[2] = uz[i]; }
                 else {
@@ -554,13 +613,14 @@ int orc_sweep(const orc_consts *c, int N, uint32_t *qx, uint32_t *qy, uint32_t *
                     int co, cn;
                     double eo = orc_colloid(c, relo, uo, &co);
                     double en = orc_colloid(c, reln, un, &cn);
+                    if (g_mutation == ORC_MUT_NO_TRIAL_COLLOID) { en = 0.0; cn = 0; }
                     dccap = cn - co;
                     dfin += (cn ? 0.0 : en) - (co ? 0.0 : eo);
                     scale += fabs(cn ? 0.0 : en) + fabs(co ? 0.0 : eo);
                     ai[6] += 2;
                 }
                 double dE = (double)dcap * 1.0e9 + (double)dccap * 1.0e8 + dfin;
-                double thr = -log(u01(B[2])) * c->temperature;
+                double thr = dr.thr;
                 rec.dcap = dcap; rec.dccap = dccap; rec.dfin = dfin; rec.scale = scale;
                 if (dE <= thr) {
                     qx[i] = qn[0]; qy[i] = qn[1]; qz[i] = qn[2];
@@ -695,4 +755,65 @@ void orc_adapt_steps(float *max_displacement, float *max_rotation, int64_t accep
     }
     *max_displacement = d;
     *max_rotation = r;
+}
+
+/* ------------------------------------------------------------------ batches and chains */
+
+void orc_gb_pair_batch(const orc_consts *c, int n, const double *ei, const double *ej,
+                       const double *dr, double *u, int *capped)
+{
+    for (int k = 0; k < n; ++k) {
+        int cp;
+        u[k] = orc_gb_pair(c, ei + 3 * (size_t)k, ej + 3 * (size_t)k, dr + 3 * (size_t)k, &cp);
+        if (capped) capped[k] = cp;
+    }
+}
+
+void orc_colloid_batch(const orc_consts *c, int n, const double *rel, const double *e, double *u,
+                       int *capped)
+{
+    for (int k = 0; k < n; ++k) {
+        int cp;
+        u[k] = orc_colloid(c, rel + 3 * (size_t)k, e + 3 * (size_t)k, &cp);
+        if (capped) capped[k] = cp;
+    }
+}
+
+int orc_run_chains(const orc_consts *c, int N, int nchains, uint32_t *q, double *u, double *L,
+                   uint64_t seed, uint32_t t0, int nsweeps, int stride, uint32_t *out_q,
+                   double *out_u, double *out_e, int nthreads)
+{
+    if (N <= 0 || nchains <= 0 || nsweeps < 0 || stride <= 0) return -1;
+    const int nrec = nsweeps / stride;
+    const size_t n3 = 3 * (size_t)N;
+    int bad = 0;
+    (void)nthreads;
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads > 0 ? nthreads : 1) reduction(|:bad)
+    for (int ch = 0; ch < nchains; ++ch) {
+        uint32_t *qc = q + (size_t)ch * n3;
+        double *uc = u + (size_t)ch * n3;
+        double *Lc = L + 3 * (size_t)ch;
+        double E[4], energy[3];
+        int64_t cnt[9];
+        if (orc_total_energy(c, N, qc, qc + N, qc + 2 * N, uc, uc + N, uc + 2 * N, Lc, E, 1)) {
+            bad = 1;
+            continue;
+        }
+        energy[0] = E[0]; energy[1] = E[1]; energy[2] = E[2];
+        for (int s = 0; s < nsweeps; ++s) {
+            memset(cnt, 0, sizeof(cnt));
+            if (orc_sweep(c, N, qc, qc + N, qc + 2 * N, uc, uc + N, uc + 2 * N, Lc, seed,
+                          (uint32_t)ch, t0 + (uint32_t)s, 0, 0.0, energy, cnt, NULL, NULL, 0, 1)) {
+                bad = 1;
+                break;
+            }
+            if ((s + 1) % stride == 0) {
+                const size_t rix = (size_t)ch * nrec + (size_t)((s + 1) / stride - 1);
+                if (out_q) memcpy(out_q + rix * n3, qc, n3 * sizeof(uint32_t));
+                if (out_u) memcpy(out_u + rix * n3, uc, n3 * sizeof(double));
+                if (out_e) out_e[rix] = energy[0] * 1.0e9 + energy[1] * 1.0e8 + energy[2];
+            }
+        }
+    }
+    return bad ? -1 : 0;
 }

@@ -110,6 +110,45 @@ int    orc_total_energy_brute(const orc_consts *c, int N, const double *x, const
 void   orc_order_parameter(int N, const double *ux, const double *uy, const double *uz,
                            double *p2, double director[3]);
 
+/* Draws of moves m0 .. m0+n-1 of active cell `cell` in pass `pass` of sweep t, by the code the
+ * sweep uses: j = raw particle selector (member (j * n_c) >> 32 of the cell), dq = fixed-point
+ * step (n*3), v = rotation vector on the unit sphere (n*3), thr = -log(U) T.  Any output may be
+ * NULL. */
+void   orc_move_draws(uint64_t seed, uint32_t replica, uint32_t t, int pass, uint32_t cell, int m0,
+                      int n, const double L[3], double max_displacement, double temperature,
+                      uint32_t *j, int32_t *dq, double *v, double *thr);
+
+/* orc_gb_pair / orc_colloid over n rows (vectors n*3 interleaved; capped may be NULL). */
+void   orc_gb_pair_batch(const orc_consts *c, int n, const double *ei, const double *ej,
+                         const double *dr, double *u, int *capped);
+void   orc_colloid_batch(const orc_consts *c, int n, const double *rel, const double *e,
+                         double *u, int *capped);
+
+/* nchains independent NVT chains of N particles each (chain ch = replica ch of `seed`), run
+ * for nsweeps sweeps from sweep index t0, OpenMP over the chains.  q (nchains*3*N, axis-major
+ * per chain), u likewise, L (nchains*3) are updated in place.  After every `stride` sweeps the
+ * state and the energy value (caps*1e9 + colloid caps*1e8 + finite) are recorded:
+ * out_q/out_u [chain][record][3N], out_e [chain][record]; any may be NULL.  Returns 0 or -1. */
+int    orc_run_chains(const orc_consts *c, int N, int nchains, uint32_t *q, double *u, double *L,
+                      uint64_t seed, uint32_t t0, int nsweeps, int stride, uint32_t *out_q,
+                      double *out_u, double *out_e, int nthreads);
+
+/* Planted mistakes, used only by the tests that show the statistical checks of the sweep fail
+ * on them (global, 0 = none):
+ *   ORC_MUT_PHI_HALF         rotation azimuth drawn on [0, pi) instead of [0, 2 pi)
+ *   ORC_MUT_Z_HALF           rotation z drawn on (0, 1) instead of (-1, 1)
+ *   ORC_MUT_BETA             Metropolis threshold -log(U)/T instead of -log(U) T
+ *   ORC_MUT_NO_TRIAL_COLLOID trial state's colloid term dropped from dE
+ *   ORC_MUT_STEP_DRIFT       fixed-point step shifted by K/8 (an asymmetric proposal)
+ *   ORC_MUT_DROP_OLD         old state's GB pair terms dropped from dE */
+#define ORC_MUT_PHI_HALF          1
+#define ORC_MUT_Z_HALF            2
+#define ORC_MUT_BETA              3
+#define ORC_MUT_NO_TRIAL_COLLOID  4
+#define ORC_MUT_STEP_DRIFT        5
+#define ORC_MUT_DROP_OLD          6
+void   orc_set_mutation(int m);
+
 #ifdef __cplusplus
 }
 #endif

@@ -74,6 +74,20 @@ def lib():
         L.orc_total_energy_brute.argtypes = [C.POINTER(OrcConsts), C.c_int, dp, dp, dp, dp, dp,
                                              dp, dp, dp]
         L.orc_order_parameter.argtypes = [C.c_int, dp, dp, dp, dp, dp]
+        i32p = C.POINTER(C.c_int32)
+        L.orc_move_draws.argtypes = [C.c_uint64, C.c_uint32, C.c_uint32, C.c_int, C.c_uint32,
+                                     C.c_int, C.c_int, dp, C.c_double, C.c_double, up, i32p, dp,
+                                     dp]
+        L.orc_move_draws.restype = None
+        L.orc_gb_pair_batch.argtypes = [C.POINTER(OrcConsts), C.c_int, dp, dp, dp, dp, ip]
+        L.orc_gb_pair_batch.restype = None
+        L.orc_colloid_batch.argtypes = [C.POINTER(OrcConsts), C.c_int, dp, dp, dp, ip]
+        L.orc_colloid_batch.restype = None
+        L.orc_run_chains.argtypes = [C.POINTER(OrcConsts), C.c_int, C.c_int, up, dp, dp,
+                                     C.c_uint64, C.c_uint32, C.c_int, C.c_int, up, dp, dp, C.c_int]
+        L.orc_run_chains.restype = C.c_int
+        L.orc_set_mutation.argtypes = [C.c_int]
+        L.orc_set_mutation.restype = None
         _lib = L
     return _lib
 
@@ -129,6 +143,67 @@ def colloid(c, rel, e):
         u[k] = L.orc_colloid(C.byref(c), _d(rel[k]), _d(e[k]), C.byref(cp))
         cap[k] = cp.value
     return u, cap
+
+
+def gb_pair_batch(c, ei, ej, dr):
+    """orc_gb_pair over rows (one C call): returns (u, capped)."""
+    ei, ej, dr = (np.ascontiguousarray(np.atleast_2d(a), dtype=np.float64) for a in (ei, ej, dr))
+    n = dr.shape[0]
+    u = np.empty(n)
+    cap = np.empty(n, dtype=np.int32)
+    lib().orc_gb_pair_batch(C.byref(c), n, _d(ei), _d(ej), _d(dr), _d(u), _i(cap))
+    return u, cap
+
+
+def colloid_batch(c, rel, e):
+    """orc_colloid over rows (one C call): returns (u, capped)."""
+    rel, e = (np.ascontiguousarray(np.atleast_2d(a), dtype=np.float64) for a in (rel, e))
+    n = rel.shape[0]
+    u = np.empty(n)
+    cap = np.empty(n, dtype=np.int32)
+    lib().orc_colloid_batch(C.byref(c), n, _d(rel), _d(e), _d(u), _i(cap))
+    return u, cap
+
+
+def move_draws(seed, replica, t, pass_, cell, m0, n, L, max_displacement, temperature):
+    """Draws of moves m0..m0+n-1 of one active cell, by the sweep's own code:
+    (j (n,) uint32, dq (n,3) int32, v (n,3), thr (n,))."""
+    j = np.empty(n, dtype=np.uint32)
+    dq = np.empty((n, 3), dtype=np.int32)
+    v = np.empty((n, 3))
+    thr = np.empty(n)
+    Lv = np.ascontiguousarray(L, dtype=np.float64)
+    lib().orc_move_draws(seed, replica, t, pass_, cell, m0, n, _d(Lv), max_displacement,
+                         temperature, _u(j), dq.ctypes.data_as(C.POINTER(C.c_int32)), _d(v),
+                         _d(thr))
+    return j, dq, v, thr
+
+
+def run_chains(c, q, u, L, seed, nsweeps, stride, t0=0, nthreads=8):
+    """Independent chains of a small system (chain ch = replica ch): q (nch, 3, N) uint32,
+    u (nch, 3, N), L (nch, 3).  Returns recorded (q, u, E) every `stride` sweeps:
+    q (nch, nrec, 3, N), u (nch, nrec, 3, N), E (nch, nrec), and the final (q, u, L)."""
+    q = np.ascontiguousarray(q, dtype=np.uint32).copy()
+    u = np.ascontiguousarray(u, dtype=np.float64).copy()
+    L = np.ascontiguousarray(L, dtype=np.float64).copy()
+    nch, _, N = q.shape
+    nrec = nsweeps // stride
+    oq = np.empty((nch, nrec, 3, N), dtype=np.uint32)
+    ou = np.empty((nch, nrec, 3, N))
+    oe = np.empty((nch, nrec))
+    rc = lib().orc_run_chains(C.byref(c), N, nch, _u(q), _d(u), _d(L), seed, t0, nsweeps, stride,
+                              _u(oq), _d(ou), _d(oe), nthreads)
+    if rc != 0:
+        raise ValueError("orc_run_chains failed (box too small for the cell grid?)")
+    return oq, ou, oe, (q, u, L)
+
+
+def set_mutation(m):
+    """Plant one of the oracle's ORC_MUT_* mistakes (0 removes it).  Tests only."""
+    lib().orc_set_mutation(int(m))
+
+
+MUT_PHI_HALF, MUT_Z_HALF, MUT_BETA, MUT_NO_TRIAL_COLLOID, MUT_STEP_DRIFT, MUT_DROP_OLD = range(1, 7)
 
 
 def philox(ctr, key):

@@ -1536,6 +1536,46 @@ int mcgb_philox_batch(int n, const uint32_t* ctr, const uint32_t* key, uint32_t*
     return MCGB_OK;
 }
 
+// The draws k_draws wrote for the last sweep of replica r, in record order
+// pass*(nact*k) + active_cell*k + move.  Returns the number of records.
+int mcgb_sweep_draws(mcgb_ctx* c, int r, int capacity, uint32_t* j, int32_t* dq, double* v,
+                     double* thr, uint32_t info[4]) {
+    int rc = check_ctx(c, r);
+    if (rc) return rc;
+    if (!info || capacity < 0) {
+        mcgb_set_error("mcgb_sweep_draws: null info or negative capacity");
+        return MCGB_ERR_ARG;
+    }
+    std::vector<RepDyn> D;
+    if ((rc = read_dyn(c, D))) return rc;
+    const RepDyn& d = D[r];
+    if (!c->state_set[r] || d.t == 0) {
+        mcgb_set_error("mcgb_sweep_draws: replica %d has run no sweep", r);
+        return MCGB_ERR_STATE;
+    }
+    const long long n = (long long)c->npass * d.nact * d.k;
+    if (n > capacity) {
+        mcgb_set_error("mcgb_sweep_draws: %lld records, capacity %d", n, capacity);
+        return MCGB_ERR_CAPACITY;
+    }
+    std::vector<MoveDraw> h((size_t)n);
+    HIPCHK(hipMemcpyAsync(h.data(), c->draws + c->hC[r].rec_off, sizeof(MoveDraw) * (size_t)n,
+                          hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    for (long long i = 0; i < n; ++i) {
+        if (j) j[i] = h[i].j;
+        for (int a = 0; a < 3; ++a) {
+            if (dq) dq[3 * i + a] = h[i].dq[a];
+            if (v) v[3 * i + a] = h[i].v[a];
+        }
+        if (thr) thr[i] = h[i].thr;
+    }
+    info[0] = d.t - 1u;
+    info[1] = (uint32_t)d.k;
+    info[2] = (uint32_t)d.nact;
+    info[3] = (uint32_t)c->npass;
+    return (int)n;
+}
 
 // ---------------------------------------------------------------- learned first stage
 

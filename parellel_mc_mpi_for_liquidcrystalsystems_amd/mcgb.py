@@ -135,6 +135,8 @@ def lib():
         "mcgb_pair_energy_batch": (C.c_int, [C.POINTER(Params), C.c_int, dp, dp, dp, C.c_int, dp, ip]),
         "mcgb_colloid_energy_batch": (C.c_int, [C.POINTER(Params), C.c_int, dp, dp, C.c_int, dp, ip]),
         "mcgb_philox_batch": (C.c_int, [C.c_int, up, up, up]),
+        "mcgb_sweep_draws": (C.c_int, [ctx, C.c_int, C.c_int, up, C.POINTER(C.c_int32), dp, dp,
+                                       up]),
         "mcgb_init_lattice": (C.c_int, [C.c_int, C.c_double, C.c_double, C.c_double, C.c_double,
                                         C.c_uint64, dp, dp, dp, dp, dp, dp, dp]),
         "mcgb_tau_int": (C.c_int, [dp, C.c_int, C.c_double, dp, ip]),
@@ -430,6 +432,20 @@ class Context:
         nrec = int(info[3]) * int(info[4])
         arr = np.frombuffer(buf, dtype=TRACE_DTYPE, count=nrec).copy()
         return arr, info
+
+    def sweep_draws(self, replica=0):
+        """Draws of the last sweep: (j, dq (n,3) int32, v (n,3), thr, info {t, k, nact, P})."""
+        p = self.params[replica]
+        cap = 4 * p.num_particles + 200000
+        j = np.empty(cap, dtype=np.uint32)
+        dq = np.empty((cap, 3), dtype=np.int32)
+        v = np.empty((cap, 3))
+        thr = np.empty(cap)
+        info = np.zeros(4, dtype=np.uint32)
+        n = check(lib().mcgb_sweep_draws(self.h, replica, cap, _u(j),
+                                         dq.ctypes.data_as(C.POINTER(C.c_int32)), _d(v), _d(thr),
+                                         _u(info)))
+        return j[:n].copy(), dq[:n].copy(), v[:n].copy(), thr[:n].copy(), info
 
     def stats(self, replica=0):
         s = Stats()

@@ -1,0 +1,184 @@
+"""The GPU sweep samples the exact Boltzmann distribution.
+
+* k_draws: every draw of a sweep equals the oracle's draw for the same (sweep, pass, cell,
+  move) counter, and the device's rotation vectors pass the unit-sphere checks on their own;
+* one rod near the colloid and two Gay-Berne rods, run as many independent replicas in one
+  context: their (distance, angle) and pair statistics against the exact integrals of
+  tests/stationary.py, for the pass schedules (sequential: the benchmarked kernel; speculative),
+  both precisions, both sublattice decompositions, and delayed acceptance with a deliberately
+  bad surrogate (whose exact second stage must still give the exact distribution).
+"""
+import time
+
+import numpy as np
+import pytest
+
+from oracle import oracle_ctypes as oc
+from parellel_mc_mpi_for_liquidcrystalsystems_amd import mcgb as m
+from tests import stationary as st
+from tests.systems import layered_system
+from tests.test_oracle_stationary import sphere_checks, two_rod_consts, rod_consts
+
+pytestmark = pytest.mark.gpu
+
+SEED = 20261205
+
+
+@pytest.fixture(scope="module", autouse=True)
+def need_gpu():
+    if m.device_count() < 1:
+        pytest.fail("no GPU visible to HIP: the gpu tests need an MI355X")
+
+
+def cells_of_records(info, n, col, order):
+    """Cell index of every draw record (pass*(nact*k) + a*k + m) of a sweep on an n^3 grid."""
+    t, k, nact, P = (int(x) for x in info)
+    hx = n // col
+    idx = np.arange(P * nact * k)
+    p = idx // (nact * k)
+    a = (idx % (nact * k)) // k
+    sig = order[p].astype(np.int64)
+    cx = col * (a % hx) + sig % col
+    cy = col * ((a // hx) % hx) + (sig // col) % col
+    cz = col * (a // (hx * hx)) + sig // (col * col)
+    return p, cx + n * (cy + n * cz), idx % k
+
+
+@pytest.mark.parametrize("prec,col", [(32, 2), (64, 2), (32, 3)])
+def test_device_draws_equal_the_oracle_draws_and_are_uniform(prec, col):
+    N, rho, T = 20000, 0.3, 0.9
+    q, u, L = layered_system(N, rho, seed=3)
+    p = m.gb_params(N, rho, T, precision=prec, seed=SEED, sublattice_colors=col)
+    vs = []
+    with m.Context(p) as ctx:
+        ctx.set_state_fixed(q, u, L)
+        for _ in range(3):
+            ctx.sweep(1)
+            j, dq, v, thr, info = ctx.sweep_draws()
+            t = int(info[0])
+            n = oc.grid_dim(L[0], 4.0, col)
+            _, order, _ = oc.sweep_setup(SEED, 0, t, col)
+            pas, cell, mv = cells_of_records(info, n, col, order)
+            k = int(info[1])
+            for s in range(0, len(j), k):            # one active cell's k moves at a time
+                jo, dqo, vo, tho = oc.move_draws(SEED, 0, t, int(pas[s]), int(cell[s]), 0, k, L,
+                                                 0.1, T)
+                assert np.array_equal(j[s:s + k], jo)
+                assert np.array_equal(dq[s:s + k], dqo)
+                # near the poles s = sqrt(1 - z^2) cancels, and the device contracts 1 - z*z
+                # into one FMA: components agree to ~1e-14 absolute there, ~1e-16 elsewhere
+                np.testing.assert_allclose(v[s:s + k], vo, rtol=0, atol=1e-13)
+                np.testing.assert_allclose(thr[s:s + k], tho, rtol=1e-15, atol=0)
+            vs.append(v)
+    assert sphere_checks(np.concatenate(vs)) == []
+
+
+# ------------------------------------------------------------------ replicas of tiny systems
+
+def tiny_params(nrep, N, T, prec, schedule, colloid, colors=2, dmax=None):
+    v = st.ROD_L ** 3 - (4.0 / 3.0 * np.pi * st.ROD_R ** 3 if colloid else 0.0)
+    dm = dmax if dmax is not None else (3.0 if colors == 2 else 1.8)
+    return [m.gb_params(N, N / v, T, colloid_radius=st.ROD_R if colloid else 0.0,
+                        precision=prec, seed=SEED, max_displacement=dm, max_rotation=1.0,
+                        pass_schedule=schedule, sublattice_colors=colors)
+            for _ in range(nrep)]
+
+
+def sample_replicas(ctx, nrep, N, rounds, stride):
+    """Positions and orientations of every replica after every `stride` sweeps, read with the
+    asynchronous snapshots: x, u (nrep, rounds, 3, N)."""
+    X = np.empty((nrep, rounds, 3, N))
+    U = np.empty((nrep, rounds, 3, N))
+    for k in range(rounds):
+        ctx.sweep_async(stride)
+        for r in range(nrep):
+            ctx.snapshot_begin(r)
+        for r in range(nrep):
+            pos, uu, _ = ctx.snapshot_wait(r)
+            X[r, k] = pos
+            U[r, k] = uu
+    ctx.sync()
+    return X, U
+
+
+def start_states(ctx, nrep, N, rng):
+    L = np.full(3, st.ROD_L)
+    for r in range(nrep):
+        pos = np.zeros((3, N))
+        if N == 2:
+            pos[0, 1] = 0.5 * st.ROD_L
+        u = st.sphere_points(rng, N).T
+        ctx.set_state(list(pos), list(u), L, replica=r)
+
+
+def x_to_q(x):
+    return np.floor(np.asarray(x) / st.ROD_L * 2.0 ** 32 + 0.5).astype(np.uint64).astype(np.uint32)
+
+
+@pytest.mark.parametrize("prec,schedule", [(32, 0), (32, 1), (64, 0)])
+def test_one_rod_near_the_colloid_matches_quadrature_gpu(prec, schedule):
+    nrep, T = 64, 0.6
+    c = rod_consts(T)
+    rng = np.random.default_rng(SEED + schedule)
+    t0 = time.time()
+    with m.Context(tiny_params(nrep, 1, T, prec, schedule, True)) as ctx:
+        start_states(ctx, nrep, 1, rng)
+        ctx.sweep(200)
+        X, U = sample_replicas(ctx, nrep, 1, 1000, 40)
+    print(f"one rod, {nrep} replicas x 40000 sweeps: {time.time() - t0:.1f} s", flush=True)
+    d, cth, un = st.rod_observables(x_to_q(X[..., 0]), U[..., 0], st.ROD_L)
+    P = st.colloid_rod_reference(c, st.ROD_L, st.ROD_D_EDGES, st.ROD_C_EDGES).ravel()
+    z, mm, s = st.zscores(st.rod_indicators(d, cth), P)
+    ok1, msg1 = st.check(st.ROD_NAMES, z, mm, P, s, "(d, |cos theta|) bins")
+    zm, mmm, sm = st.zscores(st.orientation_moments(un), np.zeros(9))
+    ok2, msg2 = st.check(st.MOMENT_NAMES, zm, mmm, np.zeros(9), sm, "orientation moments")
+    assert ok1 and ok2, msg1 + "\n" + msg2
+
+
+def two_rod_gpu(prec, schedule, colors=2, da=False, nrep=64, rounds=1000, stride=40):
+    T = 0.6
+    c = two_rod_consts(T, colors)
+    rng = np.random.default_rng(SEED + 7 * schedule + colors)
+    t0 = time.time()
+    with m.Context(tiny_params(nrep, 2, T, prec, schedule, False, colors)) as ctx:
+        start_states(ctx, nrep, 2, rng)
+        if da:
+            ctx.surrogate_init(SEED)
+            w = ctx.surrogate_get()
+            # output layer x5: errors of several kT per pair.  Much worse networks stay exact
+            # but mix slowly (moves into and out of the cutoff are suppressed by both stages),
+            # so 40k sweeps would not reach equilibrium from rods placed apart.
+            w[-65:] *= 5.0
+            ctx.surrogate_set(w)
+            ctx.da_enable(True)
+        ctx.sweep(200)
+        for r in range(nrep):
+            ctx.reset_counters(r)
+        X, U = sample_replicas(ctx, nrep, 2, rounds, stride)
+        sts = [ctx.stats(r) for r in range(nrep)]
+    print(f"two rods, {nrep} replicas x {rounds * stride} sweeps: {time.time() - t0:.1f} s",
+          flush=True)
+    q = x_to_q(X)
+    from tests.test_oracle_stationary import two_rod_samples
+    Xi = two_rod_samples(c, q, U)
+    P, se, _ = st.two_rod_reference(c, st.ROD_L, st.TWO_ROD_OBS)
+    z, mm, s = st.zscores(Xi, P, se)
+    ok, msg = st.check(st.TWO_ROD_OBS.names, z, mm, P, s, "two rods")
+    return ok, msg, sts
+
+
+@pytest.mark.parametrize("prec,schedule,colors", [(32, 0, 2), (32, 1, 2), (64, 1, 2), (32, 0, 3)])
+def test_two_rods_match_importance_sampling_gpu(prec, schedule, colors):
+    ok, msg, _ = two_rod_gpu(prec, schedule, colors)
+    assert ok, msg
+
+
+def test_delayed_acceptance_with_a_bad_surrogate_samples_exactly():
+    ok, msg, sts = two_rod_gpu(32, 0, 2, da=True)
+    s1 = sum(s["da_stage1_rejected"] for s in sts)
+    s2 = sum(s["da_stage2_rejected"] for s in sts)
+    att = sum(s["attempted"] for s in sts)
+    assert ok, msg
+    # both stages act (the rods interact in ~6% of the moves): the bad network rejects moves
+    # the exact energy would take, and the exact stage overturns many of its acceptances
+    assert s1 > 10000 and s2 > 10000, (s1, s2, att)

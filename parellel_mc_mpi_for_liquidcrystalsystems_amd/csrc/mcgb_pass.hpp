@@ -250,21 +250,16 @@ __device__ inline void pair_energy_two(const GBd& G, const PairGeom& X, double u
                                 &u[q], &cap[q]);
 }
 
-// Per-lane accumulator of a finite energy difference: the FP32 pair terms are summed in FP64
-// (each lane adds only a few terms per move; FP64 adds and DPP moves are full rate), reduced
-// across lanes in a fixed DPP order and across wavefronts in a fixed order: deterministic.
+// Per-lane accumulator of a finite energy difference, reduced across lanes in a fixed DPP order
+// and across wavefronts in a fixed order: deterministic.
 template <typename Real> struct LaneAcc;
-// FP32 pair terms: each lane sums its few terms in FP32 and the wavefront reduces them in FP32
-// (error <= ~9 u sum|terms| for <= 256 terms, far inside the 1e-5 tolerance); when a lane holds
-// a near-cap energy (|sum| > 1e30, where 64 of them could overflow FP32) the reduction runs in
-// FP64 instead (uniform branch).
+// FP32 pair terms: each lane sums its few terms in FP64 and the wavefront reduces them in FP64,
+// so a move's dE carries no summation error beyond the pair terms' own FP32 rounding (and
+// near-cap terms up to ~1e36 cannot overflow).  FP64 adds run at the rate of unpacked FP32.
 template <> struct LaneAcc<float> {
-    float s = 0.0f;
-    __device__ void add(float v) { s += v; }
-    __device__ void to_partial(WavePartial& p) {
-        if (__ballot(!(fabsf(s) <= 1.0e30f)) != 0ull) p.d = wave_sum_d63((double)s);
-        else p.d = (double)wave_sum_f63(s);
-    }
+    double s = 0.0;
+    __device__ void add(float v) { s += (double)v; }
+    __device__ void to_partial(WavePartial& p) { p.d = wave_sum_d63(s); }
     __device__ static double total(const WavePartial* p, int w) {
         double s = 0.0;
         for (int k = 0; k < w; ++k) s += p[k].d;
@@ -386,8 +381,10 @@ __device__ inline int stencil_slot(const int* s_off, const int* s_start, int par
 // GLOBAL = true: the workgroups walk that overflow list and read the stencil from HBM.
 // TR = true: every trial move is recorded in P.trace (with the sum of |pair terms|).
 // DA = true: delayed acceptance (Christen & Fox 2005).  Stage 1 accepts with
-// min(1, exp(-beta dE~)), dE~ = the surrogate's GB pair terms plus the exact colloid term; only
-// then is the exact GB energy evaluated, and stage 2 accepts with min(1, exp(-beta (dE - dE~))).
+// min(1, exp(-beta dE~)), dE~ = the surrogate's GB pair terms; only then is the exact GB energy
+// evaluated, and stage 2 accepts with min(1, exp(-beta (dE - dE~))), dE the exact energy change
+// (GB pairs, caps and colloid term).  With a zero network stage 1 always passes and stage 2 is
+// the plain Metropolis test with the plain sweep's draw: the plain chain, colloid or not.
 // The surrogate orders each pair by stencil slot (lower slot first), a fixed function of the
 // configuration for the whole move, so the composite kernel keeps detailed balance exactly.
 // pass_cell makes the moves of active-cell block b (every early return is uniform over the
@@ -831,8 +828,7 @@ __device__ __forceinline__ void pass_cell(const PassArgs& P, int b, unsigned cha
                         if (lane == 63) ssur[w] = sacc;
                         __syncthreads();
                         for (int q = 0; q < W; ++q) sur += (double)ssur[q];
-                        const double dEs = (double)ccd * 1.0e8 + cacc + sur;
-                        stage1 = dEs <= pm.thr1;
+                        stage1 = sur <= pm.thr1;
                         exact = stage1 || P.da_judge;
                     }
                     if (staged && exact) {
@@ -915,8 +911,9 @@ __device__ __forceinline__ void pass_cell(const PassArgs& P, int b, unsigned cha
                     dfin = dpair + cacc;
                     bool accept;
                     if (DA) {
-                        // stage 2: the exact GB pair terms against the surrogate's
-                        accept = stage1 && (double)capd * 1.0e9 + (dpair - sur) <= pm.thr;
+                        // stage 2: the exact energy change against the surrogate's
+                        accept = stage1 &&
+                                 (double)capd * 1.0e9 + (double)ccd * 1.0e8 + (dfin - sur) <= pm.thr;
                         if (exact && capd == 0 && tid == 0) {
                             const float x = (float)((sur - dpair) / C.temperature) / kDaRange;
                             int bin = (int)floorf((x + 1.0f) * (0.5f * kDaBins));

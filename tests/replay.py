@@ -77,13 +77,15 @@ def u01(r):
 
 
 def replay_sweep(c, q, u, L, seed, replica, t, trace, info, check_tol=1e-9, fp32=False,
-                 max_scale=np.inf):
+                 max_scale=np.inf, collect=None):
     """Replays one sweep on copies of (q, u); checks every trace record; returns final state.
 
     fp32=True mirrors the FP32 kernel's storage: the rotation vector and the stored orientation
     are rounded to float32; dE is checked to check_tol relative to the sum of |pair terms| and
     an accept/reject within that tolerance of the threshold follows the record.  Moves whose
-    sum of |pair terms| exceeds max_scale (near-cap energies) are not checked for dE."""
+    sum of |pair terms| exceeds max_scale (near-cap energies) are not checked for dE.
+    collect (a list, optional) receives (record dfin, replayed FP64 dfin, record scale, dcap)
+    for every move that reached the energy evaluation."""
     q = [a.astype(np.uint32).copy() for a in q]
     u = [a.astype(np.float64).copy() for a in u]
     N = q[0].shape[0]
@@ -155,6 +157,8 @@ def replay_sweep(c, q, u, L, seed, replica, t, trace, info, check_tol=1e-9, fp32
                     dccap = ccn - cco
                 assert rec["dcap"] == cn_ - co
                 assert rec["dccap"] == dccap
+                if collect is not None:
+                    collect.append((float(rec["dfin"]), dfin, float(rec["scale"]), cn_ - co))
                 if rec["scale"] <= max_scale:
                     assert abs(rec["dfin"] - dfin) <= check_tol * max(rec["scale"], 1.0), \
                         (rec["dfin"], dfin, rec["scale"])

@@ -2,7 +2,8 @@
 state, both chains sample the same ensemble, so <E>/N, P2 and (NPT) the density agree within
 autocorrelation-corrected error bars (Sokal tau_int, c = 5).  The FP32 and FP64 chains take
 different decisions after a few moves (rounding), so this is the comparison that remains.
-The z-scores are printed; the test bound is 3 combined standard errors."""
+The z-scores are printed; the test bound is 2 combined standard errors (the 2 sigma of the
+statistical parity criterion).  The chains are seeded, so each comparison is deterministic."""
 import os
 
 import numpy as np
@@ -59,7 +60,7 @@ def test_nvt_energy_and_order_match_oracle():
     ze, mge, moe = _z(eg, eo)
     zp, mgp, mop = _z(pg, po)
     print(f"E/N gpu {mge:.5f} oracle {moe:.5f} z {ze:.2f}; P2 gpu {mgp:.4f} oracle {mop:.4f} z {zp:.2f}")
-    assert ze < 3 and zp < 3
+    assert ze < 2 and zp < 2
 
 
 def test_npt_density_matches_oracle():
@@ -85,4 +86,4 @@ def test_npt_density_matches_oracle():
     zr, mgr, mor = _z(rg, ro)
     zp, mgp, mop = _z(pg, po)
     print(f"rho gpu {mgr:.5f} oracle {mor:.5f} z {zr:.2f}; P2 gpu {mgp:.4f} oracle {mop:.4f} z {zp:.2f}")
-    assert zr < 3 and zp < 3
+    assert zr < 2 and zp < 2

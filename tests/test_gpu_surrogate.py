@@ -178,7 +178,7 @@ def _means(ctx, sweeps):
 
 def test_delayed_acceptance_with_a_poor_network_samples_the_same_distribution():
     # an untrained network changes only the efficiency: <E>/N and P2 agree with the plain
-    # sweep within 3 combined standard errors (autocorrelation-corrected)
+    # sweep within 2 combined standard errors (autocorrelation-corrected)
     a = _equilibrated()
     b = _equilibrated()
     with a, b:
@@ -186,8 +186,8 @@ def test_delayed_acceptance_with_a_poor_network_samples_the_same_distribution():
         b.da_enable(True)
         (ea, sea), (pa, spa) = _means(a, 3000)
         (eb, seb), (pb, spb) = _means(b, 3000)
-        assert abs(ea - eb) < 3 * np.hypot(sea, seb), (ea, eb, sea, seb)
-        assert abs(pa - pb) < 3 * np.hypot(spa, spb), (pa, pb, spa, spb)
+        assert abs(ea - eb) < 2 * np.hypot(sea, seb), (ea, eb, sea, seb)
+        assert abs(pa - pb) < 2 * np.hypot(spa, spb), (pa, pb, spa, spb)
         s = b.stats()
         assert s["da_stage1_rejected"] > 0 and s["da_stage2_rejected"] > 0
 

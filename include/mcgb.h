@@ -263,6 +263,17 @@ int mcgb_cell_list(mcgb_ctx *ctx, int replica, uint32_t t,
 int mcgb_pair_energy_batch(const mcgb_params *p, int n, const double *ei, const double *ej,
                            const double *dr, int precision, double *u, int *capped);
 
+/* The same as mcgb_pair_energy_batch without allocating: no counterpart in the reference; for
+ * callers that cannot allocate device memory (HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  The device copies of the inputs and outputs live in a library-owned
+ * static array of MCGB_NOALLOC_MAX_PAIRS pairs (84 bytes each, ~5.5 MB) on every GPU the library
+ * runs on, held until the process ends; larger n fails with MCGB_ERR_CAPACITY.  Every call on
+ * one GPU shares that GPU's array, so calls that overlap on one GPU (several host threads) race. */
+#define MCGB_NOALLOC_MAX_PAIRS 65536
+int mcgb_pair_energy_batch_noalloc(const mcgb_params *p, int n, const double *ei,
+                                   const double *ej, const double *dr, int precision, double *u,
+                                   int *capped);
+
 /* Colloid-rod energy of n rods on the GPU, as GBCollRod() (MCnptColloid.c:1758-1854);
  * rel = rod centre minus colloid centre (n*3), e = rod orientation (n*3). */
 int mcgb_colloid_energy_batch(const mcgb_params *p, int n, const double *rel, const double *e,

@@ -7,6 +7,7 @@
 #include <cstdio>
 #include <cstring>
 #include <cstdlib>
+#include <mutex>
 #include <vector>
 
 #include "mcgb_internal.h"
@@ -1489,6 +1490,75 @@ int mcgb_pair_energy_batch(const mcgb_params* p, int n, const double* ei, const 
     HIPCHK(hipMemcpy(capped, d_c, n * 4, hipMemcpyDeviceToHost));
     HIPCHK(hipFree(d_ei)); HIPCHK(hipFree(d_ej)); HIPCHK(hipFree(d_dr));
     HIPCHK(hipFree(d_u)); HIPCHK(hipFree(d_c));
+    return MCGB_OK;
+}
+
+}  // extern "C"
+
+// Staging of mcgb_pair_energy_batch_noalloc: ei, ej, dr (3 doubles each), u (double) and the
+// cap flag (int) of up to MCGB_NOALLOC_MAX_PAIRS pairs, in fixed regions.  One copy per GPU,
+// held until the process ends.  In host code the array's name is only a stand-in: its device
+// address (per device) comes from hipGetSymbolAddress.
+static constexpr size_t kNoAllocMax = MCGB_NOALLOC_MAX_PAIRS;
+static constexpr size_t kNoAllocBytes = kNoAllocMax * (3 * 24 + 8 + 4);
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+namespace {
+constexpr int kNoAllocDevices = 64;
+std::mutex g_scratch_mu;
+unsigned char* g_scratch[kNoAllocDevices] = {};
+
+// The device address of `scratch` on the current device (looked up once per device).
+int scratch_ptr(unsigned char** out) {
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kNoAllocDevices) {
+        mcgb_set_error("mcgb_pair_energy_batch_noalloc: device %d beyond the %d supported", dev,
+                       kNoAllocDevices);
+        return MCGB_ERR_ARG;
+    }
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch[dev]) {
+        void* p = nullptr;
+        HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        g_scratch[dev] = (unsigned char*)p;
+    }
+    *out = g_scratch[dev];
+    return MCGB_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int mcgb_pair_energy_batch_noalloc(const mcgb_params* p, int n, const double* ei,
+                                   const double* ej, const double* dr, int precision, double* u,
+                                   int* capped) {
+    if (n > MCGB_NOALLOC_MAX_PAIRS) {
+        mcgb_set_error("mcgb_pair_energy_batch_noalloc: %d pairs exceed the staging maximum of %d "
+                       "(use mcgb_pair_energy_batch, or batches of at most %d)", n,
+                       MCGB_NOALLOC_MAX_PAIRS, MCGB_NOALLOC_MAX_PAIRS);
+        return MCGB_ERR_CAPACITY;
+    }
+    RepConst C;
+    int rc = batch_consts(p, &C);
+    if (rc) return rc;
+    if (n <= 0 || !ei || !ej || !dr || !u || !capped || (precision != 32 && precision != 64)) {
+        mcgb_set_error("mcgb_pair_energy_batch_noalloc: bad arguments");
+        return MCGB_ERR_ARG;
+    }
+    unsigned char* base = nullptr;
+    if ((rc = scratch_ptr(&base))) return rc;
+    double* d_ei = (double*)base;
+    double* d_ej = (double*)(base + kNoAllocMax * 24);
+    double* d_dr = (double*)(base + 2 * kNoAllocMax * 24);
+    double* d_u = (double*)(base + 3 * kNoAllocMax * 24);
+    int* d_c = (int*)(base + 3 * kNoAllocMax * 24 + kNoAllocMax * 8);
+    HIPCHK(hipMemcpy(d_ei, ei, (size_t)n * 24, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(d_ej, ej, (size_t)n * 24, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(d_dr, dr, (size_t)n * 24, hipMemcpyHostToDevice));
+    k_pair_batch<<<(n + 255) / 256, 256>>>(C, n, precision, d_ei, d_ej, d_dr, d_u, d_c);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpy(u, d_u, (size_t)n * 8, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(capped, d_c, (size_t)n * 4, hipMemcpyDeviceToHost));
     return MCGB_OK;
 }
 

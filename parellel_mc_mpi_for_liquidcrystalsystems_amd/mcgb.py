@@ -134,6 +134,8 @@ def lib():
         "mcgb_cell_list": (C.c_int, [ctx, C.c_int, C.c_uint32, ip, C.c_int, ip, C.c_int, up, up, ip]),
         "mcgb_pair_energy_batch": (C.c_int, [C.POINTER(Params), C.c_int, dp, dp, dp, C.c_int, dp, ip]),
         "mcgb_colloid_energy_batch": (C.c_int, [C.POINTER(Params), C.c_int, dp, dp, C.c_int, dp, ip]),
+        "mcgb_pair_energy_batch_noalloc": (C.c_int, [C.POINTER(Params), C.c_int, dp, dp, dp, C.c_int,
+                                                     dp, ip]),
         "mcgb_philox_batch": (C.c_int, [C.c_int, up, up, up]),
         "mcgb_sweep_draws": (C.c_int, [ctx, C.c_int, C.c_int, up, C.POINTER(C.c_int32), dp, dp,
                                        up]),
@@ -339,6 +341,21 @@ def pair_energy_batch(p, ei, ej, dr, precision):
     u = np.empty(n)
     cap = np.empty(n, dtype=np.int32)
     check(lib().mcgb_pair_energy_batch(C.byref(p), n, _d(ei), _d(ej), _d(dr), precision, _d(u), _i(cap)))
+    return u, cap
+
+
+NOALLOC_MAX_PAIRS = 65536
+
+
+def pair_energy_batch_noalloc(p, ei, ej, dr, precision):
+    """mcgb_pair_energy_batch_noalloc: the pair batch with its device staging in the library's
+    static array (at most NOALLOC_MAX_PAIRS pairs per call)."""
+    ei, ej, dr = (np.ascontiguousarray(a, dtype=np.float64).reshape(-1, 3) for a in (ei, ej, dr))
+    n = ei.shape[0]
+    u = np.empty(n)
+    cap = np.empty(n, dtype=np.int32)
+    check(lib().mcgb_pair_energy_batch_noalloc(C.byref(p), n, _d(ei), _d(ej), _d(dr), precision,
+                                               _d(u), _i(cap)))
     return u, cap
 
 

@@ -452,6 +452,19 @@ int mcgb_emit_pairs(mcgb_ctx *ctx, int replica, int stride, int offset, long lon
  * stage for moves the surrogate rejected (decisions unchanged; fills the histogram). */
 int mcgb_da_enable(mcgb_ctx *ctx, int on, int judge);
 int mcgb_da_histogram(mcgb_ctx *ctx, int replica, uint64_t counts[MCGB_DA_BINS], int clear);
+/* The plain kernel as automatic fallback: with window_sweeps > 0, mcgb_sweep runs delayed
+ * acceptance in windows of that many sweeps and, after a window in which the exact stage
+ * rejected more than max_stage2_rejection of the moves the surrogate let through, or the
+ * surrogate let through less than min_stage1_acceptance of the moves (a broken or poisoned
+ * network: NaN weights reject everything), switches delayed acceptance off and runs the rest
+ * with the plain kernel (counted in *fallbacks; mcgb_da_enable turns it on again).  The check
+ * sums the counters of all replicas; it applies to mcgb_sweep, not to mcgb_sweep_async.
+ * window_sweeps = 0 (the default) turns the guard off.  mcgb_da_status: whether delayed
+ * acceptance is on, the fallbacks so far, and the last window's {stage-2 rejection, stage-1
+ * acceptance}. */
+int mcgb_da_guard(mcgb_ctx *ctx, int window_sweeps, double max_stage2_rejection,
+                  double min_stage1_acceptance);
+int mcgb_da_status(mcgb_ctx *ctx, int *on, uint64_t *fallbacks, double rates[2]);
 
 #ifdef __cplusplus
 }

@@ -101,6 +101,11 @@ struct mcgb_ctx {
     MlpPacked* sur_M = nullptr;        // packed BF16 matrix-core operands
     bool sur_set = false;
     int da_on = 0, da_judge = 0;
+    // automatic fallback from delayed acceptance to the plain kernel (mcgb_da_guard)
+    int da_guard_window = 0;                 // sweeps per check, 0 = off
+    double da_guard_s2_max = 0.5, da_guard_a1_min = 0.02;
+    unsigned long long da_fallbacks = 0;
+    double da_rates[2] = {0.0, 0.0};         // last window: stage-2 rejection, stage-1 acceptance
     unsigned long long* da_hist = nullptr;   // [nrep][kDaBins + 2]
     float* pairs = nullptr;            // emitted exact pair records (kPairRec floats each)
     long long pairs_cap = 0, pairs_n = 0;
@@ -767,14 +772,7 @@ static int set_state_impl(mcgb_ctx* c, int r, int n, const uint32_t* qx, const u
             return MCGB_ERR_GEOMETRY;
         }
     }
-    const int b = c->cur;
-    const size_t off = C.part_off;
-    std::vector<int> ids(n);
-    for (int i = 0; i < n; ++i) ids[i] = i;
-    HIPCHK(hipMemcpyAsync(c->q[b][0] + off, qx, n * 4, hipMemcpyHostToDevice, c->st));
-    HIPCHK(hipMemcpyAsync(c->q[b][1] + off, qy, n * 4, hipMemcpyHostToDevice, c->st));
-    HIPCHK(hipMemcpyAsync(c->q[b][2] + off, qz, n * 4, hipMemcpyHostToDevice, c->st));
-    HIPCHK(hipMemcpyAsync(c->pid[b] + off, ids.data(), n * 4, hipMemcpyHostToDevice, c->st));
+    // every check before the first upload: a rejected call leaves the replica as it was
     std::vector<double> un[3];
     for (int d = 0; d < 3; ++d) un[d].resize(n);
     for (int i = 0; i < n; ++i) {
@@ -785,17 +783,6 @@ static int set_state_impl(mcgb_ctx* c, int r, int n, const uint32_t* qx, const u
         }
         if (rec) nn = 1.0;                      // a resumed state goes up as it was stored
         un[0][i] = ux[i] / nn; un[1][i] = uy[i] / nn; un[2][i] = uz[i] / nn;
-    }
-    std::vector<float> uf(c->rsize == 4 ? n : 0);
-    for (int d = 0; d < 3; ++d) {
-        if (c->rsize == 8) {
-            HIPCHK(hipMemcpyAsync((double*)c->u[b][d] + off, un[d].data(), n * 8, hipMemcpyHostToDevice, c->st));
-            HIPCHK(hipStreamSynchronize(c->st));
-        } else {
-            for (int i = 0; i < n; ++i) uf[i] = (float)un[d][i];
-            HIPCHK(hipMemcpyAsync((float*)c->u[b][d] + off, uf.data(), n * 4, hipMemcpyHostToDevice, c->st));
-            HIPCHK(hipStreamSynchronize(c->st));
-        }
     }
     std::vector<RepDyn> D;
     int rc = read_dyn(c, D);
@@ -811,6 +798,25 @@ static int set_state_impl(mcgb_ctx* c, int r, int n, const uint32_t* qx, const u
                            r, (double)dmax);
             return MCGB_ERR_GEOMETRY;
         }
+    const int b = c->cur;
+    const size_t off = C.part_off;
+    std::vector<int> ids(n);
+    for (int i = 0; i < n; ++i) ids[i] = i;
+    HIPCHK(hipMemcpyAsync(c->q[b][0] + off, qx, n * 4, hipMemcpyHostToDevice, c->st));
+    HIPCHK(hipMemcpyAsync(c->q[b][1] + off, qy, n * 4, hipMemcpyHostToDevice, c->st));
+    HIPCHK(hipMemcpyAsync(c->q[b][2] + off, qz, n * 4, hipMemcpyHostToDevice, c->st));
+    HIPCHK(hipMemcpyAsync(c->pid[b] + off, ids.data(), n * 4, hipMemcpyHostToDevice, c->st));
+    std::vector<float> uf(c->rsize == 4 ? n : 0);
+    for (int d = 0; d < 3; ++d) {
+        if (c->rsize == 8) {
+            HIPCHK(hipMemcpyAsync((double*)c->u[b][d] + off, un[d].data(), n * 8, hipMemcpyHostToDevice, c->st));
+            HIPCHK(hipStreamSynchronize(c->st));
+        } else {
+            for (int i = 0; i < n; ++i) uf[i] = (float)un[d][i];
+            HIPCHK(hipMemcpyAsync((float*)c->u[b][d] + off, uf.data(), n * 4, hipMemcpyHostToDevice, c->st));
+            HIPCHK(hipStreamSynchronize(c->st));
+        }
+    }
     memset(&Dr, 0, sizeof(Dr));
     Dr.t = rec ? (uint32_t)rec->sweep_index : (c->state_set[r] ? t : 0u);
     Dr.series_count = sc;
@@ -996,15 +1002,60 @@ int mcgb_get_state(mcgb_ctx* c, int r, int n, double* x, double* y, double* z, d
     return MCGB_OK;
 }
 
+namespace {
+void drop_graphs(mcgb_ctx* c);
+}  // namespace
+
+// Delayed-acceptance counters summed over the replicas: stage-1 rejections, stage-2
+// rejections, moves that reached the energy test.
+static int da_totals(mcgb_ctx* c, unsigned long long out[3]) {
+    std::vector<RepDyn> D;
+    int rc = read_dyn(c, D);
+    if (rc) return rc;
+    out[0] = out[1] = out[2] = 0;
+    for (const RepDyn& d : D) {
+        out[0] += d.da_s1;
+        out[1] += d.da_s2;
+        out[2] += d.attempted - d.left - d.hard;
+    }
+    return MCGB_OK;
+}
+
 static int sweep_impl(mcgb_ctx* c, int n_sweeps, mcgb_move_trace* trace) {
     for (int r = 0; r < c->nrep; ++r)
         if (!c->state_set[r]) {
             mcgb_set_error("replica %d has no configuration (call mcgb_set_state first)", r);
             return MCGB_ERR_STATE;
         }
-    int rc = c->prec == 64 ? enqueue_sweeps<double>(c, n_sweeps, trace)
-                           : enqueue_sweeps<float>(c, n_sweeps, trace);
-    if (rc) return rc;
+    int rc = MCGB_OK;
+    for (int left = n_sweeps; left > 0;) {
+        // with the guard on, delayed acceptance runs in windows; after a window whose surrogate
+        // let through too many moves the exact stage rejects, or let almost none through, the
+        // context falls back to the plain kernel for the rest of the call and after it
+        const bool guard = c->da_on && c->da_guard_window > 0 && !trace;
+        const int chunk = guard ? std::min(left, c->da_guard_window) : left;
+        unsigned long long a[3] = {0, 0, 0}, b[3] = {0, 0, 0};
+        if (guard && (rc = da_totals(c, a))) return rc;
+        rc = c->prec == 64 ? enqueue_sweeps<double>(c, chunk, trace)
+                           : enqueue_sweeps<float>(c, chunk, trace);
+        if (rc) return rc;
+        if (guard) {
+            if ((rc = da_totals(c, b))) return rc;
+            const unsigned long long s1 = b[0] - a[0], s2 = b[1] - a[1], ev = b[2] - a[2];
+            if (ev > 0) {
+                const double a1 = (double)(ev - s1) / (double)ev;
+                const double r2 = ev > s1 ? (double)s2 / (double)(ev - s1) : 0.0;
+                c->da_rates[0] = r2;
+                c->da_rates[1] = a1;
+                if (r2 > c->da_guard_s2_max || a1 < c->da_guard_a1_min) {
+                    c->da_on = 0;
+                    drop_graphs(c);
+                    c->da_fallbacks += 1;
+                }
+            }
+        }
+        left -= chunk;
+    }
     std::vector<RepDyn> D;
     if ((rc = read_dyn(c, D))) return rc;
     for (int r = 0; r < c->nrep; ++r) {
@@ -2015,6 +2066,29 @@ int mcgb_da_enable(mcgb_ctx* c, int on, int judge) {
     drop_graphs(c);
     c->da_on = on ? 1 : 0;
     c->da_judge = judge ? 1 : 0;
+    return MCGB_OK;
+}
+
+int mcgb_da_guard(mcgb_ctx* c, int window_sweeps, double max_stage2_rejection,
+                  double min_stage1_acceptance) {
+    int rc = check_ctx(c, 0);
+    if (rc) return rc;
+    if (window_sweeps < 0 || !(max_stage2_rejection >= 0.0) || !(min_stage1_acceptance >= 0.0)) {
+        mcgb_set_error("mcgb_da_guard: window >= 0 and non-negative bounds required");
+        return MCGB_ERR_ARG;
+    }
+    c->da_guard_window = window_sweeps;
+    c->da_guard_s2_max = max_stage2_rejection;
+    c->da_guard_a1_min = min_stage1_acceptance;
+    return MCGB_OK;
+}
+
+int mcgb_da_status(mcgb_ctx* c, int* on, uint64_t* fallbacks, double rates[2]) {
+    int rc = check_ctx(c, 0);
+    if (rc) return rc;
+    if (on) *on = c->da_on;
+    if (fallbacks) *fallbacks = c->da_fallbacks;
+    if (rates) { rates[0] = c->da_rates[0]; rates[1] = c->da_rates[1]; }
     return MCGB_OK;
 }
 

@@ -181,6 +181,8 @@ def lib():
         "mcgb_emit_pairs": (C.c_int, [ctx, C.c_int, C.c_int, C.c_int, llp]),
         "mcgb_da_enable": (C.c_int, [ctx, C.c_int, C.c_int]),
         "mcgb_da_histogram": (C.c_int, [ctx, C.c_int, u64p, C.c_int]),
+        "mcgb_da_guard": (C.c_int, [ctx, C.c_int, C.c_double, C.c_double]),
+        "mcgb_da_status": (C.c_int, [ctx, C.POINTER(C.c_int), u64p, dp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -593,6 +595,19 @@ class Context:
 
     def da_enable(self, on=True, judge=False):
         check(lib().mcgb_da_enable(self.h, 1 if on else 0, 1 if judge else 0))
+
+    def da_guard(self, window_sweeps=20, max_stage2_rejection=0.5, min_stage1_acceptance=0.02):
+        check(lib().mcgb_da_guard(self.h, window_sweeps, max_stage2_rejection,
+                                  min_stage1_acceptance))
+
+    def da_status(self):
+        """{"on": bool, "fallbacks": int, "stage2_rejection": r, "stage1_acceptance": a} (the
+        last guarded window)."""
+        on, fb = C.c_int(), C.c_uint64()
+        rates = np.zeros(2)
+        check(lib().mcgb_da_status(self.h, C.byref(on), C.byref(fb), _d(rates)))
+        return {"on": bool(on.value), "fallbacks": fb.value, "stage2_rejection": rates[0],
+                "stage1_acceptance": rates[1]}
 
     def da_histogram(self, replica=0, clear=False):
         h = np.zeros(DA_BINS, dtype=np.uint64)

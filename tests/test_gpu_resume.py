@@ -117,3 +117,41 @@ def test_resume_rejects_a_foreign_record():
         with pytest.raises(m.McgbError, match="out of range"):
             c.resume_state(*st, rec)
         assert c.get_resume().sweep_index == 2              # nothing was uploaded
+
+
+@pytest.mark.parametrize("precision", [32, 64])
+def test_rejected_set_state_leaves_the_replica_unchanged(precision):
+    """A set_state that fails its checks (a zero orientation; a current step size of half the
+    new box) leaves state, energy, counters and the chain exactly as they were: the context goes
+    on as a twin that never saw the calls."""
+    N, rho = 1300, 0.25
+    q, u, L = layered_system(N, rho, seed=4)
+    p = m.gb_params(N, rho, 1.0, precision=precision, seed=77, energy_check_interval=0)
+    with m.Context(p) as a, m.Context(p) as b:
+        for ctx in (a, b):
+            ctx.set_state_fixed(q, u, L)
+            ctx.sweep(3)
+            ctx.set_step_sizes(0.46 * L[0], 0.1)
+        q0, u0, L0 = a.get_state_fixed()
+        st0 = a.stats()
+        bad = [x.copy() for x in u0]
+        for d in range(3):
+            bad[d][7] = 0.0
+        with pytest.raises(m.McgbError) as e:
+            a.set_state_fixed(q0, bad, L0)
+        assert e.value.code == m.ERR_STATE
+        with pytest.raises(m.McgbError) as e:       # 0.46 L >= half of a box of 0.9 L
+            a.set_state_fixed([x[::-1].copy() for x in q0], u0, 0.9 * L0)
+        assert e.value.code == m.ERR_GEOMETRY
+        q1, u1, L1 = a.get_state_fixed()
+        for d in range(3):
+            assert np.array_equal(q1[d], q0[d]) and np.array_equal(u1[d], u0[d])
+        assert np.array_equal(L1, L0) and a.stats() == st0
+        for ctx in (a, b):
+            ctx.set_step_sizes(0.1, 0.1)
+            ctx.sweep(2)
+        qa, ua, _ = a.get_state_fixed()
+        qb, ub, _ = b.get_state_fixed()
+        for d in range(3):
+            assert np.array_equal(qa[d], qb[d]) and np.array_equal(ua[d], ub[d])
+        assert a.stats() == b.stats()

@@ -220,3 +220,61 @@ def test_trained_network_screens_out_exact_evaluations():
     assert s2 / max(1, evaluated - s1) < 0.4         # stage 2 overturns a minority of stage 1
     centre = h[24:40].sum()                          # |dE~ - dE| < 1 kT for most moves
     assert centre > 0.55 * h.sum()
+
+
+@pytest.mark.parametrize("poison", ["overflow", "huge"])
+def test_poisoned_network_falls_back_to_the_plain_kernel(poison):
+    """With the guard on, a poisoned network is switched off after one window and the rest of
+    the sweeps run the plain kernel: the chain moves again, and its running energy equals the
+    FP64 recompute.  Poisons: weights x1e30, finite but overflowing in the hidden layers (what
+    a diverged training run leaves): non-finite outputs, stage 1 passes nothing; weights x1000:
+    the exact stage overturns nearly every stage-1 acceptance.  (mcgb_surrogate_set itself
+    refuses non-finite weights.)"""
+    with small_ctx(N=1500) as ctx:
+        ctx.surrogate_init(3)
+        w = ctx.surrogate_get()
+        w *= 1e30 if poison == "overflow" else 1000.0
+        ctx.surrogate_set(w)
+        ctx.da_enable(True)
+        ctx.da_guard(5, 0.5, 0.02)
+        ctx.sweep(5)                               # one window: delayed acceptance
+        s5 = ctx.stats()
+        st = ctx.da_status()
+        print(f"poison {poison}: {st}", flush=True)
+        assert not st["on"] and st["fallbacks"] == 1, st
+        if poison == "overflow":
+            assert st["stage1_acceptance"] < 0.02
+        else:
+            assert st["stage2_rejection"] > 0.9
+        ctx.sweep(10)                              # the plain kernel
+        s = ctx.stats()
+        assert s["da_stage1_rejected"] == s5["da_stage1_rejected"]
+        assert s["da_stage2_rejected"] == s5["da_stage2_rejected"]
+        assert s["accepted"] - s5["accepted"] > 0.2 * (s["attempted"] - s5["attempted"])
+        tot = ctx.total_energy()
+        assert s["cap_pairs"] == tot[1]
+        assert s["energy_finite"] == pytest.approx(tot[3], rel=1e-6, abs=1e-3)
+
+
+def test_guard_keeps_a_working_network_and_the_plain_chain_for_a_zero_network():
+    """The guard judges from counts only: a zero network (stage 1 passes everything, stage 2 is
+    the plain test) rejects at stage 2 what plain Metropolis rejects, so a loose bound keeps
+    it on and a tight one makes it fall back -- to the identical chain."""
+    a = small_ctx(N=1500)
+    b = small_ctx(N=1500)
+    with a, b:
+        b.surrogate_set(np.zeros(sr.NPARAM, dtype=np.float32))
+        b.da_enable(True)
+        b.da_guard(2, 0.99, 0.0)
+        a.sweep(6)
+        b.sweep(6)
+        assert b.da_status()["on"] and b.da_status()["fallbacks"] == 0
+        b.da_guard(2, 0.1, 0.0)
+        a.sweep(4)
+        b.sweep(4)
+        st = b.da_status()
+        assert not st["on"] and st["fallbacks"] == 1 and st["stage2_rejection"] > 0.1
+        qa, ua, _ = a.get_state_fixed()
+        qb, ub, _ = b.get_state_fixed()
+        for d in range(3):
+            assert np.array_equal(qa[d], qb[d]) and np.array_equal(ua[d], ub[d])

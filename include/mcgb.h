@@ -122,10 +122,15 @@ typedef struct mcgb_stats {
     double   box[3];
     double   volume;
     double   density;              /* N / (V - V_colloid), MCnptColloid.c:2799-2804          */
-    double   max_drift;            /* largest |running - recomputed| seen at a re-summation  */
-    uint64_t resummations;         /* FP64 re-summations of the running energy              */
+    double   max_drift;            /* largest |running - recomputed| at a periodic
+                                      re-summation (energy_check_interval, mcgb_resum_energy) */
+    uint64_t resummations;         /* FP64 re-summations of the running energy (all kinds)   */
     uint64_t da_stage1_rejected;   /* delayed acceptance: moves rejected by the surrogate    */
     uint64_t da_stage2_rejected;   /* delayed acceptance: rejected by the exact second stage */
+    double   max_drift_forced;     /* the same at re-summations forced by a sweep whose
+                                      accepted moves moved > 1e10 of finite energy (near-cap
+                                      pairs), where the running sum has lost its precision    */
+    uint64_t resummations_forced;  /* how many of the re-summations were forced               */
 } mcgb_stats;
 
 typedef struct mcgb_ctx mcgb_ctx;

@@ -1277,6 +1277,8 @@ int mcgb_get_stats(mcgb_ctx* c, int r, mcgb_stats* out) {
     out->resummations = d.resums;
     out->da_stage1_rejected = d.da_s1;
     out->da_stage2_rejected = d.da_s2;
+    out->max_drift_forced = d.max_drift_forced;
+    out->resummations_forced = d.resums_forced;
     return MCGB_OK;
 }
 
@@ -1319,9 +1321,9 @@ int mcgb_reset_counters(mcgb_ctx* c, int r) {
     if ((rc = read_dyn(c, D))) return rc;
     RepDyn& d = D[r];
     d.attempted = d.accepted = d.left = d.hard = d.cand = d.pairs = d.coll = d.vol_att = d.vol_acc = 0;
-    d.max_drift = 0.0;
+    d.max_drift = d.max_drift_forced = 0.0;
     d.overflow = 0;
-    d.resums = 0;
+    d.resums = d.resums_forced = 0;
     d.da_s1 = d.da_s2 = 0;
     return write_dyn(c, D);
 }

@@ -77,6 +77,8 @@ struct RepDyn {
     float dmax, drot;            // step sizes (float, as the reference's globals :108-109)
     unsigned long long ad_att, ad_acc;   // moves since the last step-size adaptation
     unsigned long long adapts;   // step-size adaptations made
+    double max_drift_forced;     // largest |running - recomputed| at a forced re-summation
+    unsigned long long resums_forced;   // re-summations forced by a sweep's moved energy
 };
 
 // Draws of one trial move (k_draws): particle selector, fixed-point step, rotation vector,

@@ -708,11 +708,19 @@ __global__ __launch_bounds__(256) void k_energy_reduce(const RepConst* __restric
     if (mode == 3) return;                             // compute only
     if (mode == 0 || mode == 1) {
         if (mode == 1) {
+            // every re-summation records its drift: periodic ones (and those the requested
+            // interval and a large moved energy both ask for) in max_drift, the ones forced by
+            // a sweep whose accepted moves moved > 1e10 of finite energy in max_drift_forced
             D.resums += 1;
-            double run = (double)D.cap_pairs * 1e9 + (double)D.cap_coll * 1e8 + D.e_fin;
-            double now = sc[0][0] * 1e9 + sc[0][1] * 1e8 + sf[0];
-            double dr = fabs(run - now);
-            if (dr > D.max_drift && D.moved_abs <= 1.0e10) D.max_drift = dr;
+            const double run = (double)D.cap_pairs * 1e9 + (double)D.cap_coll * 1e8 + D.e_fin;
+            const double now = sc[0][0] * 1e9 + sc[0][1] * 1e8 + sf[0];
+            const double dr = fabs(run - now);
+            if (D.moved_abs <= 1.0e10) {
+                if (dr > D.max_drift) D.max_drift = dr;
+            } else {
+                D.resums_forced += 1;
+                if (dr > D.max_drift_forced) D.max_drift_forced = dr;
+            }
         }
         D.cap_pairs = sc[0][0];
         D.cap_coll = sc[0][1];

@@ -66,7 +66,8 @@ class Stats(C.Structure):
         ("cap_pairs", C.c_int64), ("cap_colloid", C.c_int64), ("energy_finite", C.c_double),
         ("energy", C.c_double), ("box", C.c_double * 3), ("volume", C.c_double),
         ("density", C.c_double), ("max_drift", C.c_double), ("resummations", C.c_uint64),
-        ("da_stage1_rejected", C.c_uint64), ("da_stage2_rejected", C.c_uint64)]
+        ("da_stage1_rejected", C.c_uint64), ("da_stage2_rejected", C.c_uint64),
+        ("max_drift_forced", C.c_double), ("resummations_forced", C.c_uint64)]
 
     def as_dict(self):
         d = {n: getattr(self, n) for n, _ in self._fields_}

@@ -143,7 +143,22 @@ def series_report(x, seconds):
             "converged": bool(abs(z) < 2.0 and n >= 50 * tau)}
 
 
-def timed(ctx, sweeps, chunk=50000):
+def progress(msg):
+    """One progress line on stderr (stdout carries only the JSON result)."""
+    print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
+
+
+def advance(ctx, sweeps, label, chunk=20000):
+    """Untimed sweeps in chunks, with a progress line per chunk."""
+    done = 0
+    while done < sweeps:
+        n = min(chunk, sweeps - done)
+        ctx.sweep(n)
+        done += n
+        progress(f"{label}: {done}/{sweeps} sweeps")
+
+
+def timed(ctx, sweeps, chunk=20000, label="timed"):
     """Runs `sweeps` sweeps (mcgb_sweep returns after hipStreamSynchronize) and returns the wall
     seconds spent in them, the counters, and the per-sweep E/N, P2 and density series, read
     after every `chunk` sweeps (the device ring holds 65536 entries; the reads are not timed)."""
@@ -160,6 +175,7 @@ def timed(ctx, sweeps, chunk=50000):
         parts.append(ctx.series(max_n=n + 16))
         ctx.series_clear()
         left -= n
+        progress(f"{label}: {sweeps - left}/{sweeps} sweeps, {dt:.1f} s")
     st = ctx.stats()
     e, p2, rho = (np.concatenate([x[k] for x in parts]) for k in range(3))
     return dt, st, e, p2, rho
@@ -188,9 +204,9 @@ def ess_leg(args, waves, schedule, T=1.0):
         ctx.set_state(pos, u, box)
         ctx.set_kernel_timing(False)
         t0 = time.perf_counter()
-        ctx.sweep(args.ess_equil)
+        advance(ctx, args.ess_equil, "ess equilibration")
         out["equilibration_s"] = time.perf_counter() - t0
-        dt, st, e, p2, _ = timed(ctx, args.ess_sweeps)
+        dt, st, e, p2, _ = timed(ctx, args.ess_sweeps, label="ess measurement")
         out["seconds"] = dt
         out["moves_per_s"] = st["attempted"] / dt
         out["acceptance"] = st["accepted"] / max(1, st["attempted"])
@@ -230,7 +246,7 @@ def da_leg(args, pairs_plain, waves, schedule):
     with m.Context(p) as ctx:
         ctx.set_state(pos, u, box)
         ctx.set_kernel_timing(False)
-        ctx.sweep(args.da_equil)
+        advance(ctx, args.da_equil, "da equilibration")
         # training pairs: this trajectory plus the T* = 1.0 nematic one
         t0 = time.perf_counter()
         ctx.pairs_reserve(int(1.2 * args.pairs) + 4 * 10 ** 6)
@@ -262,7 +278,7 @@ def da_leg(args, pairs_plain, waves, schedule):
                 ctx.set_state_fixed(q0, u0, L0)
                 ctx.da_guard(50, 0.9, 0.02)      # falls back only for a broken network
             ctx.sweep(2)                                   # graph capture outside the timing
-            dt, st, e, p2, _ = timed(ctx, args.da_sweeps)
+            dt, st, e, p2, _ = timed(ctx, args.da_sweeps, label=f"da leg, {mode}")
             evaluated = st["attempted"] - st["rejected_left_cell"] - st["rejected_hard_core"]
             r = {"moves_per_s": st["attempted"] / dt, "ms_per_sweep": 1e3 * dt / args.da_sweeps,
                  "acceptance": st["accepted"] / max(1, evaluated),
@@ -322,7 +338,7 @@ def npt_leg(args, waves, schedule):
         ctx.sweep(300)
         ctx.set_step_sizes(0.1, 0.1)
         ctx.sweep(2)
-        dt, st, e, p2, rho = timed(ctx, args.npt_sweeps)
+        dt, st, e, p2, rho = timed(ctx, args.npt_sweeps, label="npt")
     h = len(rho) // 2
     return {"workload": "NPT N=64000 P*=2 T*=1 from rho*=0.28, one ln V move per sweep",
             "sublattices": args.npt_colors ** 3,
@@ -414,7 +430,7 @@ def cfg0_leg(args):
                 qstart = ctx.get_state_fixed()
                 t_start = int(ctx.stats()["sweeps"])
             ctx.sweep(2)
-            dt, st, e, p2, _ = timed(ctx, S)
+            dt, st, e, p2, _ = timed(ctx, S, label=f"cfg0 gpu fp{prec}")
         out[f"gpu_fp{prec}"] = {"moves_per_s": st["attempted"] / dt, "seconds": dt,
                                 "E/N": series_report(e, dt), "P2": series_report(p2, dt)}
         if prec == 64:

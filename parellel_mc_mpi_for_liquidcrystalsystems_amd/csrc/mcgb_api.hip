@@ -196,6 +196,18 @@ void fill_consts(const mcgb_params& p, RepConst* C) {
 }
 
 int check_params(const mcgb_params& p, int r) {
+    // non-finite inputs would reach every energy as NaN (a NaN dE is never accepted, but the
+    // running total would be lost): refused here
+    const double vals[] = {p.system_density, p.system_temperature, p.system_pressure,
+                           p.gb_sigma_sidetoside, p.gb_epsilon0, p.gb_kappa, p.gb_kappa_prime,
+                           p.gb_mu_exponent, p.gb_nu_exponent, (double)p.max_displacement,
+                           (double)p.max_rotation, p.cutoff_radius, p.colloid_radius,
+                           p.npt_max_dlnv};
+    for (double v : vals)
+        if (!std::isfinite(v)) {
+            mcgb_set_error("replica %d: a parameter is not finite", r);
+            return MCGB_ERR_ARG;
+        }
     if (p.num_particles <= 0 || p.system_density <= 0.0 || p.system_temperature <= 0.0 ||
         p.cutoff_radius <= 0.0 || p.gb_sigma_sidetoside <= 0.0 || p.gb_mu_exponent == 0.0 ||
         p.gb_kappa <= 0.0 || p.gb_kappa_prime <= 0.0) {
@@ -552,12 +564,6 @@ int mcgb_create(const mcgb_params* p, int n_replicas, mcgb_ctx** out) {
         return MCGB_ERR_ARG;
     }
     *out = nullptr;
-    int ndev = 0;
-    hipError_t he = hipGetDeviceCount(&ndev);
-    if (he != hipSuccess || ndev == 0) {
-        mcgb_set_error("no HIP device available (%s)", hipGetErrorString(he));
-        return MCGB_ERR_HIP;
-    }
     const int prec = p[0].precision;
     if (prec != 32 && prec != 64) {
         mcgb_set_error("precision must be 32 or 64 (got %d)", prec);
@@ -570,6 +576,12 @@ int mcgb_create(const mcgb_params* p, int n_replicas, mcgb_ctx** out) {
             mcgb_set_error("replica %d: all replicas must share one precision", r);
             return MCGB_ERR_ARG;
         }
+    }
+    int ndev = 0;
+    hipError_t he = hipGetDeviceCount(&ndev);
+    if (he != hipSuccess || ndev == 0) {
+        mcgb_set_error("no HIP device available (%s)", hipGetErrorString(he));
+        return MCGB_ERR_HIP;
     }
     mcgb_ctx* c = new mcgb_ctx();
     c->nrep = n_replicas;

@@ -233,3 +233,14 @@ def test_driver_random_start_reports_jammed_density(tmp_path):
     path = _driver_params(tmp_path, num_particles=2000, initial_configuration_type=1)
     p = subprocess.run([drv, str(path)], capture_output=True, text=True, cwd=tmp_path, timeout=120)
     assert p.returncode == 1 and "no room for particle" in p.stderr
+
+
+@pytest.mark.parametrize("field", ["gb_epsilon0", "system_temperature", "max_rotation",
+                                   "colloid_radius"])
+def test_create_refuses_non_finite_parameters(field):
+    # checked before any device call: the same answer with or without a GPU
+    p = m.gb_params(8000, 0.3, 1.2)
+    setattr(p, field, float("nan"))
+    with pytest.raises(m.McgbError) as e:
+        m.Context(p)
+    assert e.value.code == m.ERR_ARG and "not finite" in str(e.value)

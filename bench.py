@@ -40,7 +40,7 @@ F_CAND, F_PAIR, F_COLL = 17.0, 68.0, 45.0
 # HBM bytes per launch of the W=2 pass kernel on this workload: rocprofv3 FETCH_SIZE x 2.000 +
 # WRITE_SIZE x 1.000 (the factors a plain 256 MiB float32 copy reads), from the two counter
 # passes summarised in profiles/sweep_fp32_N256000_colloid_hbm_traffic.md
-TRAFFIC_PASS_BYTES = 28.37e6
+TRAFFIC_PASS_BYTES = 28.26e6
 # pass schedules (mcgb_params.pass_schedule)
 SCHEDULES = ("sequential", "speculative", "sequential_4waves")
 # step sizes that move the lattice start's capped overlaps out of the cap (untimed)
@@ -472,8 +472,10 @@ def main():
     ap.add_argument("--no-cfg0", action="store_true", help="skip the N=8000 CPU-configuration leg")
     ap.add_argument("--cpu-sweeps", type=int, default=10)
     ap.add_argument("--n", type=int, default=256000)
+    ap.add_argument("--relax-big", type=int, default=300,
+                    help=f"untimed overlap-removal sweeps at {RELAX_STEP}/{RELAX_ROT}")
     ap.add_argument("--relax", type=int, default=200,
-                    help="untimed sweeps at 0.1/0.1 after the 300 overlap-removal sweeps")
+                    help="untimed sweeps at 0.1/0.1 after the overlap-removal sweeps")
     ap.add_argument("--waves", type=int, default=0, help="wavefronts per cell (0: autotune)")
     ap.add_argument("--schedule", type=int, default=1,
                     help="pass schedule with --waves: 0 sequential, 1 speculative, "
@@ -515,7 +517,7 @@ def main():
         t_relax = time.perf_counter()
         ctx.set_kernel_timing(False)
         ctx.set_step_sizes(RELAX_STEP, RELAX_ROT)
-        ctx.sweep(300)
+        ctx.sweep(args.relax_big)
         caps_after_relax = ctx.stats()["cap_pairs"]
         ctx.set_step_sizes(0.1, 0.1)
         if args.relax:
@@ -568,8 +570,9 @@ def main():
                    "cutoff": 4.0, "max_displacement": 0.1, "max_rotation": 0.1,
                    "lattice_jitter": 0.05, "seed": seed,
                    "step": f"one sweep ({args.colors ** 3} sublattice passes)",
-                   "start": f"jittered lattice; 300 untimed sweeps at {RELAX_STEP}/{RELAX_ROT} "
-                            f"(capped overlaps left: {caps_after_relax}), {args.relax} at 0.1/0.1"},
+                   "start": f"jittered lattice; {args.relax_big} untimed sweeps at "
+                            f"{RELAX_STEP}/{RELAX_ROT} (capped overlaps left: {caps_after_relax}), "
+                            f"{args.relax} at 0.1/0.1"},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP32_TFLOPS,
                      "traffic": (TRAFFIC_PASS_BYTES if (tune_w == 2 and tune_s == 0 and args.colors == 2)

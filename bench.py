@@ -42,7 +42,7 @@ F_CAND, F_PAIR, F_COLL = 17.0, 68.0, 45.0
 # passes summarised in profiles/sweep_fp32_N256000_colloid_hbm_traffic.md
 TRAFFIC_PASS_BYTES = 28.26e6
 # pass schedules (mcgb_params.pass_schedule)
-SCHEDULES = ("sequential", "speculative", "sequential_4waves")
+SCHEDULES = ("sequential", "speculative", "sequential_4waves", "two_moves_in_flight")
 # step sizes that move the lattice start's capped overlaps out of the cap (untimed)
 RELAX_STEP, RELAX_ROT = 0.6, 0.6
 
@@ -479,7 +479,8 @@ def main():
     ap.add_argument("--waves", type=int, default=0, help="wavefronts per cell (0: autotune)")
     ap.add_argument("--schedule", type=int, default=1,
                     help="pass schedule with --waves: 0 sequential, 1 speculative, "
-                         "2 sequential built for four wavefronts per SIMD")
+                         "2 sequential built for four wavefronts per SIMD, 3 two moves in "
+                         "flight (two wavefronts)")
     ap.add_argument("--ess-equil", type=int, default=20000)
     ap.add_argument("--ess-sweeps", type=int, default=20000)
     ap.add_argument("--da-equil", type=int, default=20000)

@@ -77,7 +77,9 @@ typedef struct mcgb_params {
     int      pass_schedule;                 /* 1: speculative (moves of a cell evaluated in
                                                parallel, decided in order), 0: sequential,
                                                2: sequential built for four wavefronts per
-                                               SIMD (FP32; FP64 runs 0)                    */
+                                               SIMD (FP32; FP64 runs 0), 3: two moves of a
+                                               cell in flight, one per wavefront (two
+                                               wavefronts per cell; waves_per_cell ignored) */
     int      adapt_moves;                   /* step-size adaptation every >= adapt_moves
                                                attempted moves (the reference: 10000), 0 = off */
     int      sublattice_colors;             /* 2: 8 sublattices of cells >= r_c wide (default);
@@ -233,14 +235,15 @@ int mcgb_series(mcgb_ctx *ctx, int replica, int max,
 int mcgb_series_clear(mcgb_ctx *ctx);
 
 /* Autotuner: times `sweeps` sweeps with each pass schedule (0 sequential, 1 speculative,
- * 2 sequential built for four wavefronts per SIMD) and each supported number of wavefronts per
- * active cell (1, 2, 3, 4, 8; the LDS staging capacity follows) and keeps the fastest.  The
- * timed sweeps are ordinary sweeps of the simulation (they advance it).  No counterpart in the
- * reference.  ms_per_pass (may be NULL) receives the measured pass times, index
- * 5*schedule + {0,1,2,3,4} for 1, 2, 3, 4, 8 wavefronts (0 for a schedule it does not run: the
- * speculative one with sublattice_colors 3, schedule 2 in FP64). */
+ * 2 sequential built for four wavefronts per SIMD, 3 two moves in flight) and each supported
+ * number of wavefronts per active cell (1, 2, 3, 4, 8; schedule 3: 2 only; the LDS staging
+ * capacity follows) and keeps the fastest.  The timed sweeps are ordinary sweeps of the
+ * simulation (they advance it).  No counterpart in the reference.  ms_per_pass (may be NULL)
+ * receives the measured pass times, index 5*schedule + {0,1,2,3,4} for 1, 2, 3, 4, 8
+ * wavefronts (0 for a combination it does not run: the speculative schedule with
+ * sublattice_colors 3, schedule 2 in FP64, schedule 3 with other than 2 wavefronts). */
 int mcgb_autotune(mcgb_ctx *ctx, int sweeps, int *best_waves, int *best_schedule,
-                  double ms_per_pass[15]);
+                  double ms_per_pass[20]);
 
 /* Average device time per launch of the sweep-pass kernel, the cell-list rebuild and the whole
  * sweep, measured with HIP events on the context's stream over the last mcgb_sweep() call. */

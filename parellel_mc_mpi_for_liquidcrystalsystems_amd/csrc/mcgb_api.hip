@@ -12,6 +12,7 @@
 
 #include "mcgb_internal.h"
 #include "mcgb_kernels.hpp"
+#include "mcgb_duo.hpp"
 
 using namespace mcgb;
 
@@ -61,8 +62,10 @@ struct mcgb_ctx {
     int waves = 4;
     size_t lds = 0;
     size_t lds_spec = 0;
+    size_t lds_duo = 0;
     bool spec = true;          // speculative pass schedule for untraced plain sweeps
     bool o4 = false;           // FP32 sequential pass built for four wavefronts per SIMD
+    bool duo = false;          // two moves in flight per cell (two wavefronts), traced or not
     int resum_interval = 100;
     int series_cap = kSeriesCapDefault;
     bool any_npt = false;
@@ -193,6 +196,8 @@ void fill_consts(const mcgb_params& p, RepConst* C) {
     C->f_g2c1 = (float)(C->chip * (1.0 - C->chip));
     C->f_g2d0 = (float)(1.0 - C->chip2);
     C->f_rc2_pre = (float)(C->rc2 * (1.0 + 1e-5));
+    const bool imu = C->mu == 1.0 || C->mu == 2.0, inu = C->nu == 1.0 || C->nu == 2.0;
+    C->f_ipow = imu && inu ? (int)C->mu + 2 * ((int)C->nu - 1) : 0;
 }
 
 int check_params(const mcgb_params& p, int r) {
@@ -312,6 +317,8 @@ void launch_pass_t(mcgb_ctx* c, const PassArgs& P) {
         case 8: k_pass_da<Real, 8><<<c->nblk, 512, c->lds, c->st>>>(P); break;
         default: k_pass_da<Real, 4><<<c->nblk, 256, c->lds, c->st>>>(P); break;
         }
+    } else if (c->duo && c->waves == 2) {
+        k_pass_duo<Real, TR><<<c->nblk, 128, c->lds_duo, c->st>>>(P);
     } else if (!TR && c->spec) {
         switch (c->waves) {
         case 1: k_pass_spec<Real, 1><<<c->nblk, 64, c->lds_spec, c->st>>>(P); break;
@@ -371,10 +378,12 @@ size_t pass_lds_spec(int waves, int cap) {
     default: return SpecLds<Real, 4>::bytes(cap);
     }
 }
-// LDS of the larger of the two schedules' staging layouts (both use the same capacity)
+// LDS of the largest of the schedules' staging layouts (all use the same capacity; the
+// two-moves schedule runs with two wavefronts only)
 template <typename Real>
 size_t pass_lds(int waves, int cap) {
-    return std::max(pass_lds_seq<Real>(waves, cap), pass_lds_spec<Real>(waves, cap));
+    size_t b = std::max(pass_lds_seq<Real>(waves, cap), pass_lds_spec<Real>(waves, cap));
+    return waves == 2 ? std::max(b, DuoLds<Real>::bytes(cap)) : b;
 }
 
 // One sweep of every replica.  ev (may be null): npass + 3 events recorded at the sweep start,
@@ -672,6 +681,7 @@ int mcgb_create(const mcgb_params* p, int n_replicas, mcgb_ctx** out) {
         mcgb_destroy(c);
         return MCGB_ERR_ARG;
     }
+    if (p[0].pass_schedule == 3) c->waves = 2;         // one move per wavefront, two in flight
     const double ncells_stencil = c->col == 3 ? 125.0 : 27.0;
     int cap = (int)(ncells_stencil * occ_max * 1.10 + 32.0);
     cap = (cap + 31) & ~31;
@@ -683,8 +693,10 @@ int mcgb_create(const mcgb_params* p, int n_replicas, mcgb_ctx** out) {
     c->cap = cap;
     c->lds = c->rsize == 8 ? pass_lds_seq<double>(c->waves, cap) : pass_lds_seq<float>(c->waves, cap);
     c->lds_spec = c->rsize == 8 ? pass_lds_spec<double>(c->waves, cap) : pass_lds_spec<float>(c->waves, cap);
+    c->lds_duo = c->rsize == 8 ? DuoLds<double>::bytes(cap) : DuoLds<float>::bytes(cap);
     c->spec = p[0].pass_schedule == 1 && c->col == 2;    // the speculative pass: 8 sublattices
     c->o4 = p[0].pass_schedule == 2 && c->rsize == 4;    // four wavefronts per SIMD: FP32
+    c->duo = p[0].pass_schedule == 3;                    // two moves in flight: two wavefronts
 
     // ---- device memory
     int rc = 0;
@@ -1435,11 +1447,12 @@ static int set_waves(mcgb_ctx* c, int waves) {
     c->cap = cap;
     c->lds = c->rsize == 8 ? pass_lds_seq<double>(waves, cap) : pass_lds_seq<float>(waves, cap);
     c->lds_spec = c->rsize == 8 ? pass_lds_spec<double>(waves, cap) : pass_lds_spec<float>(waves, cap);
+    c->lds_duo = c->rsize == 8 ? DuoLds<double>::bytes(cap) : DuoLds<float>::bytes(cap);
     return MCGB_OK;
 }
 
 int mcgb_autotune(mcgb_ctx* c, int sweeps, int* best_waves, int* best_schedule,
-                  double ms_per_pass[15]) {
+                  double ms_per_pass[20]) {
     int rc = check_ctx(c, 0);
     if (rc) return rc;
     if (sweeps <= 0) { mcgb_set_error("mcgb_autotune: sweeps must be > 0"); return MCGB_ERR_ARG; }
@@ -1448,10 +1461,11 @@ int mcgb_autotune(mcgb_ctx* c, int sweeps, int* best_waves, int* best_schedule,
     c->timing = true;
     double best = 1e300;
     int bw = c->waves;
-    int bs = c->spec ? 1 : (c->o4 ? 2 : 0);
-    for (int sched = 0; sched < 3; ++sched) {
+    int bs = c->duo ? 3 : (c->spec ? 1 : (c->o4 ? 2 : 0));
+    for (int sched = 0; sched < 4; ++sched) {
         c->spec = sched == 1;
         c->o4 = sched == 2;
+        c->duo = sched == 3;
         // the speculative pass runs 8 sublattices only, the four-wavefront build FP32 only
         if ((c->spec && c->col != 2) || (c->o4 && c->rsize != 4)) {
             if (ms_per_pass)
@@ -1459,6 +1473,11 @@ int mcgb_autotune(mcgb_ctx* c, int sweeps, int* best_waves, int* best_schedule,
             continue;
         }
         for (int k = 0; k < 5; ++k) {
+            // the two-moves schedule runs two wavefronts per cell only
+            if (c->duo && cand[k] != 2) {
+                if (ms_per_pass) ms_per_pass[5 * sched + k] = 0.0;
+                continue;
+            }
             set_waves(c, cand[k]);
             if ((rc = sweep_impl(c, sweeps, nullptr))) { c->timing = tm; return rc; }
             if (ms_per_pass) ms_per_pass[5 * sched + k] = c->ms_pass;
@@ -1467,6 +1486,7 @@ int mcgb_autotune(mcgb_ctx* c, int sweeps, int* best_waves, int* best_schedule,
     }
     c->spec = bs == 1;
     c->o4 = bs == 2;
+    c->duo = bs == 3;
     set_waves(c, bw);
     c->timing = tm;
     if (best_waves) *best_waves = bw;

@@ -44,6 +44,9 @@ struct RepConst {
     // g(om) = (c0 + c1 (|p|^2+|q|^2)/r^2 + om^2 (r.w)^2/r^2) / (d0 + om^2 |w|^2)
     float f_g1c0, f_g1c1, f_g1d0, f_g2c0, f_g2c1, f_g2d0;
     float f_rc2_pre;       // candidate pre-filter, slightly above rc^2 (final test in FP64)
+    // exponents of eps as small integers: 0 general; mu + 2 (nu - 1) for mu, nu in {1, 2}
+    // (1: mu = nu = 1, 2: mu = 2, nu = 1, 3: mu = 1, nu = 2, 4: mu = nu = 2)
+    int f_ipow;
 };
 
 // Per-replica state updated on the device.
@@ -97,6 +100,7 @@ struct GBf {
     float sigma0, four_eps0, chi2, chip2, mu, half_nu;
     float g1c0, g1c1, g1d0, g2c0, g2c1, g2d0;
     double sigma0d;
+    int ipow;            // exponent case of eps (see f_ipow)
 };
 
 // FP64 constants of the pair function.
@@ -168,11 +172,14 @@ __host__ __device__ inline int grid_dim(double L, double rc, int col) {
 // |p x q|^2 = (r.w)^2 r^2):
 //   g(om) = [ (1-om)^2 + om(1-om)(|p|^2 + |q|^2)/r^2 + om^2 (r.w)^2/r^2 ] / [ (1-om^2) + om^2 |w|^2 ],
 // a ratio of sums of non-negative terms for 0 <= om < 1 (the reference's form subtracts two
-// nearly equal numbers near end-to-end contact).  eps1^nu eps2^mu =
-// exp2(mu (log2 N' - log2 D') - (nu/2) log2 D), D = 1 - chi^2 c^2.  The distance r comes from
+// nearly equal numbers near end-to-end contact).  sigma = sigma0 sqrt(D/N) = sigma0 D rsq(N D),
+// D = 1 - chi^2 c^2.  eps1^nu eps2^mu = (N'/D')^mu D^(-nu/2): for integer exponents
+// mu, nu in {1, 2} (GB(3,5,2,1) among them) N'^mu rsq(D'^(2 mu) D^nu), one reciprocal square
+// root; otherwise exp2(mu (log2 N' - log2 D') - (nu/2) log2 D).  The distance r comes from
 // the exact FP64 r^2 (r2d) with one FP64 Newton step on the hardware rsqrt, and
 // r - sigma + sigma0 is formed in FP64: the remaining error is that of sigma (a few FP32 ulps).
-// Transcendentals: v_rsq x2, v_rcp x2, v_log x3, v_exp x1.  cap = the 1e9 cap.
+// Transcendentals: v_rsq x3, v_rcp x1 (general exponents: v_rsq x2, v_rcp x1, v_log x3,
+// v_exp x1).  cap = the 1e9 cap.
 // Written for Q pairs that share e_i, each step done for every pair before the next step, so
 // that Q independent dependency chains interleave; Q = 1 is the single-pair function.  Every
 // pair goes through the same operations whatever Q is.
@@ -218,11 +225,29 @@ __device__ inline void gb_pair_q_f32(const GBf& C, float uix, float uiy, float u
         d2[q] = C.g2d0 + C.chip2 * w2[q];
     }
 #pragma unroll
-    MCGB_EACH {
-        eps[q] = __builtin_amdgcn_exp2f(
+    MCGB_EACH sigma[q] = C.sigma0 * d1[q] * __builtin_amdgcn_rsqf(n1[q] * d1[q]);
+    // the exponent case is uniform (a scalar branch)
+    const int ip = __builtin_amdgcn_readfirstlane(C.ipow);
+    if (ip == 2) {                                   // mu = 2, nu = 1
+#pragma unroll
+        MCGB_EACH {
+            const float d2s = d2[q] * d2[q];
+            eps[q] = n2[q] * n2[q] * __builtin_amdgcn_rsqf(d2s * d2s * d1[q]);
+        }
+    } else if (ip != 0) {                            // mu, nu in {1, 2}
+        const bool mu2 = ip == 4, nu2 = ip >= 3;
+#pragma unroll
+        MCGB_EACH {
+            const float d2s = d2[q] * d2[q];
+            const float num = mu2 ? n2[q] * n2[q] : n2[q];
+            const float den = (mu2 ? d2s * d2s : d2s) * (nu2 ? d1[q] * d1[q] : d1[q]);
+            eps[q] = num * __builtin_amdgcn_rsqf(den);
+        }
+    } else {
+#pragma unroll
+        MCGB_EACH eps[q] = __builtin_amdgcn_exp2f(
             C.mu * (__builtin_amdgcn_logf(n2[q]) - __builtin_amdgcn_logf(d2[q])) -
             C.half_nu * __builtin_amdgcn_logf(d1[q]));
-        sigma[q] = C.sigma0 * __builtin_amdgcn_rsqf(n1[q] * __builtin_amdgcn_rcpf(d1[q]));
     }
     // r in FP64: one Newton step on the FP32 reciprocal square root
 #pragma unroll
@@ -345,6 +370,7 @@ __host__ __device__ inline GBf make_gbf(const RepConst& C) {
     g.g1c0 = C.f_g1c0; g.g1c1 = C.f_g1c1; g.g1d0 = C.f_g1d0;
     g.g2c0 = C.f_g2c0; g.g2c1 = C.f_g2c1; g.g2d0 = C.f_g2d0;
     g.sigma0d = C.sigma0;
+    g.ipow = C.f_ipow;
     return g;
 }
 

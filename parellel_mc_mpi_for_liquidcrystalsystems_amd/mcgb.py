@@ -500,13 +500,13 @@ class Context:
         check(lib().mcgb_series_clear(self.h))
 
     def autotune(self, sweeps=4):
-        """Returns (waves_per_cell, schedule, {"seq/W", "spec/W" or "seq4/W": ms per pass}).
-        With delayed acceptance on, every schedule runs the delayed-acceptance kernel, so it
-        tunes W for that kernel."""
+        """Returns (waves_per_cell, schedule, {"seq/W", "spec/W", "seq4/W" or "duo/W": ms per
+        pass}; 0 for a combination not run).  With delayed acceptance on, every schedule runs
+        the delayed-acceptance kernel, so it tunes W for that kernel."""
         best, sched = C.c_int(), C.c_int()
-        ms = np.empty(15)
+        ms = np.empty(20)
         check(lib().mcgb_autotune(self.h, sweeps, C.byref(best), C.byref(sched), _d(ms)))
-        keys = [f"{s}/{w}" for s in ("seq", "spec", "seq4") for w in (1, 2, 3, 4, 8)]
+        keys = [f"{s}/{w}" for s in ("seq", "spec", "seq4", "duo") for w in (1, 2, 3, 4, 8)]
         return best.value, sched.value, dict(zip(keys, ms.tolist()))
 
     def set_kernel_timing(self, on=True):

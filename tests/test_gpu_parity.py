@@ -17,6 +17,10 @@ GOLD = os.path.join(os.path.dirname(__file__), "golden")
 PHYS = np.load(os.path.join(GOLD, "ref_physics.npz"))
 SETS = {"gb3521": (1.0, 1.0, 3.0, 5.0, 2.0, 1.0), "gb44_20_1_1": (1.0, 1.0, 4.4, 20.0, 1.0, 1.0),
         "lj_limit": (1.0, 1.0, 1.0, 1.0, 2.0, 1.0), "gb3512_s12": (1.2, 0.7, 3.0, 5.0, 1.0, 2.0)}
+# the FP32 pair takes eps through one reciprocal square root for integer exponents mu, nu in
+# {1, 2} and through log2/exp2 otherwise: every case, against the FP64 kernel
+SETS_FP32 = dict(SETS, gb3522=(1.0, 1.0, 3.0, 5.0, 2.0, 2.0),
+                 gb35_frac=(1.0, 1.0, 3.0, 5.0, 1.5, 0.5))
 
 
 @pytest.fixture(scope="module", autouse=True)
@@ -26,7 +30,7 @@ def need_gpu():
 
 
 def params_for(name, R=0.0, precision=64):
-    s0, e0, k, kp, mu, nu = SETS[name]
+    s0, e0, k, kp, mu, nu = SETS_FP32[name]
     p = m.gb_params(1000, 0.3, 1.0, colloid_radius=R, precision=precision, kappa=k,
                     kappa_prime=kp, mu=mu, nu=nu)
     p.gb_sigma_sidetoside, p.gb_epsilon0 = s0, e0
@@ -80,10 +84,10 @@ def _terms(c, ei, ej, dr):
     return c.four_eps0 * eps * rho ** 6 * (rho ** 6 + 1), rho
 
 
-@pytest.mark.parametrize("name", sorted(SETS))
+@pytest.mark.parametrize("name", sorted(SETS_FP32))
 def test_pair_fp32_within_1e5(name):
     p = params_for(name, precision=32)
-    s0, e0, k, kp, mu, nu = SETS[name]
+    s0, e0, k, kp, mu, nu = SETS_FP32[name]
     c = oc.consts(sigma0=s0, epsilon0=e0, kappa=k, kappa_prime=kp, mu=mu, nu=nu)
     rng = np.random.default_rng(11)
     n = 20000
@@ -386,11 +390,11 @@ def test_speculative_schedule_matches_oracle_fp64(R, start):
 
 
 def test_pass_schedules_agree():
-    # sequential and speculative schedules: identical FP64 decisions, every wavefront count
-    # (schedule 2 is the sequential one in FP64)
+    # sequential, speculative and two-moves schedules: identical FP64 decisions, every
+    # wavefront count (schedule 2 is the sequential one in FP64; schedule 3 runs two)
     ref = None
-    for sched in (0, 1, 2):
-        for w in ((2, 3) if sched != 1 else (1, 2, 3, 4, 8)):
+    for sched in (0, 1, 2, 3):
+        for w in ((2,) if sched == 3 else (2, 3) if sched != 1 else (1, 2, 3, 4, 8)):
             q, u, L = layered_system(2500, 0.3, colloid_radius=3.0, seed=44)
             p = m.gb_params(2500, 0.3, 1.0, colloid_radius=3.0, precision=64, seed=7070,
                             energy_check_interval=0, waves_per_cell=w, pass_schedule=sched)
@@ -412,10 +416,11 @@ def test_autotune_keeps_results_and_picks_a_supported_width():
     ctx, s, c = make_pair(N=1300, precision=64, seed=17)
     with ctx:
         best, sched, ms = ctx.autotune(2)
-        assert best in (1, 2, 3, 4, 8) and sched in (0, 1)
-        assert all(v > 0 for k, v in ms.items() if not k.startswith("seq4/"))
-        assert all(v == 0 for k, v in ms.items() if k.startswith("seq4/"))   # FP32 only
-        for _ in range(2 * 5 * 2):                   # 2 schedules x 5 widths x 2 sweeps
+        assert best in (1, 2, 3, 4, 8) and sched in (0, 1, 3)
+        run = [k for k in ms if not k.startswith("seq4/") and (not k.startswith("duo/") or k == "duo/2")]
+        assert all(ms[k] > 0 for k in run)
+        assert all(ms[k] == 0 for k in ms if k not in run)    # seq4: FP32 only; duo: W = 2 only
+        for _ in range((2 * 5 + 1) * 2):             # 2 schedules x 5 widths, duo at W = 2; 2 sweeps
             s.sweep()
         q, u, _ = ctx.get_state_fixed()
         for d in range(3):
@@ -428,8 +433,8 @@ def test_autotune_fp32_times_every_schedule_and_keeps_the_energy():
     ctx, s, c = make_pair(N=9000, R=5.0, precision=32, seed=19)
     with ctx:
         best, sched, ms = ctx.autotune(2)
-        assert best in (1, 2, 3, 4, 8) and sched in (0, 1, 2)
-        assert all(v > 0 for v in ms.values())
+        assert best in (1, 2, 3, 4, 8) and sched in (0, 1, 2, 3)
+        assert all((v > 0) == (not k.startswith("duo/") or k == "duo/2") for k, v in ms.items())
         ctx.sweep(4)
         st = ctx.stats()
         tot = ctx.total_energy()

@@ -235,6 +235,14 @@ def test_poisoned_network_falls_back_to_the_plain_kernel(poison):
         w = ctx.surrogate_get()
         w *= 1e30 if poison == "overflow" else 1000.0
         ctx.surrogate_set(w)
+        if poison == "overflow":
+            # the poison forces non-finite surrogate energies (inf - inf: NaN) into stage 1,
+            # where the ordered compare rejects them
+            y = ctx.surrogate_eval(features(4096, 5), mfma=True)
+            bad = np.mean(~np.isfinite(y))
+            print(f"non-finite surrogate outputs: {bad:.3f} (NaN {np.mean(np.isnan(y)):.3f})",
+                  flush=True)
+            assert bad > 0.9
         ctx.da_enable(True)
         ctx.da_guard(5, 0.5, 0.02)
         ctx.sweep(5)                               # one window: delayed acceptance
@@ -254,6 +262,8 @@ def test_poisoned_network_falls_back_to_the_plain_kernel(poison):
         tot = ctx.total_energy()
         assert s["cap_pairs"] == tot[1]
         assert s["energy_finite"] == pytest.approx(tot[3], rel=1e-6, abs=1e-3)
+        q, u, _ = ctx.get_state()
+        assert all(np.all(np.isfinite(x)) for x in (*q, *u))
 
 
 def test_guard_keeps_a_working_network_and_the_plain_chain_for_a_zero_network():

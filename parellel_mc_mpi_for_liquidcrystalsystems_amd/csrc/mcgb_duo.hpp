@@ -274,6 +274,8 @@ __device__ __forceinline__ void pass_cell_duo(const PassArgs& P, int b, unsigned
                                         (double)mp.ny, (double)mp.nz, &cn);
                     mp.flags = cn ? 1u << 16 : 0u;
                 }
+                mp.flags |= prep_tests(qj.x, qj.y, qj.z, d.dq[0], d.dq[1], d.dq[2], gx, gy, gz,
+                                       nx, ny, nz, cx, cy, cz, cnear, R2, X, FC, &mp.fr2);
             }
             if (w == 0) {
                 uint32_t dup = 0;
@@ -335,19 +337,23 @@ __device__ __forceinline__ void pass_cell_duo(const PassArgs& P, int b, unsigned
                     qnx = qix + (uint32_t)dqx; qny = qiy + (uint32_t)dqy; qnz = qiz + (uint32_t)dqz;
                     thr = pm.thr;
                     ++n_att;
-                    const double rnx = (double)(int32_t)(qnx - 0x80000000u) * X.s0;
-                    const double rny = (double)(int32_t)(qny - 0x80000000u) * X.s1;
-                    const double rnz = (double)(int32_t)(qnz - 0x80000000u) * X.s2;
-                    if (cell_coord(qnx, gx, (uint32_t)nx) != (uint32_t)cx ||
-                        cell_coord(qny, gy, (uint32_t)ny) != (uint32_t)cy ||
-                        cell_coord(qnz, gz, (uint32_t)nz) != (uint32_t)cz) {
+                    float fr2;
+                    const uint32_t tests =
+                        fresh ? fl & (kPrepLeft | kPrepHard)
+                              : prep_tests(qix, qiy, qiz, dqx, dqy, dqz, gx, gy, gz, nx, ny, nz,
+                                           cx, cy, cz, cnear, R2, X, FC, &fr2);
+                    if (fresh) fr2 = uni(pm.fr2);
+                    if (tests & kPrepLeft) {
                         outcome = MCGB_MOVE_LEFT;
                         ++n_left;
-                    } else if (cnear && rnx * rnx + rny * rny + rnz * rnz <= R2) {
+                    } else if (tests & kPrepHard) {
                         outcome = MCGB_MOVE_HARD;
                         ++n_hard;
                     } else {
                         evaluated = true;
+                        const double rnx = (double)(int32_t)(qnx - 0x80000000u) * X.s0;
+                        const double rny = (double)(int32_t)(qny - 0x80000000u) * X.s1;
+                        const double rnz = (double)(int32_t)(qnz - 0x80000000u) * X.s2;
                         if (fresh) {
                             unx = uni(pm.nx); uny = uni(pm.ny); unz = uni(pm.nz);
                         } else {
@@ -383,7 +389,7 @@ __device__ __forceinline__ void pass_cell_duo(const PassArgs& P, int b, unsigned
                             }
                         }
                         // candidates of both states over the whole stencil (wavefront-private)
-                        const MidFilter M = mid_filter(qix, qiy, qiz, dqx, dqy, dqz, FC);
+                        const MidFilter M = mid_filter_r(qix, qiy, qiz, dqx, dqy, dqz, FC, fr2);
                         int base = 0;
                         auto scan = [&](auto iso) {
                             constexpr int kG = 4;

@@ -117,11 +117,15 @@ struct alignas(16) MovePrep {
     uint32_t qx, qy, qz;        // position at the start of the chunk
     int32_t dqx, dqy, dqz;      // displacement, fixed point
     int32_t jsel;               // cell-local index of the particle
-    uint32_t flags;             // bit q: move q of the chunk draws the same particle; bit 16: capped trial term
+    uint32_t flags;             // bit q: move q of the chunk draws the same particle; bit 16: capped
+                                // trial term; bit 17: the trial position leaves the cell; bit 18:
+                                // it lies in the colloid core (tested when bit 17 is clear)
     Real ux, uy, uz;            // orientation at the start of the chunk
     Real nx, ny, nz;            // trial orientation from it
     Real vx, vy, vz;            // rotation draw
+    float fr2;                  // squared pre-filter radius (fixed-point units for a cubic box)
 };
+constexpr uint32_t kPrepLeft = 1u << 17, kPrepHard = 1u << 18;
 
 template <typename Real, int W>
 struct PassLds {
@@ -347,7 +351,18 @@ __device__ inline FilterConst filter_const(float sx, float sy, float sz, float r
     F.iso = sx == sy && sy == sz;
     return F;
 }
-// The hardware square root (<= 1 ulp) is covered by the (1 + 1e-6) factors.
+// The hardware square root (<= 1 ulp) is covered by the (1 + 1e-6) factors.  Per lane: the
+// squared radius r2 and (cubic box) the same in fixed-point units r2u, rounded up by one part
+// in 1e6 (conservative).
+__device__ inline void filter_radius2(int dqx, int dqy, int dqz, const FilterConst& F, float* r2,
+                                      float* r2u) {
+    const float hx = (float)dqx * F.sx, hy = (float)dqy * F.sy, hz = (float)dqz * F.sz;
+    const float rm = (F.rc_root + 0.5f * __builtin_amdgcn_sqrtf(hx * hx + hy * hy + hz * hz)) *
+                         (1.0f + 1e-6f) + 2.0f * (F.sx + F.sy + F.sz);
+    *r2 = rm * rm;
+    *r2u = rm * rm * F.inv_s2 * (1.0f + 1e-6f);
+}
+// The filter of a move (wavefront-uniform inputs; the radius held as a scalar).
 __device__ inline MidFilter mid_filter(uint32_t qix, uint32_t qiy, uint32_t qiz, int dqx, int dqy,
                                        int dqz, const FilterConst& F) {
     MidFilter M;
@@ -355,14 +370,53 @@ __device__ inline MidFilter mid_filter(uint32_t qix, uint32_t qiy, uint32_t qiz,
     M.my = qiy + (uint32_t)(dqy >> 1);
     M.mz = qiz + (uint32_t)(dqz >> 1);
     M.sx = F.sx; M.sy = F.sy; M.sz = F.sz;
-    const float hx = (float)dqx * F.sx, hy = (float)dqy * F.sy, hz = (float)dqz * F.sz;
-    const float rm = (F.rc_root + 0.5f * __builtin_amdgcn_sqrtf(hx * hx + hy * hy + hz * hz)) *
-                         (1.0f + 1e-6f) + 2.0f * (F.sx + F.sy + F.sz);
-    M.r2 = uni(rm * rm);
+    float r2, r2u;
+    filter_radius2(dqx, dqy, dqz, F, &r2, &r2u);
+    M.r2 = uni(r2);
     M.iso = F.iso;
-    // the radius in fixed-point units, rounded up by one part in 1e6 (conservative)
-    M.r2u = uni(rm * rm * F.inv_s2 * (1.0f + 1e-6f));
+    M.r2u = uni(r2u);
     return M;
+}
+
+// The same filter with its squared radius already known (r2u for a cubic box, else r2).
+__device__ inline MidFilter mid_filter_r(uint32_t qix, uint32_t qiy, uint32_t qiz, int dqx,
+                                         int dqy, int dqz, const FilterConst& F, float fr2) {
+    MidFilter M;
+    M.mx = qix + (uint32_t)(dqx >> 1);
+    M.my = qiy + (uint32_t)(dqy >> 1);
+    M.mz = qiz + (uint32_t)(dqz >> 1);
+    M.sx = F.sx; M.sy = F.sy; M.sz = F.sz;
+    M.r2 = fr2;
+    M.r2u = fr2;
+    M.iso = F.iso;
+    return M;
+}
+
+// A move's outcome tests from the particle's state (q) and displacement: kPrepLeft when the
+// trial position leaves the active cell (cx, cy, cz) of the grid (n, shift g), else kPrepHard
+// when it lies in the colloid core (cells within reach of the colloid: MCnptColloid.c:2516-2520);
+// *fr2 = the squared pre-filter radius of mid_filter (r2u for a cubic box, else r2).  Per lane
+// (the chunk's records are prepared one move per lane).
+__device__ inline uint32_t prep_tests(uint32_t qx, uint32_t qy, uint32_t qz, int dqx, int dqy,
+                                      int dqz, uint32_t gx, uint32_t gy, uint32_t gz, int nx,
+                                      int ny, int nz, int cx, int cy, int cz, bool cnear,
+                                      double R2, const PairGeom& X, const FilterConst& F,
+                                      float* fr2) {
+    const uint32_t qnx = qx + (uint32_t)dqx, qny = qy + (uint32_t)dqy, qnz = qz + (uint32_t)dqz;
+    float r2, r2u;
+    filter_radius2(dqx, dqy, dqz, F, &r2, &r2u);
+    *fr2 = F.iso ? r2u : r2;
+    if (cell_coord(qnx, gx, (uint32_t)nx) != (uint32_t)cx ||
+        cell_coord(qny, gy, (uint32_t)ny) != (uint32_t)cy ||
+        cell_coord(qnz, gz, (uint32_t)nz) != (uint32_t)cz)
+        return kPrepLeft;
+    if (cnear) {
+        const double rx = (double)(int32_t)(qnx - 0x80000000u) * X.s0;
+        const double ry = (double)(int32_t)(qny - 0x80000000u) * X.s1;
+        const double rz = (double)(int32_t)(qnz - 0x80000000u) * X.s2;
+        if (rx * rx + ry * ry + rz * rz <= R2) return kPrepHard;
+    }
+    return 0u;
 }
 
 // Flat stencil index e -> global particle slot (segments of the ns neighbour cells).
@@ -638,6 +692,9 @@ __device__ __forceinline__ void pass_cell(const PassArgs& P, int b, unsigned cha
                                             (double)mp.ny, (double)mp.nz, &cn);
                         mp.flags = cn ? 1u << 16 : 0u;
                     }
+                    // the move's outcome tests and pre-filter radius, from the same state
+                    mp.flags |= prep_tests(qj.x, qj.y, qj.z, d.dq[0], d.dq[1], d.dq[2], gx, gy, gz,
+                                           nx, ny, nz, cx, cy, cz, cnear, R2, X, FC, &mp.fr2);
                 }
             }
             if (w == 0) {
@@ -687,18 +744,24 @@ __device__ __forceinline__ void pass_cell(const PassArgs& P, int b, unsigned cha
                 int capd = 0, ccd = 0;
                 double dfin = 0.0, scale = 0.0;
                 ++n_att;
-                const double rnx = (double)(int32_t)(qnx - 0x80000000u) * X.s0;
-                const double rny = (double)(int32_t)(qny - 0x80000000u) * X.s1;
-                const double rnz = (double)(int32_t)(qnz - 0x80000000u) * X.s2;
-                if (cell_coord(qnx, gx, (uint32_t)nx) != (uint32_t)cx ||
-                    cell_coord(qny, gy, (uint32_t)ny) != (uint32_t)cy ||
-                    cell_coord(qnz, gz, (uint32_t)nz) != (uint32_t)cz) {
+                // the outcome tests and the pre-filter radius: from the chunk's record, or
+                // again from the current state
+                float fr2;
+                const uint32_t tests =
+                    fresh ? fl & (kPrepLeft | kPrepHard)
+                          : prep_tests(qix, qiy, qiz, dqx, dqy, dqz, gx, gy, gz, nx, ny, nz, cx, cy,
+                                       cz, cnear, R2, X, FC, &fr2);
+                if (fresh) fr2 = uni(pm.fr2);
+                if (tests & kPrepLeft) {
                     outcome = MCGB_MOVE_LEFT;       // moves are confined to the active cell
                     ++n_left;
-                } else if (cnear && rnx * rnx + rny * rny + rnz * rnz <= R2) {
+                } else if (tests & kPrepHard) {
                     outcome = MCGB_MOVE_HARD;       // MCnptColloid.c:2516-2520
                     ++n_hard;
                 } else {
+                    const double rnx = (double)(int32_t)(qnx - 0x80000000u) * X.s0;
+                    const double rny = (double)(int32_t)(qny - 0x80000000u) * X.s1;
+                    const double rnz = (double)(int32_t)(qnz - 0x80000000u) * X.s2;
                     Real unx, uny, unz;
                     if (fresh) {
                         unx = uni(pm.nx); uny = uni(pm.ny); unz = uni(pm.nz);
@@ -714,7 +777,7 @@ __device__ __forceinline__ void pass_cell(const PassArgs& P, int b, unsigned cha
                         // candidates of both states, wavefront-private compaction: a partner
                         // within the cutoff of either state lies within rc + |dq|/2 of their
                         // midpoint (triangle inequality), so one FP32 distance per candidate
-                        const MidFilter M = mid_filter(qix, qiy, qiz, dqx, dqy, dqz, FC);
+                        const MidFilter M = mid_filter_r(qix, qiy, qiz, dqx, dqy, dqz, FC, fr2);
                         // up to four wavefront-wide groups of 64 records per step (this
                         // wavefront's groups w, w + W, ...), their LDS loads issued before the
                         // tests; the padded tail holds sentinels the filter rejects

@@ -511,15 +511,17 @@ __device__ __forceinline__ void pass_cell_duo(const PassArgs& P, int b, unsigned
                         WavePartial part;
                         A.m.to_partial(part);
                         d_main = from63(part.d);
-                        c_main = from63(wave_sum_i63(A.cm));
+                        c_main = __builtin_amdgcn_ballot_w64(A.cm != 0) ? from63(wave_sum_i63(A.cm)) : 0;
                         if (TR) s_main = from63(wave_sum_d63((double)A.sm));
                         if (second) {
                             A.ro.to_partial(part);
                             d_old = from63(part.d);
                             A.rt.to_partial(part);
                             d_trial = from63(part.d);
-                            c_old = from63(wave_sum_i63(A.co));
-                            c_trial = from63(wave_sum_i63(A.ct));
+                            if (__builtin_amdgcn_ballot_w64((A.co | A.ct) != 0)) {
+                                c_old = from63(wave_sum_i63(A.co));
+                                c_trial = from63(wave_sum_i63(A.ct));
+                            }
                             p_old = from63(wave_sum_i63(A.po));
                             p_trial = from63(wave_sum_i63(A.pt));
                             if (TR) {

@@ -959,7 +959,9 @@ __device__ __forceinline__ void pass_cell(const PassArgs& P, int b, unsigned cha
                         // wavefront partials -> LDS
                         WavePartial part;
                         acc.to_partial(part);
-                        part.capd = wave_sum_i63(cpl);
+                        // capped pairs are rare once a start's overlaps are gone: their count is
+                        // reduced only when some lane holds one
+                        part.capd = __builtin_amdgcn_ballot_w64(cpl != 0) ? wave_sum_i63(cpl) : 0;
                         part.scale = tracing ? wave_sum_d63((double)scl) : 0.0;
                         if (lane == 63) sred[w] = part;
                         __syncthreads();

@@ -3,9 +3,10 @@
 // attempt_monte_carlo_move, :2445-2604).
 //
 // Per activation: the stencil ((2R+1)^3 cell segments of the device cell list, R = 1 for 8
-// sublattices, 2 for 27) is staged in LDS.  The moves run in chunks of 8: the chunk's draws
-// (written by k_draws) arrive from HBM one chunk ahead, and 8 lanes prepare one record per
-// move (draws, the particle's state, trial orientation, trial colloid term).  Per move: each
+// sublattices, 2 for 27) is staged in LDS.  The moves run in chunks of 9: the chunk's draws
+// (written by k_draws) arrive from HBM one chunk ahead, and 9 lanes prepare one record per
+// move (draws, the particle's state, trial orientation, trial colloid term, the leave-cell and
+// colloid-core tests, the pre-filter radius).  Per move: each
 // wavefront tests its share of the stencil against the old and trial positions (FP32
 // midpoint pre-filter), compacts the hits into a wavefront-private LDS list, evaluates both GB
 // energies of every listed pair (cutoff decided on the FP64 r^2) two items per lane at a time,
@@ -129,7 +130,7 @@ constexpr uint32_t kPrepLeft = 1u << 17, kPrepHard = 1u << 18;
 
 template <typename Real, int W>
 struct PassLds {
-    static constexpr int kDraws = 8;                 // moves whose draws are staged at a time
+    static constexpr int kDraws = 9;                 // moves whose draws are staged at a time (9 x 7 8-byte words fill one wavefront)
     __host__ __device__ static inline int list_cap(int cap) {
         return ((cap + 64 * W - 1) / (64 * W)) * 64;
     }

@@ -268,29 +268,55 @@ def da_leg(args, pairs_plain, waves, schedule):
         out["loss_first_last"] = [float(losses[:10].mean()), float(losses[-200:].mean())]
         q0, u0, L0 = ctx.get_state_fixed()
         res = {}
+        steps = [float(x) for x in args.da_steps.split(",")]
+        by_step = {"plain": {}, "da": {}}
+        da_w = waves
+        # every plain run first: the autotuner then tunes W for the delayed-acceptance kernel,
+        # which stays for the rest of this context
         for mode in ("plain", "da"):
-            ctx.set_state_fixed(q0, u0, L0)
-            ctx.da_enable(mode == "da")
-            if mode == "da":
-                # the delayed-acceptance kernel has its own register budget: tune W for it
-                # (every schedule runs that kernel while it is on), then restart from q0
-                da_w, _, _ = ctx.autotune(2)
+            for step in steps:
                 ctx.set_state_fixed(q0, u0, L0)
-                ctx.da_guard(50, 0.9, 0.02)      # falls back only for a broken network
-            ctx.sweep(2)                                   # graph capture outside the timing
-            dt, st, e, p2, _ = timed(ctx, args.da_sweeps, label=f"da leg, {mode}")
-            evaluated = st["attempted"] - st["rejected_left_cell"] - st["rejected_hard_core"]
-            r = {"moves_per_s": st["attempted"] / dt, "ms_per_sweep": 1e3 * dt / args.da_sweeps,
-                 "acceptance": st["accepted"] / max(1, evaluated),
-                 "E/N": series_report(e, dt), "P2": series_report(p2, dt),
-                 "waves_per_cell": da_w if mode == "da" else waves}
-            if mode == "da":
-                s1, s2 = st["da_stage1_rejected"], st["da_stage2_rejected"]
-                r["stage1_rejection"] = s1 / max(1, evaluated)
-                r["stage2_rejection"] = s2 / max(1, evaluated - s1)
-                r["exact_evaluations_skipped"] = s1 / max(1, evaluated)
-                r["guard"] = ctx.da_status()
-            res[mode] = r
+                ctx.da_enable(mode == "da")
+                if mode == "da" and step == steps[0]:
+                    # the delayed-acceptance kernel has its own register budget: tune W for it
+                    # (every schedule runs that kernel while it is on), then restart from q0
+                    da_w, _, _ = ctx.autotune(2)
+                    ctx.set_state_fixed(q0, u0, L0)
+                if mode == "da":
+                    ctx.da_guard(50, 0.9, 0.02)  # falls back only for a broken network
+                ctx.set_step_sizes(step, step)
+                ctx.sweep(2)                               # graph capture outside the timing
+                dt, st, e, p2, _ = timed(ctx, args.da_sweeps, label=f"da leg, {mode}, step {step}")
+                evaluated = st["attempted"] - st["rejected_left_cell"] - st["rejected_hard_core"]
+                r = {"step": step, "moves_per_s": st["attempted"] / dt,
+                     "ms_per_sweep": 1e3 * dt / args.da_sweeps,
+                     "acceptance": st["accepted"] / max(1, evaluated),
+                     "E/N": series_report(e, dt), "P2": series_report(p2, dt),
+                     "waves_per_cell": da_w if mode == "da" else waves}
+                if mode == "da":
+                    s1, s2 = st["da_stage1_rejected"], st["da_stage2_rejected"]
+                    r["stage1_rejection"] = s1 / max(1, evaluated)
+                    r["stage2_rejection"] = s2 / max(1, evaluated - s1)
+                    r["exact_evaluations_skipped"] = s1 / max(1, evaluated)
+                    r["guard"] = ctx.da_status()
+                by_step[mode][step] = r
+        ctx.set_step_sizes(steps[0], steps[0])
+
+        # each chain at its own best step size: the largest min(ESS/s E/N, ESS/s P2)
+        def score(r):
+            v = [r[o]["ess_per_s"] for o in ("E/N", "P2") if r[o] and r[o]["ess_per_s"]]
+            return min(v) if len(v) == 2 else 0.0
+        for mode in ("plain", "da"):
+            best = max(steps, key=lambda x: score(by_step[mode][x]))
+            res[mode] = dict(by_step[mode][best], best_of_steps=steps)
+        if len(steps) > 1:
+            out["by_step"] = {mode: {str(x): {"moves_per_s": by_step[mode][x]["moves_per_s"],
+                                              "acceptance": by_step[mode][x]["acceptance"],
+                                              "ess_per_s": {o: (by_step[mode][x][o] or {}).get("ess_per_s")
+                                                            for o in ("E/N", "P2")},
+                                              "tau_int": {o: (by_step[mode][x][o] or {}).get("tau_int")
+                                                          for o in ("E/N", "P2")}}
+                                      for x in steps} for mode in by_step}
         # the plain kernel as judge: exact stage for every move, surrogate error histogram
         ctx.set_state_fixed(q0, u0, L0)
         ctx.da_enable(True, judge=True)
@@ -485,6 +511,9 @@ def main():
     ap.add_argument("--ess-sweeps", type=int, default=20000)
     ap.add_argument("--da-equil", type=int, default=20000)
     ap.add_argument("--da-sweeps", type=int, default=8000)
+    ap.add_argument("--da-steps", default="0.1",
+                    help="comma-separated step sizes (translation = rotation) at which plain and "
+                         "delayed acceptance both run; each chain is reported at its best")
     ap.add_argument("--pairs", type=int, default=10 ** 7, help="training pairs for the surrogate")
     ap.add_argument("--train-steps", type=int, default=20000)
     ap.add_argument("--no-npt", action="store_true", help="skip the NPT leg")

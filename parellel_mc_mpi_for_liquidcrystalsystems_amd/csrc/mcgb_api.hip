@@ -113,7 +113,6 @@ struct mcgb_ctx {
     float* pairs = nullptr;            // emitted exact pair records (kPairRec floats each)
     long long pairs_cap = 0, pairs_n = 0;
     float *tr_grad = nullptr, *tr_m = nullptr, *tr_v = nullptr, *tr_partial = nullptr;
-    float* tr_scratch = nullptr;
     double *tr_lpart = nullptr, *tr_loss = nullptr;
     int tr_nblk = 0, adam_t = 0;
     // asynchronous snapshots: one staging record and one pinned host record per replica
@@ -1795,7 +1794,6 @@ int train_alloc(mcgb_ctx* c, int nblk) {
     if (nblk <= c->tr_nblk) return MCGB_OK;
     int rc;
     if ((rc = dalloc_t(c, &c->tr_partial, (size_t)nblk * kNParam))) return rc;
-    if ((rc = dalloc_t(c, &c->tr_scratch, (size_t)nblk * 2 * 4096))) return rc;
     if ((rc = dalloc_t(c, &c->tr_lpart, (size_t)nblk))) return rc;
     c->tr_nblk = nblk;
     return MCGB_OK;
@@ -1811,7 +1809,7 @@ int enqueue_grad(mcgb_ctx* c, const float* P, const float* data, long long ndata
                  const float* ts, int nexp, uint32_t step, uint32_t seed, float clip, float wfloor,
                  int nblk, float inv_n) {
     k_mlp_grad<<<nblk, 64, 0, c->st>>>(P, data, ndata, xs, ts, nexp, step, seed, 0x6D6C7000u, clip,
-                                       wfloor, inv_n, c->tr_scratch, c->tr_partial, c->tr_lpart);
+                                       wfloor, inv_n, c->tr_partial, c->tr_lpart);
     k_mlp_reduce<<<(kNParam + 255) / 256, 256, 0, c->st>>>(c->tr_partial, nblk, c->tr_lpart,
                                                           c->tr_grad, c->tr_loss);
     HIPCHK(hipGetLastError());

@@ -20,10 +20,10 @@
 // the k-slot (g, idx) of K step m carries hidden unit hidx(m / 2, 8 (m % 2) + idx, g), and the
 // host packs W2 into the matching A operand.  Only the k order changes, never the sum.
 //
-// Training (FP32, VALU): one workgroup of 64 samples computes forward and backward, the
-// per-sample intermediates meet in LDS and a per-workgroup scratch, and each thread reduces
-// one output row of every gradient; a second kernel sums the workgroup partials in a fixed
-// order; a third applies Adam.
+// Training (FP32): one single-wavefront workgroup of 64 samples computes forward and backward,
+// the three 64 x 64 x 64 products on the matrix cores (v_mfma_f32_32x32x2_f32), the rest on the
+// vector ALU with the per-sample intermediates in LDS; a second kernel sums the workgroup
+// partials in a fixed order; a third applies Adam.
 #pragma once
 
 #include "mcgb_device.hpp"
@@ -198,30 +198,52 @@ __global__ __launch_bounds__(64) void k_mlp_eval(const float* __restrict__ P,
 
 // ------------------------------------------------------------------ training kernels
 
-// Gradient of (1/n) sum w (y - min(t, clip))^2 over one workgroup's 64 samples, with
-// w = 1 (wfloor <= 0) or w = (wfloor + 1) / (wfloor + |min(t, clip)|) (relative weighting).
+// One 32 x 32 tile of a 64-deep product on the matrix cores, FP32 operands and accumulation
+// (v_mfma_f32_32x32x2_f32, 32 K steps of 2): sum_k a(i, k) b(k, j) for tile rows i and columns
+// j.  Operand layouts (lane l, g = l / 32): A (32 x 2) lane l holds a(l % 32, k0 + g); B (2 x 32)
+// lane l holds b(k0 + g, l % 32); C (32 x 32) lane l, element r holds row mfma_row(r, g),
+// column l % 32.
+template <typename FA, typename FB>
+__device__ inline floatx16 mm64_tile(int lane, FA a, FB b) {
+    const int i = lane & 31, g = lane >> 5;
+    floatx16 c = {};
+#pragma unroll 8
+    for (int k0 = 0; k0 < kHid; k0 += 2)
+        c = __builtin_amdgcn_mfma_f32_32x32x2f32(a(i, k0 + g), b(k0 + g, i), c, 0, 0, 0);
+    return c;
+}
+
+// Gradient of (1/n) sum w (y - min(t, clip))^2 over one workgroup's 64 samples (one wavefront),
+// with w = 1 (wfloor <= 0) or w = (wfloor + 1) / (wfloor + |min(t, clip)|) (relative weighting).
 //   data != nullptr: sample s of step `step` is record floor(u * ndata), u a Philox draw
 //   (key seed, counter {sample, step, 0, 0x5A5A0000}), mapped by a random element of the
 //   energy's symmetry group (u_i -> -u_i: (a, c) -> (-a, -c); u_j -> -u_j: (b, c) -> (-b, -c);
 //   i <-> j: (a, b) -> (-b, -a)); else samples are xs/ts[0, nexp) as given.
+// The three 64 x 64 x 64 products run on the matrix cores in FP32:
+//   Z2 = H1 W2^T (forward), gW2 = dZ2^T H1, dH1 = dZ2 W2 (backward);
+// layer 1 (4 inputs), the output layer and the per-row and per-column sums on the vector ALU,
+// thread t owning sample t for the row work and output row t of a gradient for the column work.
 // Writes kNParam partial sums to partial[blockIdx.x] and the loss sum to lpart[blockIdx.x].
-// scratch: per-workgroup 2 x 64 x 64 floats (h2 and dz1 rows).
 __global__ __launch_bounds__(64) void k_mlp_grad(const float* __restrict__ Pg,
                                                  const float* __restrict__ data, long long ndata,
                                                  const float* __restrict__ xs,
                                                  const float* __restrict__ ts, int nexp,
                                                  uint32_t step, uint32_t key0, uint32_t key1,
                                                  float clip, float wfloor, float inv_n,
-                                                 float* __restrict__ scratch,
                                                  float* __restrict__ partial,
                                                  double* __restrict__ lpart) {
-    __shared__ float P[kNParam + 3];
-    __shared__ float A[64][65];       // h1 rows
-    __shared__ float B[64][65];       // dz2 rows
-    __shared__ float X[64][4];
-    __shared__ float DY[64];
+    // rows padded to 65 floats: column walks and row walks both stay free of bank conflicts
+    __shared__ float W2[kHid][kHid + 1];      // W2[o][h]
+    __shared__ float Q[kOffW2 + 1];           // W1 (256) and b1 (64)
+    __shared__ float R[kHid * 2 + 1];         // b2 (64), W3 (64), b3
+    __shared__ float H1[kHid][kHid + 1];      // h1 rows [sample][h]; later dz1 rows
+    __shared__ float Z2[kHid][kHid + 1];      // z2 rows [sample][o]; later dz2 rows
+    __shared__ float X[kHid][4];
+    __shared__ float DY[kHid];
     const int t = threadIdx.x;
-    for (int p = t; p < kNParam; p += 64) P[p] = Pg[p];
+    for (int p = t; p < kHid * kHid; p += 64) W2[p >> 6][p & 63] = Pg[kOffW2 + p];
+    for (int p = t; p < kOffW2; p += 64) Q[p] = Pg[p];
+    for (int p = t; p < 2 * kHid + 1; p += 64) R[p] = Pg[kOffB2 + p];
     const size_t s = (size_t)blockIdx.x * 64 + t;
     float x[4] = {0.f, 0.f, 0.f, 0.f}, tg = 0.f;
     bool valid;
@@ -247,84 +269,100 @@ __global__ __launch_bounds__(64) void k_mlp_grad(const float* __restrict__ Pg,
     for (int k = 0; k < 4; ++k) X[t][k] = x[k];
     __syncthreads();
 
-    float* H2 = scratch + ((size_t)blockIdx.x * 2 + 0) * 4096;    // h2 of sample q at q*64 + o
-    float* DZ1 = scratch + ((size_t)blockIdx.x * 2 + 1) * 4096;   // dz1 of sample q at q*64 + h
-
-    // forward
-    float h1[kHid];
+    // layer 1 (thread t: sample t)
 #pragma unroll
     for (int h = 0; h < kHid; ++h) {
-        float z = P[kOffB1 + h];
+        float z = Q[kOffB1 + h];
 #pragma unroll
-        for (int k = 0; k < 4; ++k) z += P[kOffW1 + 4 * h + k] * x[k];
-        h1[h] = silu_f(z);
-        A[t][h] = h1[h];
+        for (int k = 0; k < 4; ++k) z += Q[kOffW1 + 4 * h + k] * x[k];
+        H1[t][h] = silu_f(z);
     }
-    float z2[kHid];
+    __syncthreads();
+    const int g = t >> 5, j = t & 31;
+    // layer 2 on the matrix cores: Z2[s][o] = sum_h H1[s][h] W2[o][h] + b2[o]
 #pragma unroll
-    for (int o = 0; o < kHid; ++o) {
-        float z = P[kOffB2 + o];
+    for (int mt = 0; mt < 2; ++mt)
 #pragma unroll
-        for (int h = 0; h < kHid; ++h) z += P[kOffW2 + kHid * o + h] * h1[h];
-        z2[o] = z;
-    }
-    float y = P[kOffB3];
+        for (int nt = 0; nt < 2; ++nt) {
+            const floatx16 c = mm64_tile(
+                t, [&](int i, int k) { return H1[32 * mt + i][k]; },
+                [&](int k, int i) { return W2[32 * nt + i][k]; });
 #pragma unroll
-    for (int o = 0; o < kHid; ++o) y += P[kOffW3 + o] * silu_f(z2[o]);
+            for (int r = 0; r < 16; ++r)
+                Z2[32 * mt + mfma_row(r, g)][32 * nt + j] = c[r] + R[32 * nt + j];
+        }
+    __syncthreads();
+    // output layer and loss (thread t: sample t)
+    float y = R[2 * kHid];
+#pragma unroll 8
+    for (int o = 0; o < kHid; ++o) y += R[kHid + o] * silu_f(Z2[t][o]);
     const float diff = y - tg;
     const float wt = wfloor > 0.0f ? (wfloor + 1.0f) / (wfloor + fabsf(tg)) : 1.0f;
     const float dy = valid ? 2.0f * wt * diff * inv_n : 0.0f;
     DY[t] = dy;
     const double loss = valid ? (double)wt * (double)diff * (double)diff : 0.0;
-    // backward to layer 2
-#pragma unroll
-    for (int o = 0; o < kHid; ++o) {
-        H2[t * 64 + o] = silu_f(z2[o]);
-        B[t][o] = dy * P[kOffW3 + o] * silu_grad(z2[o]);
-    }
-    // backward to layer 1 (dz2 from this thread's own LDS row)
-    float dh1[kHid];
-#pragma unroll
-    for (int h = 0; h < kHid; ++h) dh1[h] = 0.0f;
-    for (int o = 0; o < kHid; ++o) {
-        const float d = B[t][o];
-#pragma unroll
-        for (int h = 0; h < kHid; ++h) dh1[h] += P[kOffW2 + kHid * o + h] * d;
-    }
-#pragma unroll
-    for (int h = 0; h < kHid; ++h) {
-        float z = P[kOffB1 + h];
-#pragma unroll
-        for (int k = 0; k < 4; ++k) z += P[kOffW1 + 4 * h + k] * x[k];
-        DZ1[t * 64 + h] = dh1[h] * silu_grad(z);
-    }
-    __threadfence_block();
     __syncthreads();
-
-    // reductions over the 64 samples: thread t owns output row t of every matrix
-    float gw2[kHid];
-#pragma unroll
-    for (int h = 0; h < kHid; ++h) gw2[h] = 0.0f;
-    float gb2 = 0.f, gw3 = 0.f, gb1 = 0.f, gw1[4] = {0.f, 0.f, 0.f, 0.f};
-    for (int q = 0; q < 64; ++q) {
-        const float d2 = B[q][t];
-#pragma unroll
-        for (int h = 0; h < kHid; ++h) gw2[h] += d2 * A[q][h];
-        gb2 += d2;
-        gw3 += DY[q] * H2[q * 64 + t];
-        const float d1 = DZ1[q * 64 + t];
-        gb1 += d1;
-#pragma unroll
-        for (int k = 0; k < 4; ++k) gw1[k] += d1 * X[q][k];
-    }
+    // thread t: column o = t of Z2 -> gW3[o], gb2[o], and dz2 in place
     float* out = partial + (size_t)blockIdx.x * kNParam;
+    {
+        const float w3 = R[kHid + t];
+        float gw3 = 0.f, gb2 = 0.f;
+        for (int q = 0; q < 64; ++q) {
+            const float z = Z2[q][t];
+            gw3 += DY[q] * silu_f(z);
+            const float d2 = DY[q] * w3 * silu_grad(z);
+            gb2 += d2;
+            Z2[q][t] = d2;
+        }
+        out[kOffW3 + t] = gw3;
+        out[kOffB2 + t] = gb2;
+    }
+    __syncthreads();
+    // gW2[o][h] = sum_s dZ2[s][o] H1[s][h] on the matrix cores, written from the C tiles
 #pragma unroll
-    for (int h = 0; h < kHid; ++h) out[kOffW2 + kHid * t + h] = gw2[h];
-    out[kOffB2 + t] = gb2;
-    out[kOffW3 + t] = gw3;
-    out[kOffB1 + t] = gb1;
+    for (int mt = 0; mt < 2; ++mt)
 #pragma unroll
-    for (int k = 0; k < 4; ++k) out[kOffW1 + 4 * t + k] = gw1[k];
+        for (int nt = 0; nt < 2; ++nt) {
+            const floatx16 c = mm64_tile(
+                t, [&](int i, int k) { return Z2[k][32 * mt + i]; },
+                [&](int k, int i) { return H1[k][32 * nt + i]; });
+#pragma unroll
+            for (int r = 0; r < 16; ++r)
+                out[kOffW2 + kHid * (32 * mt + mfma_row(r, g)) + 32 * nt + j] = c[r];
+        }
+    __syncthreads();
+    // dH1[s][h] = sum_o dZ2[s][o] W2[o][h] on the matrix cores; dz1 = dh1 silu'(z1) into H1
+#pragma unroll
+    for (int mt = 0; mt < 2; ++mt)
+#pragma unroll
+        for (int nt = 0; nt < 2; ++nt) {
+            const floatx16 c = mm64_tile(
+                t, [&](int i, int k) { return Z2[32 * mt + i][k]; },
+                [&](int k, int i) { return W2[k][32 * nt + i]; });
+            const int h = 32 * nt + j;
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int sr = 32 * mt + mfma_row(r, g);
+                float z = Q[kOffB1 + h];
+#pragma unroll
+                for (int k = 0; k < 4; ++k) z += Q[kOffW1 + 4 * h + k] * X[sr][k];
+                H1[sr][h] = c[r] * silu_grad(z);
+            }
+        }
+    __syncthreads();
+    // thread t: hidden unit h = t of layer 1 -> gb1[h], gW1[h][k]
+    {
+        float gb1 = 0.f, gw1[4] = {0.f, 0.f, 0.f, 0.f};
+        for (int q = 0; q < 64; ++q) {
+            const float d1 = H1[q][t];
+            gb1 += d1;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) gw1[k] += d1 * X[q][k];
+        }
+        out[kOffB1 + t] = gb1;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) out[kOffW1 + 4 * t + k] = gw1[k];
+    }
     // db3: fixed-order sum by thread 0
     if (t == 0) {
         float gb3 = 0.f;

@@ -80,7 +80,9 @@ struct mcgb_ctx {
     int cur = 0;
     int *cell_id = nullptr, *tmp = nullptr, *count = nullptr, *cell_start = nullptr;
     int *cell_start2 = nullptr, *list2 = nullptr;
-    int* tsum = nullptr;                 // [nrep][scan_tiles] tile sums of the count scan
+    unsigned long long* scan_st = nullptr;   // [nrep][scan_tiles] look-back status of the scan
+    int* slot = nullptr;                 // each particle's place within its cell (rebuild)
+    int* end_ticket = nullptr;           // [nrep] blocks of k_end done this sweep
     EndPartial* endp = nullptr;          // [nrep][kEndParts] partial sweep totals
     int scan_tiles = 1;
     CellAcc* acc = nullptr;
@@ -236,22 +238,27 @@ int check_params(const mcgb_params& p, int r) {
 
 // Exclusive scan of the cell counts (three launches over tiles of kScanTile cells).
 void enqueue_scan(mcgb_ctx* c, int* cell_start, int trial) {
-    const dim3 gt(c->scan_tiles, c->nrep);
-    k_scan_tiles<<<gt, 256, 0, c->st>>>(c->dC, c->dD, c->count, c->tsum, c->scan_tiles, trial);
-    k_scan_top<<<c->nrep, 64, 0, c->st>>>(c->dD, c->tsum, c->scan_tiles, trial);
-    k_scan_apply<<<gt, 256, 0, c->st>>>(c->dC, c->dD, c->count, c->tsum, c->scan_tiles, cell_start,
-                                        trial);
+    k_scan<<<dim3(c->scan_tiles, c->nrep), 256, 0, c->st>>>(c->dC, c->dD, c->count, c->scan_st,
+                                                            c->scan_tiles, cell_start, trial);
 }
 
+// Cell-list rebuild of the current buffer into the other one.  draws: the sweep's move draws
+// are made in the counting launch (k_count_draws).
 template <typename Real>
-int enqueue_rebuild(mcgb_ctx* c) {
+int enqueue_rebuild(mcgb_ctx* c, bool draws = false) {
     const int src = c->cur, dst = 1 - c->cur;
     const dim3 gp((c->Nmax + 255) / 256, c->nrep);
     // counts start at zero: set once at creation, reset by k_sort_gather after every rebuild
-    k_count<<<gp, 256, 0, c->st>>>(c->dC, c->dD, c->q[src][0], c->q[src][1], c->q[src][2],
-                                   c->cell_id, c->count, 0);
+    if (draws)
+        k_count_draws<<<dim3((std::max(c->rec_max, c->Nmax) + 255) / 256, c->nrep), 256, 0, c->st>>>(
+            c->dC, c->dD, c->draws, c->q[src][0], c->q[src][1], c->q[src][2], c->cell_id, c->slot,
+            c->count);
+    else
+        k_count<<<gp, 256, 0, c->st>>>(c->dC, c->dD, c->q[src][0], c->q[src][1], c->q[src][2],
+                                       c->cell_id, c->slot, c->count, 0);
     enqueue_scan(c, c->cell_start, 0);
-    k_scatter<<<gp, 256, 0, c->st>>>(c->dC, c->dD, c->cell_id, c->count, c->cell_start, c->tmp, 0);
+    k_scatter<<<gp, 256, 0, c->st>>>(c->dC, c->dD, c->cell_id, c->slot, c->cell_start, c->tmp,
+                                     c->scan_st, c->scan_tiles, 0);
     const dim3 gs((c->ncell_max + 3) / 4, c->nrep);
     k_sort_gather<Real><<<gs, 256, 0, c->st>>>(
         c->dC, c->dD, c->cell_start, c->tmp, c->pid[src], 0, 0, nullptr, c->count, c->q[src][0], c->q[src][1],
@@ -265,7 +272,7 @@ int enqueue_rebuild(mcgb_ctx* c) {
 
 // FP64 energy of the current cell list; mode as in k_energy_reduce.
 template <typename Real>
-int enqueue_energy(mcgb_ctx* c, int mode) {
+int enqueue_energy(mcgb_ctx* c, int mode, bool record = false) {
     const int b = c->cur;
     const dim3 ge((c->Nmax + 127) / 128, c->nrep);
     const int* cs = mode == 2 ? c->cell_start2 : c->cell_start;
@@ -273,7 +280,8 @@ int enqueue_energy(mcgb_ctx* c, int mode) {
                                           (const Real*)c->u[b][0], (const Real*)c->u[b][1],
                                           (const Real*)c->u[b][2], cs, c->list2, c->efin, c->ecap,
                                           c->eccap, c->ehard);
-    k_energy_reduce<<<c->nrep, 256, 0, c->st>>>(c->dC, c->dD, mode, c->efin, c->ecap, c->eccap, c->ehard);
+    k_energy_reduce<<<c->nrep, 256, 0, c->st>>>(c->dC, c->dD, mode, c->efin, c->ecap, c->eccap, c->ehard,
+                                                record ? c->series : nullptr, c->series_cap);
     HIPCHK(hipGetLastError());
     return MCGB_OK;
 }
@@ -283,9 +291,11 @@ int enqueue_npt(mcgb_ctx* c) {
     const int b = c->cur;
     k_npt_trial<<<(c->nrep + 63) / 64, 64, 0, c->st>>>(c->dC, c->dD, c->nrep);
     const dim3 gp((c->Nmax + 255) / 256, c->nrep);
-    k_count<<<gp, 256, 0, c->st>>>(c->dC, c->dD, c->q[b][0], c->q[b][1], c->q[b][2], c->cell_id, c->count, 1);
+    k_count<<<gp, 256, 0, c->st>>>(c->dC, c->dD, c->q[b][0], c->q[b][1], c->q[b][2], c->cell_id,
+                                   c->slot, c->count, 1);
     enqueue_scan(c, c->cell_start2, 1);
-    k_scatter<<<gp, 256, 0, c->st>>>(c->dC, c->dD, c->cell_id, c->count, c->cell_start2, c->tmp, 1);
+    k_scatter<<<gp, 256, 0, c->st>>>(c->dC, c->dD, c->cell_id, c->slot, c->cell_start2, c->tmp,
+                                     c->scan_st, c->scan_tiles, 1);
     const dim3 gs((c->ncell_max + 3) / 4, c->nrep);
     k_sort_gather<Real><<<gs, 256, 0, c->st>>>(
         c->dC, c->dD, c->cell_start2, c->tmp, c->pid[b], 1, 1, c->list2, c->count, nullptr, nullptr, nullptr,
@@ -294,14 +304,13 @@ int enqueue_npt(mcgb_ctx* c) {
     return enqueue_energy<Real>(c, 2);
 }
 
+// Orientation products of the current state, per block (mcgb_order_parameter).
 template <typename Real>
-int enqueue_observe(mcgb_ctx* c, int advance, int record) {
+int enqueue_observe(mcgb_ctx* c) {
     const int b = c->cur;
     k_observe<Real><<<dim3(64, c->nrep), 256, 0, c->st>>>(c->dC, (const Real*)c->u[b][0],
                                                           (const Real*)c->u[b][1],
                                                           (const Real*)c->u[b][2], c->obs);
-    k_observe_reduce<<<(c->nrep + 63) / 64, 64, 0, c->st>>>(c->dC, c->dD, c->nrep, c->obs, c->series,
-                                                            record ? c->series_cap : 0, advance);
     HIPCHK(hipGetLastError());
     return MCGB_OK;
 }
@@ -393,8 +402,7 @@ int enqueue_one(mcgb_ctx* c, mcgb_move_trace* trace, const hipEvent_t* ev) {
     if (ev) HIPCHK(hipEventRecord(ev[0], c->st));
     k_begin<<<(c->nrep + 63) / 64, 64, 0, c->st>>>(c->dC, c->dD, c->nrep, c->resum_interval,
                                                   c->ovf_count);
-    k_draws<<<dim3((c->rec_max + 255) / 256, c->nrep), 256, 0, c->st>>>(c->dC, c->dD, c->draws);
-    if ((rc = enqueue_rebuild<Real>(c))) return rc;
+    if ((rc = enqueue_rebuild<Real>(c, true))) return rc;
     if (ev) HIPCHK(hipEventRecord(ev[1], c->st));
     PassArgs P;
     P.RC = c->dC; P.RD = c->dD; P.blk_rep = c->dBlk;
@@ -419,11 +427,17 @@ int enqueue_one(mcgb_ctx* c, mcgb_move_trace* trace, const hipEvent_t* ev) {
         if (ev) HIPCHK(hipEventRecord(ev[2 + p], c->st));
     }
     HIPCHK(hipGetLastError());
-    k_end_partial<<<dim3(kEndParts, c->nrep), 256, 0, c->st>>>(c->dC, c->dD, c->acc, c->endp);
-    k_end_moves<<<c->nrep, 64, 0, c->st>>>(c->dC, c->dD, c->endp);
-    if ((rc = enqueue_energy<Real>(c, 1))) return rc;
-    if (c->any_npt && (rc = enqueue_npt<Real>(c))) return rc;
-    if ((rc = enqueue_observe<Real>(c, 1, 1))) return rc;
+    k_end<Real><<<dim3(kEndParts, c->nrep), 256, 0, c->st>>>(
+        c->dC, c->dD, c->acc, (const Real*)c->u[b][0], (const Real*)c->u[b][1],
+        (const Real*)c->u[b][2], c->endp, c->end_ticket);
+    // the re-summation (when flagged), the NPT volume move, the sweep's record: the record goes
+    // with the energy reduction when no volume move follows
+    if ((rc = enqueue_energy<Real>(c, 1, !c->any_npt))) return rc;
+    if (c->any_npt) {
+        if ((rc = enqueue_npt<Real>(c))) return rc;
+        k_record<<<(c->nrep + 63) / 64, 64, 0, c->st>>>(c->dC, c->dD, c->nrep, c->series,
+                                                        c->series_cap, 1);
+    }
     if (ev) HIPCHK(hipEventRecord(ev[2 + c->npass], c->st));
     return MCGB_OK;
 }
@@ -712,7 +726,9 @@ int mcgb_create(const mcgb_params* p, int n_replicas, mcgb_ctx** out) {
     if (!rc) rc = dalloc_t(c, &c->cell_start, c->ncs_tot);
     if (!rc) rc = dalloc_t(c, &c->cell_start2, c->ncs_tot);
     c->scan_tiles = (c->ncell_max + kScanTile - 1) / kScanTile;
-    if (!rc) rc = dalloc_t(c, &c->tsum, (size_t)c->nrep * c->scan_tiles);
+    if (!rc) rc = dalloc_t(c, &c->scan_st, (size_t)c->nrep * c->scan_tiles);
+    if (!rc) rc = dalloc_t(c, &c->slot, N);
+    if (!rc) rc = dalloc_t(c, &c->end_ticket, c->nrep);
     if (!rc) rc = dalloc_t(c, &c->endp, (size_t)c->nrep * kEndParts);
     if (!rc) rc = dalloc_t(c, &c->acc, c->ncell_tot);
     if (!rc) rc = dalloc_t(c, &c->draws, (size_t)c->Ntot + c->ncell_tot);
@@ -729,7 +745,11 @@ int mcgb_create(const mcgb_params* p, int n_replicas, mcgb_ctx** out) {
     if (!rc) rc = dalloc_t(c, &c->dBlk, c->nblk);
     if (rc) { mcgb_destroy(c); return rc; }
     // the cell counts are zero between rebuilds (each rebuild's k_sort_gather resets them)
-    if (hipMemset(c->count, 0, sizeof(int) * (size_t)c->ncell_tot) != hipSuccess) {
+    // so are the scan's status words (cleared by k_scatter) and the k_end tickets (reset by the
+    // last block)
+    if (hipMemset(c->count, 0, sizeof(int) * (size_t)c->ncell_tot) != hipSuccess ||
+        hipMemset(c->scan_st, 0, sizeof(unsigned long long) * (size_t)c->nrep * c->scan_tiles) != hipSuccess ||
+        hipMemset(c->end_ticket, 0, sizeof(int) * (size_t)c->nrep) != hipSuccess) {
         mcgb_set_error("hipMemset of the cell counts failed");
         mcgb_destroy(c);
         return MCGB_ERR_HIP;
@@ -1382,7 +1402,7 @@ int mcgb_resum_energy(mcgb_ctx* c, int r, double* drift) {
 int mcgb_order_parameter(mcgb_ctx* c, int r, double* p2, double director[3]) {
     int rc = check_ctx(c, r);
     if (rc) return rc;
-    rc = c->prec == 64 ? enqueue_observe<double>(c, 0, 0) : enqueue_observe<float>(c, 0, 0);
+    rc = c->prec == 64 ? enqueue_observe<double>(c) : enqueue_observe<float>(c);
     if (rc) return rc;
     std::vector<double> part((size_t)c->nrep * 64 * 6);
     HIPCHK(hipMemcpyAsync(part.data(), c->obs, part.size() * 8, hipMemcpyDeviceToHost, c->st));

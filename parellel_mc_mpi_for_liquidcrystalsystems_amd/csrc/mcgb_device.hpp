@@ -82,6 +82,8 @@ struct RepDyn {
     unsigned long long adapts;   // step-size adaptations made
     double max_drift_forced;     // largest |running - recomputed| at a forced re-summation
     unsigned long long resums_forced;   // re-summations forced by a sweep's moved energy
+    double qsum[6];              // sum over particles of u_a u_b (xx, yy, zz, xy, xz, yz) after
+                                 // the passes of the last sweep (order-parameter record)
 };
 
 // Draws of one trial move (k_draws): particle selector, fixed-point step, rotation vector,

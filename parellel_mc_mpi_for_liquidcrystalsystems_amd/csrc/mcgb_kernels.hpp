@@ -1,9 +1,10 @@
 // mcgb_kernels.hpp — HIP kernels of the checkerboard Gay-Berne sweep (gfx950, wave64).
 //
 // One sweep of every replica is the launch sequence
-//   k_begin -> k_count -> k_scan_* -> k_scatter -> k_sort_gather   (cell-list rebuild)
-//   -> k_pass x P (sublattice passes, P = 8 or 27) -> k_end_moves -> k_energy/k_energy_reduce (re-sum, only
-//   when flagged) -> [NPT: trial cell list, k_energy, k_volume] -> k_observe -> k_observe_reduce.
+//   k_begin -> k_count_draws -> k_scan -> k_scatter -> k_sort_gather   (draws, cell-list rebuild)
+//   -> k_pass x P (sublattice passes, P = 8 or 27) -> k_end (sweep totals, orientation sums)
+//   -> k_energy/k_energy_reduce (re-sum, only when flagged; the sweep's record)
+//   [NPT: -> trial cell list, k_energy, volume test -> k_record].
 // Nothing in it synchronises with the host.
 #pragma once
 
@@ -178,109 +179,82 @@ __device__ inline int wave_cell_add(int* __restrict__ ctr, int c, bool valid) {
     return slot;
 }
 
-__global__ void k_count(const RepConst* __restrict__ RC, const RepDyn* __restrict__ RD,
-                        const uint32_t* __restrict__ qx, const uint32_t* __restrict__ qy,
-                        const uint32_t* __restrict__ qz, int* __restrict__ cell_id,
-                        int* __restrict__ count, int trial) {
-    const int r = blockIdx.y;
-    const RepConst& C = RC[r];
-    const RepDyn& D = RD[r];
-    if (trial && !D.trial_ok) return;                  // uniform: the whole grid row
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+// Cell and slot of particle i of replica r (every lane of the wavefront calls it: the counting
+// is a wavefront-wide operation; valid = i < N).  The slot is the particle's place among the
+// members of its cell, so the scatter after the scan needs no atomics.
+__device__ inline void count_particle(const RepConst& C, const RepDyn& D, int i,
+                                      const uint32_t* __restrict__ qx, const uint32_t* __restrict__ qy,
+                                      const uint32_t* __restrict__ qz, int* __restrict__ cell_id,
+                                      int* __restrict__ slot, int* __restrict__ count, int trial) {
     const bool valid = i < C.N;
     int c = 0;
+    const int gi = C.part_off + i;
     if (valid) {
-        const int gi = C.part_off + i;
         const uint32_t nx = trial ? D.nt[0] : D.n[0], ny = trial ? D.nt[1] : D.n[1],
                        nz = trial ? D.nt[2] : D.n[2];
         const uint32_t gx = trial ? 0u : D.g[0], gy = trial ? 0u : D.g[1], gz = trial ? 0u : D.g[2];
         const uint32_t cx = cell_coord(qx[gi], gx, nx), cy = cell_coord(qy[gi], gy, ny),
                        cz = cell_coord(qz[gi], gz, nz);
         c = (int)(cx + nx * (cy + ny * cz));
-        cell_id[gi] = c;
     }
-    wave_cell_add(count + C.cell_off, c, valid);
+    const int s = wave_cell_add(count + C.cell_off, c, valid);
+    if (valid) {
+        cell_id[gi] = c;
+        slot[gi] = s;
+    }
 }
 
-// Exclusive scan of the counts into cell_start in three steps over tiles of kScanTile cells:
-// tile sums (k_scan_tiles), an exclusive scan of the tile sums per replica (k_scan_top), and
-// the scan inside every tile from its base (k_scan_apply), which also zeroes the counts for
-// reuse as fill pointers.  trial != 0: the NPT trial grid.
-constexpr int kScanTile = 1024;
+__global__ void k_count(const RepConst* __restrict__ RC, const RepDyn* __restrict__ RD,
+                        const uint32_t* __restrict__ qx, const uint32_t* __restrict__ qy,
+                        const uint32_t* __restrict__ qz, int* __restrict__ cell_id,
+                        int* __restrict__ slot, int* __restrict__ count, int trial) {
+    const int r = blockIdx.y;
+    const RepConst& C = RC[r];
+    const RepDyn& D = RD[r];
+    if (trial && !D.trial_ok) return;                  // uniform: the whole grid row
+    count_particle(C, D, blockIdx.x * blockDim.x + threadIdx.x, qx, qy, qz, cell_id, slot, count,
+                   trial);
+}
+
+constexpr int kScanTile = 1024;    // cells per tile of the count scan
 
 __device__ inline int scan_ncell(const RepDyn& D, int trial) {
     return trial ? D.nt[0] * D.nt[1] * D.nt[2] : D.ncell;
 }
 
-__global__ __launch_bounds__(256) void k_scan_tiles(const RepConst* __restrict__ RC,
-                                                    const RepDyn* __restrict__ RD,
-                                                    const int* __restrict__ count,
-                                                    int* __restrict__ tsum, int tmax, int trial) {
+// Exclusive scan of the counts into cell_start in one launch over tiles of kScanTile cells,
+// with decoupled look-back (Merrill & Garland, "Single-pass parallel prefix scan with decoupled
+// look-back", 2016): tile t publishes its aggregate, then its inclusive prefix once it has
+// looked back through the published words of tiles t-1, t-2, ... (lower tiles are dispatched
+// first, so every wait ends).  A status word is (kind << 32) | value, kind 1 = aggregate,
+// 2 = inclusive prefix, 0 = not yet published; k_scatter, which runs after every scan, clears
+// them for the next one.  The counts are zeroed for reuse as fill pointers; trial != 0: the NPT
+// trial grid.
+__global__ __launch_bounds__(256) void k_scan(const RepConst* __restrict__ RC,
+                                              const RepDyn* __restrict__ RD,
+                                              int* __restrict__ count,
+                                              unsigned long long* __restrict__ status, int tmax,
+                                              int* __restrict__ cell_start, int trial) {
     __shared__ int ws[4];
-    const int r = blockIdx.y;
-    const RepDyn& D = RD[r];
-    if (trial && !D.trial_ok) return;
-    const int ncell = scan_ncell(D, trial);
-    const int t = blockIdx.x;
-    if (t * kScanTile >= ncell) return;                     // uniform over the block
-    const int* cnt = count + RC[r].cell_off;
-    int s = 0;
-    for (int j = 0; j < 4; ++j) {
-        const int c = t * kScanTile + threadIdx.x * 4 + j;
-        s += c < ncell ? cnt[c] : 0;
-    }
-    s = wave_sum(s);
-    if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = s;
-    __syncthreads();
-    if (threadIdx.x == 0) tsum[(size_t)r * tmax + t] = ws[0] + ws[1] + ws[2] + ws[3];
-}
-
-__global__ __launch_bounds__(64) void k_scan_top(const RepDyn* __restrict__ RD,
-                                                 int* __restrict__ tsum, int tmax, int trial) {
-    const int r = blockIdx.x;
-    const RepDyn& D = RD[r];
-    if (trial && !D.trial_ok) return;
-    const int nt = (scan_ncell(D, trial) + kScanTile - 1) / kScanTile;
-    int* ts = tsum + (size_t)r * tmax;
-    const int lane = threadIdx.x;
-    int carry = 0;
-    for (int t0 = 0; t0 < nt; t0 += 64) {                   // 64 tile sums per step
-        const int t = t0 + lane;
-        const int v = t < nt ? ts[t] : 0;
-        int incl = v;
-#pragma unroll
-        for (int off = 1; off < 64; off <<= 1) {
-            const int u = __shfl_up(incl, off);
-            if (lane >= off) incl += u;
-        }
-        if (t < nt) ts[t] = carry + incl - v;
-        carry += __shfl(incl, 63);
-    }
-}
-
-__global__ __launch_bounds__(256) void k_scan_apply(const RepConst* __restrict__ RC,
-                                                    const RepDyn* __restrict__ RD,
-                                                    int* __restrict__ count,
-                                                    const int* __restrict__ tsum, int tmax,
-                                                    int* __restrict__ cell_start, int trial) {
-    __shared__ int ws[4];
+    __shared__ int s_excl;
     const int r = blockIdx.y;
     const RepConst& C = RC[r];
     const RepDyn& D = RD[r];
     if (trial && !D.trial_ok) return;
     const int ncell = scan_ncell(D, trial);
     const int t = blockIdx.x;
-    if (t * kScanTile >= ncell) return;
+    if (t * kScanTile >= ncell) return;                     // uniform over the block
     int* cnt = count + C.cell_off;
     int* cs = cell_start + C.cs_off;
+    unsigned long long* st = status + (size_t)r * tmax;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int c0 = t * kScanTile + threadIdx.x * 4;
-    int v[4], s = 0;
+    int v[4], sum = 0;
     for (int j = 0; j < 4; ++j) {
         v[j] = c0 + j < ncell ? cnt[c0 + j] : 0;
-        s += v[j];
+        sum += v[j];
     }
-    int incl = s;
+    int incl = sum;
 #pragma unroll
     for (int off = 1; off < 64; off <<= 1) {
         const int u = __shfl_up(incl, off);
@@ -288,7 +262,31 @@ __global__ __launch_bounds__(256) void k_scan_apply(const RepConst* __restrict__
     }
     if (lane == 63) ws[w] = incl;
     __syncthreads();
-    int run = tsum[(size_t)r * tmax + t] + incl - s;
+    if (threadIdx.x == 0) {
+        const int agg = ws[0] + ws[1] + ws[2] + ws[3];
+        int excl = 0;
+        if (t == 0) {
+            __hip_atomic_store(&st[0], (2ull << 32) | (unsigned)agg, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            __hip_atomic_store(&st[t], (1ull << 32) | (unsigned)agg, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+            for (int j = t - 1; j >= 0;) {
+                const unsigned long long x =
+                    __hip_atomic_load(&st[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                const unsigned kind = (unsigned)(x >> 32);
+                if (kind == 0u) continue;                   // not yet published: poll again
+                excl += (int)(unsigned)x;
+                if (kind == 2u) break;
+                --j;
+            }
+            __hip_atomic_store(&st[t], (2ull << 32) | (unsigned)(excl + agg), __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+        }
+        s_excl = excl;
+    }
+    __syncthreads();
+    int run = s_excl + incl - sum;
     for (int q = 0; q < w; ++q) run += ws[q];
     for (int j = 0; j < 4; ++j) {
         if (c0 + j < ncell) {
@@ -300,18 +298,21 @@ __global__ __launch_bounds__(256) void k_scan_apply(const RepConst* __restrict__
     if (t == 0 && threadIdx.x == 0) cs[ncell] = C.N;
 }
 
-// Unordered scatter of particle slots into their cell segments.
+// Scatter of particle slots into their cell segments (the slot within the cell came with the
+// count); block 0 of each replica row clears the scan's status words for the next scan.
 __global__ void k_scatter(const RepConst* __restrict__ RC, const RepDyn* __restrict__ RD,
-                          const int* __restrict__ cell_id, int* __restrict__ fill,
-                          const int* __restrict__ cell_start, int* __restrict__ tmp, int trial) {
+                          const int* __restrict__ cell_id, const int* __restrict__ slot,
+                          const int* __restrict__ cell_start, int* __restrict__ tmp,
+                          unsigned long long* __restrict__ status, int tmax, int trial) {
     const int r = blockIdx.y;
     const RepConst& C = RC[r];
     if (trial && !RD[r].trial_ok) return;              // uniform: the whole grid row
+    if (blockIdx.x == 0)
+        for (int t = threadIdx.x; t < tmax; t += blockDim.x) status[(size_t)r * tmax + t] = 0ull;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool valid = i < C.N;
-    const int c = valid ? cell_id[C.part_off + i] : 0;
-    const int slot = wave_cell_add(fill + C.cell_off, c, valid);
-    if (valid) tmp[C.part_off + cell_start[C.cs_off + c] + slot] = i;
+    if (i >= C.N) return;
+    const int gi = C.part_off + i;
+    tmp[C.part_off + cell_start[C.cs_off + cell_id[gi]] + slot[gi]] = i;
 }
 
 // Orders every cell segment by particle id (a deterministic cell list: dist_batch_to_cells,
@@ -370,16 +371,11 @@ __global__ __launch_bounds__(256) void k_sort_gather(
 // (MCnptColloid.c:2502-2509), rotation vector on the unit sphere (:2526-2550) and the
 // Metropolis threshold -log(U) T (:2566-2574), from A = philox({m, cell, t, p<<8}) and
 // B = philox({m, cell, t, (p<<8)|1}).
-__global__ __launch_bounds__(256) void k_draws(const RepConst* __restrict__ RC,
-                                               const RepDyn* __restrict__ RD,
-                                               MoveDraw* __restrict__ rec) {
-    const int r = blockIdx.y;
-    const RepConst& C = RC[r];
-    const RepDyn& D = RD[r];
+__device__ inline void draw_move(const RepConst& C, const RepDyn& D, MoveDraw* __restrict__ rec,
+                                 int idx) {
     const int nact = D.nact, k = D.k;
     const int col = C.col;
     const int total = col * col * col * nact * k;
-    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= total) return;
     const int p = idx / (nact * k);
     const int rem = idx - p * nact * k;
@@ -412,27 +408,61 @@ __global__ __launch_bounds__(256) void k_draws(const RepConst* __restrict__ RC,
     rec[C.rec_off + idx] = d;
 }
 
+__global__ __launch_bounds__(256) void k_draws(const RepConst* __restrict__ RC,
+                                               const RepDyn* __restrict__ RD,
+                                               MoveDraw* __restrict__ rec) {
+    const int r = blockIdx.y;
+    draw_move(RC[r], RD[r], rec, blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+// The sweep's move draws and the cell counts of the rebuild in one launch (both follow k_begin
+// and neither reads the other's output): thread i draws move i and counts particle i.
+__global__ __launch_bounds__(256) void k_count_draws(const RepConst* __restrict__ RC,
+                                                     const RepDyn* __restrict__ RD,
+                                                     MoveDraw* __restrict__ rec,
+                                                     const uint32_t* __restrict__ qx,
+                                                     const uint32_t* __restrict__ qy,
+                                                     const uint32_t* __restrict__ qz,
+                                                     int* __restrict__ cell_id, int* __restrict__ slot,
+                                                     int* __restrict__ count) {
+    const int r = blockIdx.y;
+    const RepConst& C = RC[r];
+    const RepDyn& D = RD[r];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (blockIdx.x * blockDim.x < C.N)                 // uniform: wavefronts holding particles
+        count_particle(C, D, i, qx, qy, qz, cell_id, slot, count, 0);
+    draw_move(C, D, rec, i);
+}
+
 // ------------------------------------------------------------------ end of the passes
 
-// Folds the per-cell accumulators of the sweep into the running energy and counters (fixed
-// reduction order: deterministic) and flags an FP64 re-summation when needed.
-// Per-cell accumulators of a sweep folded in two levels with a fixed partition and order
-// (deterministic): kEndParts chunks of cells per replica (k_end_partial), then the chunks.
+// End of the passes, one launch (k_end): block b of replica r folds the per-cell accumulators
+// of cell chunk b and the orientation products u_a u_b of particles b*256 + tid + j*64*256 into
+// a partial; the last block of the replica to finish (a ticket) folds the kEndParts partials
+// in a fixed order (deterministic) into the running energy and counters, runs the step-size
+// controller, flags an FP64 re-summation when needed and keeps the orientation sums for the
+// sweep's record.
 constexpr int kEndParts = 64;
 struct EndPartial {
     long long v[11];
     double f, mv;
+    double q[6];
 };
 
-__global__ __launch_bounds__(256) void k_end_partial(const RepConst* __restrict__ RC,
-                                                     const RepDyn* __restrict__ RD,
-                                                     const CellAcc* __restrict__ acc,
-                                                     EndPartial* __restrict__ part) {
+template <typename Real>
+__global__ __launch_bounds__(256) void k_end(const RepConst* __restrict__ RC,
+                                             RepDyn* __restrict__ RD,
+                                             const CellAcc* __restrict__ acc,
+                                             const Real* __restrict__ ux, const Real* __restrict__ uy,
+                                             const Real* __restrict__ uz, EndPartial* part,
+                                             int* __restrict__ ticket) {
     __shared__ long long si[256][11];
-    __shared__ double sd[256][2];
+    __shared__ double sd[256][8];
+    __shared__ int s_last;
     const int r = blockIdx.y;
     const RepConst& C = RC[r];
-    const int ncell = RD[r].ncell;
+    RepDyn& D = RD[r];
+    const int ncell = D.ncell;
     const int chunk = (ncell + kEndParts - 1) / kEndParts;
     const int c0 = blockIdx.x * chunk, c1 = min(ncell, c0 + chunk);
     const int tid = threadIdx.x;
@@ -446,51 +476,54 @@ __global__ __launch_bounds__(256) void k_end_partial(const RepConst* __restrict_
         f += A.dfin;
         mv += A.moved_abs;
     }
-    for (int q = 0; q < 11; ++q) si[tid][q] = v[q];
+    double q[6] = {0, 0, 0, 0, 0, 0};
+    for (int i = blockIdx.x * 256 + tid; i < C.N; i += kEndParts * 256) {
+        const double x = ux[C.part_off + i], y = uy[C.part_off + i], z = uz[C.part_off + i];
+        q[0] += x * x; q[1] += y * y; q[2] += z * z; q[3] += x * y; q[4] += x * z; q[5] += y * z;
+    }
+    for (int k = 0; k < 11; ++k) si[tid][k] = v[k];
     sd[tid][0] = f;
     sd[tid][1] = mv;
+    for (int k = 0; k < 6; ++k) sd[tid][2 + k] = q[k];
     __syncthreads();
     for (int off = 128; off >= 1; off >>= 1) {
         if (tid < off) {
-            for (int q = 0; q < 11; ++q) si[tid][q] += si[tid + off][q];
-            sd[tid][0] += sd[tid + off][0];
-            sd[tid][1] += sd[tid + off][1];
+            for (int k = 0; k < 11; ++k) si[tid][k] += si[tid + off][k];
+            for (int k = 0; k < 8; ++k) sd[tid][k] += sd[tid + off][k];
         }
         __syncthreads();
     }
     if (tid == 0) {
         EndPartial& P = part[(size_t)r * kEndParts + blockIdx.x];
-        for (int q = 0; q < 11; ++q) P.v[q] = si[0][q];
+        for (int k = 0; k < 11; ++k) P.v[k] = si[0][k];
         P.f = sd[0][0];
         P.mv = sd[0][1];
+        for (int k = 0; k < 6; ++k) P.q[k] = sd[0][2 + k];
+        __threadfence();                                 // the partial before the ticket
+        s_last = atomicAdd(&ticket[r], 1) == kEndParts - 1;
     }
-}
-
-__global__ __launch_bounds__(64) void k_end_moves(const RepConst* __restrict__ RC,
-                                                  RepDyn* __restrict__ RD,
-                                                  const EndPartial* __restrict__ part) {
-    __shared__ long long si[kEndParts][11];
-    __shared__ double sd[kEndParts][2];
-    const int r = blockIdx.x;
-    const RepConst& C = RC[r];
-    RepDyn& D = RD[r];
-    const int tid = threadIdx.x;                     // one chunk per lane, then a fixed tree
-    {
+    __syncthreads();
+    if (!s_last) return;                                 // uniform over the block
+    __threadfence();                                     // every partial after the ticket
+    // the kEndParts partials: one per lane of wavefront 0, then a fixed tree (as before the
+    // fusion: the same sums in the same order)
+    if (tid < kEndParts) {
         const EndPartial& P = part[(size_t)r * kEndParts + tid];
-        for (int q = 0; q < 11; ++q) si[tid][q] = P.v[q];
+        for (int k = 0; k < 11; ++k) si[tid][k] = P.v[k];
         sd[tid][0] = P.f;
         sd[tid][1] = P.mv;
     }
     __syncthreads();
     for (int off = kEndParts / 2; off >= 1; off >>= 1) {
         if (tid < off) {
-            for (int q = 0; q < 11; ++q) si[tid][q] += si[tid + off][q];
+            for (int k = 0; k < 11; ++k) si[tid][k] += si[tid + off][k];
             sd[tid][0] += sd[tid + off][0];
             sd[tid][1] += sd[tid + off][1];
         }
         __syncthreads();
     }
     if (tid == 0) {
+        ticket[r] = 0;                                   // for the next sweep
         D.attempted += si[0][0]; D.accepted += si[0][1]; D.left += si[0][2]; D.hard += si[0][3];
         D.cand += si[0][4]; D.pairs += si[0][5]; D.coll += si[0][6];
         D.cap_pairs += si[0][7];
@@ -510,6 +543,10 @@ __global__ __launch_bounds__(64) void k_end_moves(const RepConst* __restrict__ R
                 D.adapts += 1;
             }
         }
+        double qs[6] = {0, 0, 0, 0, 0, 0};
+        for (int b = 0; b < kEndParts; ++b)
+            for (int k = 0; k < 6; ++k) qs[k] += part[(size_t)r * kEndParts + b].q[k];
+        for (int k = 0; k < 6; ++k) D.qsum[k] = qs[k];
     }
 }
 
@@ -670,19 +707,11 @@ __global__ __launch_bounds__(128) void k_energy(
 // 0 replace the running energy; 1 the same when a re-summation was flagged (records the drift);
 // 2 the NPT ln V acceptance test (absent from the reference, whose NPT is a label only,
 // MCnptColloid.c:1197, :1291); 3 only D.e_res.
-__global__ __launch_bounds__(256) void k_energy_reduce(const RepConst* __restrict__ RC,
-                                                       RepDyn* __restrict__ RD, int mode,
-                                                       const double* __restrict__ efin,
-                                                       const int* __restrict__ ecap,
-                                                       const int* __restrict__ eccap,
-                                                       const int* __restrict__ ehard) {
+__device__ inline void energy_reduce(const RepConst& C, RepDyn& D, int mode,
+                                     const double* __restrict__ efin, const int* __restrict__ ecap,
+                                     const int* __restrict__ eccap, const int* __restrict__ ehard) {
     __shared__ double sf[256];
     __shared__ long long sc[256][3];
-    const int r = blockIdx.x;
-    const RepConst& C = RC[r];
-    RepDyn& D = RD[r];
-    if (mode == 1 && !D.resum) return;
-    if (mode == 2 && !D.trial_ok) return;
     const int tid = threadIdx.x;
     double f = 0.0;
     long long a = 0, b = 0, h = 0;
@@ -743,6 +772,27 @@ __global__ __launch_bounds__(256) void k_energy_reduce(const RepConst* __restric
     }
 }
 
+__device__ inline void series_record(const RepConst& C, RepDyn& D, int r, double* __restrict__ series,
+                                     int series_cap, int advance);
+
+// One block per replica; series != null: then the sweep's record (series_record, advancing the
+// sweep index), after the energy.
+__global__ __launch_bounds__(256) void k_energy_reduce(const RepConst* __restrict__ RC,
+                                                       RepDyn* __restrict__ RD, int mode,
+                                                       const double* __restrict__ efin,
+                                                       const int* __restrict__ ecap,
+                                                       const int* __restrict__ eccap,
+                                                       const int* __restrict__ ehard,
+                                                       double* __restrict__ series = nullptr,
+                                                       int series_cap = 0) {
+    const int r = blockIdx.x;
+    const RepConst& C = RC[r];
+    RepDyn& D = RD[r];
+    const bool run = !(mode == 1 && !D.resum) && !(mode == 2 && !D.trial_ok);   // uniform
+    if (run) energy_reduce(C, D, mode, efin, ecap, eccap, ehard);
+    if (series && threadIdx.x == 0) series_record(C, D, r, series, series_cap, 1);
+}
+
 // NPT: trial box L' = L exp(d lnV / 3) and its cell grid (trial_ok = 0 when too small).
 __global__ void k_npt_trial(const RepConst* __restrict__ RC, RepDyn* __restrict__ RD, int nrep) {
     int r = blockIdx.x * blockDim.x + threadIdx.x;
@@ -796,22 +846,15 @@ __global__ __launch_bounds__(256) void k_observe(const RepConst* __restrict__ RC
 
 constexpr int kSeriesWidth = 12;   // E/N, Q(6)/N, density, L(3), sweep index
 
-// Appends the sweep's observables to the series ring and advances the sweep index.
-__global__ void k_observe_reduce(const RepConst* __restrict__ RC, RepDyn* __restrict__ RD, int nrep,
-                                 const double* __restrict__ part, double* __restrict__ series,
-                                 int series_cap, int advance) {
-    int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= nrep) return;
-    const RepConst& C = RC[r];
-    RepDyn& D = RD[r];
-    double q[6] = {0, 0, 0, 0, 0, 0};
-    for (int b = 0; b < 64; ++b)
-        for (int k = 0; k < 6; ++k) q[k] += part[((size_t)r * 64 + b) * 6 + k];
+// Appends the sweep's observables (running energy, the orientation sums of k_end, box) of
+// replica r to the series ring and advances the sweep index.
+__device__ inline void series_record(const RepConst& C, RepDyn& D, int r, double* __restrict__ series,
+                                     int series_cap, int advance) {
     if (series_cap > 0) {
         double* e = series + ((size_t)r * series_cap + (size_t)(D.series_count % series_cap)) * kSeriesWidth;
         const double E = (double)D.cap_pairs * 1e9 + (double)D.cap_coll * 1e8 + D.e_fin;
         e[0] = E / C.N;
-        for (int k = 0; k < 6; ++k) e[1 + k] = q[k] / C.N;
+        for (int k = 0; k < 6; ++k) e[1 + k] = D.qsum[k] / C.N;
         const double V = D.L[0] * D.L[1] * D.L[2];
         e[7] = C.N / (V - C.vcoll);
         e[8] = D.L[0]; e[9] = D.L[1]; e[10] = D.L[2];
@@ -819,6 +862,13 @@ __global__ void k_observe_reduce(const RepConst* __restrict__ RC, RepDyn* __rest
         D.series_count += 1;
     }
     if (advance) D.t += 1;
+}
+
+__global__ void k_record(const RepConst* __restrict__ RC, RepDyn* __restrict__ RD, int nrep,
+                         double* __restrict__ series, int series_cap, int advance) {
+    int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nrep) return;
+    series_record(RC[r], RD[r], r, series, series_cap, advance);
 }
 
 // ------------------------------------------------------------------ snapshots

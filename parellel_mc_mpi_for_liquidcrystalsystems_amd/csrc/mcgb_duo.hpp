@@ -223,6 +223,9 @@ __device__ __forceinline__ void pass_cell_duo(const PassArgs& P, int b, unsigned
     }
     auto rel = [](uint32_t q, double s) { return (double)(int32_t)(q - 0x80000000u) * s; };
     const bool ccache = cnear && nc <= PL::kCache;
+    // the cell's staged records come from both wavefronts' staging loads: a barrier before the
+    // cache reads them (uniform: ccache is the workgroup's)
+    if (ccache) __syncthreads();
     if (ccache)
         for (int j = tid; j < nc; j += 64 * W) {
             const Pos3 qj = sq[coff + j];

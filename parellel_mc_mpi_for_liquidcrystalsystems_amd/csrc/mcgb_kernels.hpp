@@ -28,6 +28,13 @@ __device__ inline double bcast(double v) {
     return __builtin_bit_cast(double, ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
 }
 
+__device__ inline long long bcast(long long v) {
+    unsigned long long u = (unsigned long long)v;
+    int lo = __builtin_amdgcn_readfirstlane((int)(u & 0xFFFFFFFFull));
+    int hi = __builtin_amdgcn_readfirstlane((int)(u >> 32));
+    return (long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
+}
+
 template <typename T>
 __device__ inline T wave_sum(T v) {
 #pragma unroll
@@ -439,14 +446,14 @@ __global__ __launch_bounds__(256) void k_count_draws(const RepConst* __restrict_
 // End of the passes, one launch (k_end): block b of replica r folds the per-cell accumulators
 // of cell chunk b and the orientation products u_a u_b of particles b*256 + tid + j*64*256 into
 // a partial; the last block of the replica to finish (a ticket) folds the kEndParts partials
-// in a fixed order (deterministic) into the running energy and counters, runs the step-size
-// controller, flags an FP64 re-summation when needed and keeps the orientation sums for the
-// sweep's record.
+// into the running energy and counters, runs the step-size controller, flags an FP64
+// re-summation when needed and keeps the orientation sums for the sweep's record.  Every sum
+// runs in a fixed order (lane butterflies, then the wavefronts in order): deterministic.
 constexpr int kEndParts = 64;
+constexpr int kEndVals = 19;       // 11 counters, finite dE, moved |dE|, 6 orientation sums
 struct EndPartial {
     long long v[11];
-    double f, mv;
-    double q[6];
+    double d[8];                   // finite dE, moved |dE|, orientation sums
 };
 
 template <typename Real>
@@ -456,8 +463,8 @@ __global__ __launch_bounds__(256) void k_end(const RepConst* __restrict__ RC,
                                              const Real* __restrict__ ux, const Real* __restrict__ uy,
                                              const Real* __restrict__ uz, EndPartial* part,
                                              int* __restrict__ ticket) {
-    __shared__ long long si[256][11];
-    __shared__ double sd[256][8];
+    __shared__ long long si[4][11];
+    __shared__ double sd[4][8];
     __shared__ int s_last;
     const int r = blockIdx.y;
     const RepConst& C = RC[r];
@@ -465,89 +472,65 @@ __global__ __launch_bounds__(256) void k_end(const RepConst* __restrict__ RC,
     const int ncell = D.ncell;
     const int chunk = (ncell + kEndParts - 1) / kEndParts;
     const int c0 = blockIdx.x * chunk, c1 = min(ncell, c0 + chunk);
-    const int tid = threadIdx.x;
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     long long v[11] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-    double f = 0.0, mv = 0.0;
+    double d[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     for (int c = c0 + tid; c < c1; c += 256) {
         const CellAcc& A = acc[C.cell_off + c];
         v[0] += A.att; v[1] += A.acc; v[2] += A.left; v[3] += A.hard;
         v[4] += A.cand; v[5] += A.pairs; v[6] += A.coll; v[7] += A.dcap; v[8] += A.dccap;
         v[9] += A.s1rej; v[10] += A.s2rej;
-        f += A.dfin;
-        mv += A.moved_abs;
+        d[0] += A.dfin;
+        d[1] += A.moved_abs;
     }
-    double q[6] = {0, 0, 0, 0, 0, 0};
     for (int i = blockIdx.x * 256 + tid; i < C.N; i += kEndParts * 256) {
         const double x = ux[C.part_off + i], y = uy[C.part_off + i], z = uz[C.part_off + i];
-        q[0] += x * x; q[1] += y * y; q[2] += z * z; q[3] += x * y; q[4] += x * z; q[5] += y * z;
+        d[2] += x * x; d[3] += y * y; d[4] += z * z; d[5] += x * y; d[6] += x * z; d[7] += y * z;
     }
-    for (int k = 0; k < 11; ++k) si[tid][k] = v[k];
-    sd[tid][0] = f;
-    sd[tid][1] = mv;
-    for (int k = 0; k < 6; ++k) sd[tid][2 + k] = q[k];
+    for (int k = 0; k < 11; ++k) v[k] = wave_sum(v[k]);
+    for (int k = 0; k < 8; ++k) d[k] = wave_sum(d[k]);
+    if (lane == 0) {
+        for (int k = 0; k < 11; ++k) si[w][k] = v[k];
+        for (int k = 0; k < 8; ++k) sd[w][k] = d[k];
+    }
     __syncthreads();
-    for (int off = 128; off >= 1; off >>= 1) {
-        if (tid < off) {
-            for (int k = 0; k < 11; ++k) si[tid][k] += si[tid + off][k];
-            for (int k = 0; k < 8; ++k) sd[tid][k] += sd[tid + off][k];
-        }
-        __syncthreads();
-    }
     if (tid == 0) {
         EndPartial& P = part[(size_t)r * kEndParts + blockIdx.x];
-        for (int k = 0; k < 11; ++k) P.v[k] = si[0][k];
-        P.f = sd[0][0];
-        P.mv = sd[0][1];
-        for (int k = 0; k < 6; ++k) P.q[k] = sd[0][2 + k];
+        for (int k = 0; k < 11; ++k) P.v[k] = si[0][k] + si[1][k] + si[2][k] + si[3][k];
+        for (int k = 0; k < 8; ++k) P.d[k] = ((sd[0][k] + sd[1][k]) + sd[2][k]) + sd[3][k];
         __threadfence();                                 // the partial before the ticket
         s_last = atomicAdd(&ticket[r], 1) == kEndParts - 1;
     }
     __syncthreads();
-    if (!s_last) return;                                 // uniform over the block
+    if (!s_last || w != 0) return;                       // uniform per wavefront
     __threadfence();                                     // every partial after the ticket
-    // the kEndParts partials: one per lane of wavefront 0, then a fixed tree (as before the
-    // fusion: the same sums in the same order)
-    if (tid < kEndParts) {
-        const EndPartial& P = part[(size_t)r * kEndParts + tid];
-        for (int k = 0; k < 11; ++k) si[tid][k] = P.v[k];
-        sd[tid][0] = P.f;
-        sd[tid][1] = P.mv;
-    }
-    __syncthreads();
-    for (int off = kEndParts / 2; off >= 1; off >>= 1) {
-        if (tid < off) {
-            for (int k = 0; k < 11; ++k) si[tid][k] += si[tid + off][k];
-            sd[tid][0] += sd[tid + off][0];
-            sd[tid][1] += sd[tid + off][1];
+    // one partial per lane, lane butterflies
+    static_assert(kEndParts == 64, "one partial per lane of a wavefront");
+    const EndPartial& P = part[(size_t)r * kEndParts + lane];
+    for (int k = 0; k < 11; ++k) v[k] = wave_sum(P.v[k]);
+    for (int k = 0; k < 8; ++k) d[k] = wave_sum(P.d[k]);
+    if (lane != 0) return;
+    ticket[r] = 0;                                       // for the next sweep
+    D.attempted += v[0]; D.accepted += v[1]; D.left += v[2]; D.hard += v[3];
+    D.cand += v[4]; D.pairs += v[5]; D.coll += v[6];
+    D.cap_pairs += v[7];
+    D.cap_coll += v[8];
+    D.da_s1 += v[9];
+    D.da_s2 += v[10];
+    D.e_fin += d[0];
+    D.moved_abs = d[1];
+    if (d[1] > 1.0e10) D.resum = 1;
+    if (C.adapt_moves > 0) {
+        D.ad_att += (unsigned long long)v[0];
+        D.ad_acc += (unsigned long long)v[1];
+        if (D.ad_att >= (unsigned long long)C.adapt_moves) {
+            const double half_box = 0.5 * fmin(D.L[0], fmin(D.L[1], D.L[2]));
+            adapt_steps(D.dmax, D.drot, D.ad_acc, D.ad_att, C.min_thick, half_box);
+            D.ad_att = D.ad_acc = 0;
+            D.adapts += 1;
         }
-        __syncthreads();
     }
-    if (tid == 0) {
-        ticket[r] = 0;                                   // for the next sweep
-        D.attempted += si[0][0]; D.accepted += si[0][1]; D.left += si[0][2]; D.hard += si[0][3];
-        D.cand += si[0][4]; D.pairs += si[0][5]; D.coll += si[0][6];
-        D.cap_pairs += si[0][7];
-        D.cap_coll += si[0][8];
-        D.da_s1 += si[0][9];
-        D.da_s2 += si[0][10];
-        D.e_fin += sd[0][0];
-        D.moved_abs = sd[0][1];
-        if (sd[0][1] > 1.0e10) D.resum = 1;
-        if (C.adapt_moves > 0) {
-            D.ad_att += (unsigned long long)si[0][0];
-            D.ad_acc += (unsigned long long)si[0][1];
-            if (D.ad_att >= (unsigned long long)C.adapt_moves) {
-                const double half_box = 0.5 * fmin(D.L[0], fmin(D.L[1], D.L[2]));
-                adapt_steps(D.dmax, D.drot, D.ad_acc, D.ad_att, C.min_thick, half_box);
-                D.ad_att = D.ad_acc = 0;
-                D.adapts += 1;
-            }
-        }
-        double qs[6] = {0, 0, 0, 0, 0, 0};
-        for (int b = 0; b < kEndParts; ++b)
-            for (int k = 0; k < 6; ++k) qs[k] += part[(size_t)r * kEndParts + b].q[k];
-        for (int k = 0; k < 6; ++k) D.qsum[k] = qs[k];
-    }
+    for (int k = 0; k < 6; ++k) D.qsum[k] = d[2 + k];
 }
 
 // ------------------------------------------------------------------ FP64 full energy

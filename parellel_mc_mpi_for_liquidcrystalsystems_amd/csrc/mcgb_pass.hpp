@@ -638,6 +638,9 @@ __device__ __forceinline__ void pass_cell(const PassArgs& P, int b, unsigned cha
     // term is exactly 0 beyond the colloid cutoff, so evaluating it everywhere in such cells
     // gives the same sums as testing the reach first.
     const bool ccache = staged && cnear && nc <= L::kCache;
+    // the cell's staged records come from every wavefront's staging loads: a barrier before
+    // the cache reads them (uniform: ccache is the workgroup's)
+    if (ccache) __syncthreads();
     if (ccache)
         for (int j = tid; j < nc; j += 64 * W) {
             const Pos3 qj = sq[coff + j];

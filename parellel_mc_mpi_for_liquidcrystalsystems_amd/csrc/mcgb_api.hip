@@ -1710,7 +1710,7 @@ int mcgb_philox_batch(int n, const uint32_t* ctr, const uint32_t* key, uint32_t*
     return MCGB_OK;
 }
 
-// The draws k_draws wrote for the last sweep of replica r, in record order
+// The move draws the last sweep of replica r made (k_count_draws), in record order
 // pass*(nact*k) + active_cell*k + move.  Returns the number of records.
 int mcgb_sweep_draws(mcgb_ctx* c, int r, int capacity, uint32_t* j, int32_t* dq, double* v,
                      double* thr, uint32_t info[4]) {

@@ -86,7 +86,7 @@ struct RepDyn {
                                  // the passes of the last sweep (order-parameter record)
 };
 
-// Draws of one trial move (k_draws): particle selector, fixed-point step, rotation vector,
+// Draws of one trial move (draw_move): particle selector, fixed-point step, rotation vector,
 // Metropolis threshold -log(U) T, and the threshold of the surrogate stage of delayed
 // acceptance -log(U') T (U' an independent uniform).
 struct MoveDraw {

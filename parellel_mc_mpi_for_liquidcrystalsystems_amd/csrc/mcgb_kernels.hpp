@@ -415,13 +415,6 @@ __device__ inline void draw_move(const RepConst& C, const RepDyn& D, MoveDraw* _
     rec[C.rec_off + idx] = d;
 }
 
-__global__ __launch_bounds__(256) void k_draws(const RepConst* __restrict__ RC,
-                                               const RepDyn* __restrict__ RD,
-                                               MoveDraw* __restrict__ rec) {
-    const int r = blockIdx.y;
-    draw_move(RC[r], RD[r], rec, blockIdx.x * blockDim.x + threadIdx.x);
-}
-
 // The sweep's move draws and the cell counts of the rebuild in one launch (both follow k_begin
 // and neither reads the other's output): thread i draws move i and counts particle i.
 __global__ __launch_bounds__(256) void k_count_draws(const RepConst* __restrict__ RC,

@@ -1,6 +1,6 @@
 """The GPU sweep samples the exact Boltzmann distribution.
 
-* k_draws: every draw of a sweep equals the oracle's draw for the same (sweep, pass, cell,
+* the sweep's move draws: every draw of a sweep equals the oracle's draw for the same (sweep, pass, cell,
   move) counter, and the device's rotation vectors pass the unit-sphere checks on their own;
 * one rod near the colloid and two Gay-Berne rods, run as many independent replicas in one
   context: their (distance, angle) and pair statistics against the exact integrals of

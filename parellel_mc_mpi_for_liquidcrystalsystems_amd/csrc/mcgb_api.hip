@@ -82,6 +82,7 @@ struct mcgb_ctx {
     int *cell_start2 = nullptr, *list2 = nullptr;
     unsigned long long* scan_st = nullptr;   // [nrep][scan_tiles] look-back status of the scan
     int* slot = nullptr;                 // each particle's place within its cell (rebuild)
+    int* tmp_pid = nullptr;              // particle ids scattered beside tmp (rebuild)
     int* end_ticket = nullptr;           // [nrep] blocks of k_end done this sweep
     EndPartial* endp = nullptr;          // [nrep][kEndParts] partial sweep totals
     int scan_tiles = 1;
@@ -249,19 +250,21 @@ int enqueue_rebuild(mcgb_ctx* c, bool draws = false) {
     const int src = c->cur, dst = 1 - c->cur;
     const dim3 gp((c->Nmax + 255) / 256, c->nrep);
     // counts start at zero: set once at creation, reset by k_sort_gather after every rebuild
-    if (draws)
-        k_count_draws<<<dim3((std::max(c->rec_max, c->Nmax) + 255) / 256, c->nrep), 256, 0, c->st>>>(
+    if (draws) {
+        const int nbc = (c->Nmax + 255) / 256, nbd = (c->rec_max + 255) / 256;
+        k_count_draws<<<dim3(nbc + nbd, c->nrep), 256, 0, c->st>>>(
             c->dC, c->dD, c->draws, c->q[src][0], c->q[src][1], c->q[src][2], c->cell_id, c->slot,
-            c->count);
-    else
+            c->count, nbc);
+    } else {
         k_count<<<gp, 256, 0, c->st>>>(c->dC, c->dD, c->q[src][0], c->q[src][1], c->q[src][2],
                                        c->cell_id, c->slot, c->count, 0);
+    }
     enqueue_scan(c, c->cell_start, 0);
-    k_scatter<<<gp, 256, 0, c->st>>>(c->dC, c->dD, c->cell_id, c->slot, c->cell_start, c->tmp,
-                                     c->scan_st, c->scan_tiles, 0);
+    k_scatter<<<gp, 256, 0, c->st>>>(c->dC, c->dD, c->cell_id, c->slot, c->cell_start, c->pid[src],
+                                     c->tmp, c->tmp_pid, c->scan_st, c->scan_tiles, 0);
     const dim3 gs((c->ncell_max + 3) / 4, c->nrep);
     k_sort_gather<Real><<<gs, 256, 0, c->st>>>(
-        c->dC, c->dD, c->cell_start, c->tmp, c->pid[src], 0, 0, nullptr, c->count, c->q[src][0], c->q[src][1],
+        c->dC, c->dD, c->cell_start, c->tmp, c->tmp_pid, 0, 0, nullptr, c->count, c->q[src][0], c->q[src][1],
         c->q[src][2], (const Real*)c->u[src][0], (const Real*)c->u[src][1], (const Real*)c->u[src][2],
         c->q[dst][0], c->q[dst][1], c->q[dst][2], (Real*)c->u[dst][0], (Real*)c->u[dst][1],
         (Real*)c->u[dst][2], c->pid[dst]);
@@ -294,11 +297,11 @@ int enqueue_npt(mcgb_ctx* c) {
     k_count<<<gp, 256, 0, c->st>>>(c->dC, c->dD, c->q[b][0], c->q[b][1], c->q[b][2], c->cell_id,
                                    c->slot, c->count, 1);
     enqueue_scan(c, c->cell_start2, 1);
-    k_scatter<<<gp, 256, 0, c->st>>>(c->dC, c->dD, c->cell_id, c->slot, c->cell_start2, c->tmp,
-                                     c->scan_st, c->scan_tiles, 1);
+    k_scatter<<<gp, 256, 0, c->st>>>(c->dC, c->dD, c->cell_id, c->slot, c->cell_start2, c->pid[b],
+                                     c->tmp, c->tmp_pid, c->scan_st, c->scan_tiles, 1);
     const dim3 gs((c->ncell_max + 3) / 4, c->nrep);
     k_sort_gather<Real><<<gs, 256, 0, c->st>>>(
-        c->dC, c->dD, c->cell_start2, c->tmp, c->pid[b], 1, 1, c->list2, c->count, nullptr, nullptr, nullptr,
+        c->dC, c->dD, c->cell_start2, c->tmp, c->tmp_pid, 1, 1, c->list2, c->count, nullptr, nullptr, nullptr,
         nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
     HIPCHK(hipGetLastError());
     return enqueue_energy<Real>(c, 2);
@@ -728,6 +731,7 @@ int mcgb_create(const mcgb_params* p, int n_replicas, mcgb_ctx** out) {
     c->scan_tiles = (c->ncell_max + kScanTile - 1) / kScanTile;
     if (!rc) rc = dalloc_t(c, &c->scan_st, (size_t)c->nrep * c->scan_tiles);
     if (!rc) rc = dalloc_t(c, &c->slot, N);
+    if (!rc) rc = dalloc_t(c, &c->tmp_pid, N);
     if (!rc) rc = dalloc_t(c, &c->end_ticket, c->nrep);
     if (!rc) rc = dalloc_t(c, &c->endp, (size_t)c->nrep * kEndParts);
     if (!rc) rc = dalloc_t(c, &c->acc, c->ncell_tot);

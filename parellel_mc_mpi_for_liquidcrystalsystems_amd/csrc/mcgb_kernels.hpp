@@ -305,11 +305,13 @@ __global__ __launch_bounds__(256) void k_scan(const RepConst* __restrict__ RC,
     if (t == 0 && threadIdx.x == 0) cs[ncell] = C.N;
 }
 
-// Scatter of particle slots into their cell segments (the slot within the cell came with the
-// count); block 0 of each replica row clears the scan's status words for the next scan.
+// Scatter of particle slots and ids into their cell segments (the slot within the cell came
+// with the count); block 0 of each replica row clears the scan's status words for the next
+// scan.
 __global__ void k_scatter(const RepConst* __restrict__ RC, const RepDyn* __restrict__ RD,
                           const int* __restrict__ cell_id, const int* __restrict__ slot,
-                          const int* __restrict__ cell_start, int* __restrict__ tmp,
+                          const int* __restrict__ cell_start, const int* __restrict__ pid,
+                          int* __restrict__ tmp, int* __restrict__ tmp_pid,
                           unsigned long long* __restrict__ status, int tmax, int trial) {
     const int r = blockIdx.y;
     const RepConst& C = RC[r];
@@ -319,7 +321,9 @@ __global__ void k_scatter(const RepConst* __restrict__ RC, const RepDyn* __restr
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= C.N) return;
     const int gi = C.part_off + i;
-    tmp[C.part_off + cell_start[C.cs_off + cell_id[gi]] + slot[gi]] = i;
+    const int dst = C.part_off + cell_start[C.cs_off + cell_id[gi]] + slot[gi];
+    tmp[dst] = i;
+    tmp_pid[dst] = pid[gi];
 }
 
 // Orders every cell segment by particle id (a deterministic cell list: dist_batch_to_cells,
@@ -329,7 +333,7 @@ __global__ void k_scatter(const RepConst* __restrict__ RC, const RepDyn* __restr
 template <typename Real>
 __global__ __launch_bounds__(256) void k_sort_gather(
     const RepConst* __restrict__ RC, const RepDyn* __restrict__ RD, const int* __restrict__ cell_start,
-    const int* __restrict__ tmp, const int* __restrict__ pid_src, int index_only, int trial,
+    const int* __restrict__ tmp, const int* __restrict__ tmp_pid, int index_only, int trial,
     int* __restrict__ list_out, int* __restrict__ count,
     const uint32_t* __restrict__ sqx, const uint32_t* __restrict__ sqy, const uint32_t* __restrict__ sqz,
     const Real* __restrict__ sux, const Real* __restrict__ suy, const Real* __restrict__ suz,
@@ -349,16 +353,16 @@ __global__ __launch_bounds__(256) void k_sort_gather(
     // the fill pointers of k_scatter are not read again: back to zero for the next rebuild
     if (valid && lane == 0) count[C.cell_off + c] = 0;
     const int* seg = tmp + C.part_off + s;
-    const int* pids = pid_src + C.part_off;
+    const int* pids = tmp_pid + C.part_off + s;          // the segment's ids, in slot order
     const bool in_lds = nc <= 256;
     if (in_lds)
-        for (int k = lane; k < nc; k += 64) spid[w][k] = pids[seg[k]];
+        for (int k = lane; k < nc; k += 64) spid[w][k] = pids[k];
     __syncthreads();
     for (int k = lane; k < nc; k += 64) {
         const int idx = seg[k];
-        const int p = in_lds ? spid[w][k] : pids[idx];
+        const int p = in_lds ? spid[w][k] : pids[k];
         int rank = 0;
-        for (int j = 0; j < nc; ++j) rank += (in_lds ? spid[w][j] : pids[seg[j]]) < p;
+        for (int j = 0; j < nc; ++j) rank += (in_lds ? spid[w][j] : pids[j]) < p;
         const int dst = C.part_off + s + rank;
         if (index_only) {
             list_out[dst] = idx;
@@ -416,7 +420,8 @@ __device__ inline void draw_move(const RepConst& C, const RepDyn& D, MoveDraw* _
 }
 
 // The sweep's move draws and the cell counts of the rebuild in one launch (both follow k_begin
-// and neither reads the other's output): thread i draws move i and counts particle i.
+// and neither reads the other's output): blocks [0, nbc) count particles, the others draw
+// moves, so the counting's atomic round trips and the draws' arithmetic run side by side.
 __global__ __launch_bounds__(256) void k_count_draws(const RepConst* __restrict__ RC,
                                                      const RepDyn* __restrict__ RD,
                                                      MoveDraw* __restrict__ rec,
@@ -424,25 +429,29 @@ __global__ __launch_bounds__(256) void k_count_draws(const RepConst* __restrict_
                                                      const uint32_t* __restrict__ qy,
                                                      const uint32_t* __restrict__ qz,
                                                      int* __restrict__ cell_id, int* __restrict__ slot,
-                                                     int* __restrict__ count) {
+                                                     int* __restrict__ count, int nbc) {
     const int r = blockIdx.y;
     const RepConst& C = RC[r];
     const RepDyn& D = RD[r];
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (blockIdx.x * blockDim.x < C.N)                 // uniform: wavefronts holding particles
-        count_particle(C, D, i, qx, qy, qz, cell_id, slot, count, 0);
-    draw_move(C, D, rec, i);
+    if ((int)blockIdx.x < nbc) {                       // uniform over the block
+        const int i = blockIdx.x * blockDim.x + threadIdx.x;
+        if (blockIdx.x * blockDim.x < (unsigned)C.N)   // wavefronts holding particles
+            count_particle(C, D, i, qx, qy, qz, cell_id, slot, count, 0);
+    } else {
+        draw_move(C, D, rec, (blockIdx.x - nbc) * blockDim.x + threadIdx.x);
+    }
 }
 
 // ------------------------------------------------------------------ end of the passes
 
 // End of the passes, one launch (k_end): block b of replica r folds the per-cell accumulators
-// of cell chunk b and the orientation products u_a u_b of particles b*256 + tid + j*64*256 into
-// a partial; the last block of the replica to finish (a ticket) folds the kEndParts partials
-// into the running energy and counters, runs the step-size controller, flags an FP64
+// of cell chunk b and the orientation products u_a u_b of particles b*256 + tid + j*kEndParts*256
+// into a partial; the last block of the replica to finish (a ticket) folds the kEndParts
+// partials into the running energy and counters, runs the step-size controller, flags an FP64
 // re-summation when needed and keeps the orientation sums for the sweep's record.  Every sum
-// runs in a fixed order (lane butterflies, then the wavefronts in order): deterministic.
-constexpr int kEndParts = 64;
+// runs in a fixed order (four particles' loads issued together, lane butterflies, then the
+// wavefronts in order): deterministic.
+constexpr int kEndParts = 256;
 constexpr int kEndVals = 19;       // 11 counters, finite dE, moved |dE|, 6 orientation sums
 struct EndPartial {
     long long v[11];
@@ -456,8 +465,9 @@ __global__ __launch_bounds__(256) void k_end(const RepConst* __restrict__ RC,
                                              const Real* __restrict__ ux, const Real* __restrict__ uy,
                                              const Real* __restrict__ uz, EndPartial* part,
                                              int* __restrict__ ticket) {
-    __shared__ long long si[4][11];
-    __shared__ double sd[4][8];
+    // the 19 sums travel as doubles: the counters are integers far below 2^53, exact in FP64,
+    // so one fixed-order DPP reduction (result in lane 63) serves them all
+    __shared__ double sv[4][kEndVals];
     __shared__ int s_last;
     const int r = blockIdx.y;
     const RepConst& C = RC[r];
@@ -476,33 +486,57 @@ __global__ __launch_bounds__(256) void k_end(const RepConst* __restrict__ RC,
         d[0] += A.dfin;
         d[1] += A.moved_abs;
     }
-    for (int i = blockIdx.x * 256 + tid; i < C.N; i += kEndParts * 256) {
-        const double x = ux[C.part_off + i], y = uy[C.part_off + i], z = uz[C.part_off + i];
-        d[2] += x * x; d[3] += y * y; d[4] += z * z; d[5] += x * y; d[6] += x * z; d[7] += y * z;
+    constexpr int kStride = kEndParts * 256;
+    for (int i0 = blockIdx.x * 256 + tid; i0 < C.N; i0 += 4 * kStride) {
+        double x[4], y[4], z[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int i = i0 + u * kStride;
+            const bool in = i < C.N;
+            x[u] = in ? (double)ux[C.part_off + i] : 0.0;
+            y[u] = in ? (double)uy[C.part_off + i] : 0.0;
+            z[u] = in ? (double)uz[C.part_off + i] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            d[2] += x[u] * x[u]; d[3] += y[u] * y[u]; d[4] += z[u] * z[u];
+            d[5] += x[u] * y[u]; d[6] += x[u] * z[u]; d[7] += y[u] * z[u];
+        }
     }
-    for (int k = 0; k < 11; ++k) v[k] = wave_sum(v[k]);
-    for (int k = 0; k < 8; ++k) d[k] = wave_sum(d[k]);
-    if (lane == 0) {
-        for (int k = 0; k < 11; ++k) si[w][k] = v[k];
-        for (int k = 0; k < 8; ++k) sd[w][k] = d[k];
-    }
+    double e[kEndVals];
+#pragma unroll
+    for (int k = 0; k < 11; ++k) e[k] = wave_sum_d63((double)v[k]);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) e[11 + k] = wave_sum_d63(d[k]);
+    if (lane == 63)
+        for (int k = 0; k < kEndVals; ++k) sv[w][k] = e[k];
     __syncthreads();
     if (tid == 0) {
         EndPartial& P = part[(size_t)r * kEndParts + blockIdx.x];
-        for (int k = 0; k < 11; ++k) P.v[k] = si[0][k] + si[1][k] + si[2][k] + si[3][k];
-        for (int k = 0; k < 8; ++k) P.d[k] = ((sd[0][k] + sd[1][k]) + sd[2][k]) + sd[3][k];
+        for (int k = 0; k < 11; ++k)
+            P.v[k] = (long long)sv[0][k] + (long long)sv[1][k] + (long long)sv[2][k] + (long long)sv[3][k];
+        for (int k = 0; k < 8; ++k) P.d[k] = ((sv[0][11 + k] + sv[1][11 + k]) + sv[2][11 + k]) + sv[3][11 + k];
         __threadfence();                                 // the partial before the ticket
         s_last = atomicAdd(&ticket[r], 1) == kEndParts - 1;
     }
     __syncthreads();
     if (!s_last || w != 0) return;                       // uniform per wavefront
     __threadfence();                                     // every partial after the ticket
-    // one partial per lane, lane butterflies
-    static_assert(kEndParts == 64, "one partial per lane of a wavefront");
-    const EndPartial& P = part[(size_t)r * kEndParts + lane];
-    for (int k = 0; k < 11; ++k) v[k] = wave_sum(P.v[k]);
-    for (int k = 0; k < 8; ++k) d[k] = wave_sum(P.d[k]);
-    if (lane != 0) return;
+    // partials lane, lane + 64, ... in order per lane, then one DPP reduction per sum
+    static_assert(kEndParts % 64 == 0, "whole partials per lane");
+    for (int k = 0; k < 11; ++k) v[k] = 0;
+    for (int k = 0; k < 8; ++k) d[k] = 0.0;
+    for (int j = lane; j < kEndParts; j += 64) {
+        const EndPartial& P = part[(size_t)r * kEndParts + j];
+        for (int k = 0; k < 11; ++k) v[k] += P.v[k];
+        for (int k = 0; k < 8; ++k) d[k] += P.d[k];
+    }
+#pragma unroll
+    for (int k = 0; k < 11; ++k) e[k] = wave_sum_d63((double)v[k]);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) e[11 + k] = wave_sum_d63(d[k]);
+    if (lane != 63) return;
+    for (int k = 0; k < 11; ++k) v[k] = (long long)e[k];
     ticket[r] = 0;                                       // for the next sweep
     D.attempted += v[0]; D.accepted += v[1]; D.left += v[2]; D.hard += v[3];
     D.cand += v[4]; D.pairs += v[5]; D.coll += v[6];
@@ -510,9 +544,9 @@ __global__ __launch_bounds__(256) void k_end(const RepConst* __restrict__ RC,
     D.cap_coll += v[8];
     D.da_s1 += v[9];
     D.da_s2 += v[10];
-    D.e_fin += d[0];
-    D.moved_abs = d[1];
-    if (d[1] > 1.0e10) D.resum = 1;
+    D.e_fin += e[11];
+    D.moved_abs = e[12];
+    if (e[12] > 1.0e10) D.resum = 1;
     if (C.adapt_moves > 0) {
         D.ad_att += (unsigned long long)v[0];
         D.ad_acc += (unsigned long long)v[1];
@@ -523,7 +557,7 @@ __global__ __launch_bounds__(256) void k_end(const RepConst* __restrict__ RC,
             D.adapts += 1;
         }
     }
-    for (int k = 0; k < 6; ++k) D.qsum[k] = d[2 + k];
+    for (int k = 0; k < 6; ++k) D.qsum[k] = e[13 + k];
 }
 
 // ------------------------------------------------------------------ FP64 full energy

@@ -798,7 +798,10 @@ __device__ __forceinline__ void pass_cell(const PassArgs& P, int b, unsigned cha
                                 for (int j = 0; j < kG; ++j)
                                     if (j < nv) {
                                         const int e = (g0 + j * W) * 64 + lane;
-                                        const bool kk = M.near(p[j], iso) & (e != li);
+                                        // the moved particle itself passes the filter (it lies
+                                        // within |dq|/2 of the midpoint): listed, and dropped
+                                        // where its items are evaluated
+                                        const bool kk = M.near(p[j], iso);
                                         const unsigned long long mk = __builtin_amdgcn_ballot_w64(kk);
                                         // every lane writes: unselected ones into the spare slot
                                         const int rank = lane_rank(mk);
@@ -868,7 +871,7 @@ __device__ __forceinline__ void pass_cell(const PassArgs& P, int b, unsigned cha
                                 const double dy = (double)(int)(pe.y - (nw ? qny : qiy)) * X.s1;
                                 const double dz = (double)(int)(pe.z - (nw ? qnz : qiz)) * X.s2;
                                 const double r2 = dx * dx + dy * dy + dz * dz;
-                                if (r2 < X.rc2) {
+                                if (r2 < X.rc2 && e != li) {
                                     on = true;
                                     const float ex = (float)(nw ? unx : uix), ey = (float)(nw ? uny : uiy),
                                                 ez = (float)(nw ? unz : uiz);
@@ -921,7 +924,7 @@ __device__ __forceinline__ void pass_cell(const PassArgs& P, int b, unsigned cha
                             pair_energy_two(G, X, ox, oy, oz, jx, jy, jz, dxs, dys, dzs, v2, c2, in2);
                             // selects, not branches: a branch would let the compiler sink
                             // each pair's chain into its own block (adding 0 leaves a sum as is)
-                            const bool ia = in2[0], ib = in2[1] & hb;
+                            const bool ia = in2[0] & (ea != li), ib = in2[1] & hb & (eb != li);
                             const bool fa = ia & !c2[0], fb = ib & !c2[1];
                             const Real ta = fa ? (nw ? v2[0] : -v2[0]) : (Real)0;
                             const Real tb = fb ? (nw ? v2[1] : -v2[1]) : (Real)0;

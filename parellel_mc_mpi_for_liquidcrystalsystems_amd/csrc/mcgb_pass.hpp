@@ -730,18 +730,21 @@ __device__ __forceinline__ void pass_cell(const PassArgs& P, int b, unsigned cha
                 uint32_t qix, qiy, qiz;
                 Real uix, uiy, uiz;
                 if (fresh) {
-                    qix = uni(pm.qx); qiy = uni(pm.qy); qiz = uni(pm.qz);
-                    uix = uni(pm.ux); uiy = uni(pm.uy); uiz = uni(pm.uz);
+                    // the particle's state and the step stay in vector registers (every lane
+                    // read the same record): they feed vector arithmetic only, and as scalars
+                    // they cost a readfirstlane and, for use by the lanes, a move back each
+                    qix = pm.qx; qiy = pm.qy; qiz = pm.qz;
+                    uix = pm.ux; uiy = pm.uy; uiz = pm.uz;
                 } else if (staged) {
                     const Pos3 qi = sq[li];
                     const R3 ui = su[li];
-                    qix = uni(qi.x); qiy = uni(qi.y); qiz = uni(qi.z);
-                    uix = uni(ui.x); uiy = uni(ui.y); uiz = uni(ui.z);
+                    qix = qi.x; qiy = qi.y; qiz = qi.z;
+                    uix = ui.x; uiy = ui.y; uiz = ui.z;
                 } else {
                     qix = P.qx[gsel]; qiy = P.qy[gsel]; qiz = P.qz[gsel];
                     uix = Ux[gsel]; uiy = Uy[gsel]; uiz = Uz[gsel];
                 }
-                const int dqx = uni(pm.dqx), dqy = uni(pm.dqy), dqz = uni(pm.dqz);
+                const int dqx = pm.dqx, dqy = pm.dqy, dqz = pm.dqz;
                 const uint32_t qnx = qix + (uint32_t)dqx, qny = qiy + (uint32_t)dqy,
                                qnz = qiz + (uint32_t)dqz;
                 int outcome;
@@ -753,9 +756,9 @@ __device__ __forceinline__ void pass_cell(const PassArgs& P, int b, unsigned cha
                 float fr2;
                 const uint32_t tests =
                     fresh ? fl & (kPrepLeft | kPrepHard)
-                          : prep_tests(qix, qiy, qiz, dqx, dqy, dqz, gx, gy, gz, nx, ny, nz, cx, cy,
-                                       cz, cnear, R2, X, FC, &fr2);
-                if (fresh) fr2 = uni(pm.fr2);
+                          : uni(prep_tests(qix, qiy, qiz, dqx, dqy, dqz, gx, gy, gz, nx, ny, nz, cx,
+                                           cy, cz, cnear, R2, X, FC, &fr2));
+                if (fresh) fr2 = pm.fr2;
                 if (tests & kPrepLeft) {
                     outcome = MCGB_MOVE_LEFT;       // moves are confined to the active cell
                     ++n_left;
@@ -768,10 +771,9 @@ __device__ __forceinline__ void pass_cell(const PassArgs& P, int b, unsigned cha
                     const double rnz = (double)(int32_t)(qnz - 0x80000000u) * X.s2;
                     Real unx, uny, unz;
                     if (fresh) {
-                        unx = uni(pm.nx); uny = uni(pm.ny); unz = uni(pm.nz);
+                        unx = pm.nx; uny = pm.ny; unz = pm.nz;
                     } else {
                         rotate(drot, uix, uiy, uiz, pm.vx, pm.vy, pm.vz, unx, uny, unz);
-                        unx = uni(unx); uny = uni(uny); unz = uni(unz);
                     }
                     LaneAcc<Real> acc;
                     Real scl = 0;

@@ -34,11 +34,14 @@ template <int CTRL, int ROWMASK>
 __device__ inline float dpp_f(float v) {
     return __builtin_bit_cast(float, dpp_i<CTRL, ROWMASK>(__builtin_bit_cast(int, v)));
 }
+// FP64 halves moved with an undefined "old" operand: a lane that a step's row mask leaves out
+// gets an arbitrary value, but such lanes never feed lane 63's chain (rows 0 and 2 in the two
+// broadcast steps), and no move of a zero or a copy precedes each DPP move.
 template <int CTRL, int ROWMASK>
 __device__ inline double dpp_d(double v) {
     unsigned long long u = __builtin_bit_cast(unsigned long long, v);
-    int lo = dpp_i<CTRL, ROWMASK>((int)(u & 0xFFFFFFFFull));
-    int hi = dpp_i<CTRL, ROWMASK>((int)(u >> 32));
+    int lo = __builtin_amdgcn_mov_dpp((int)(u & 0xFFFFFFFFull), CTRL, ROWMASK, 0xF, false);
+    int hi = __builtin_amdgcn_mov_dpp((int)(u >> 32), CTRL, ROWMASK, 0xF, false);
     return __builtin_bit_cast(double, ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
 }
 
@@ -805,9 +808,8 @@ __device__ __forceinline__ void pass_cell(const PassArgs& P, int b, unsigned cha
                                         // where its items are evaluated
                                         const bool kk = M.near(p[j], iso);
                                         const unsigned long long mk = __builtin_amdgcn_ballot_w64(kk);
-                                        // every lane writes: unselected ones into the spare slot
-                                        const int rank = lane_rank(mk);
-                                        slist[kk ? base + rank : lcap] = (unsigned short)e;
+                                        // the selected lanes write (under the compare's mask)
+                                        if (kk) slist[base + lane_rank(mk)] = (unsigned short)e;
                                         base += __popcll(mk);
                                     }
                             }

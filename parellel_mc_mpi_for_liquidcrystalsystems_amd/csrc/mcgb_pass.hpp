@@ -795,10 +795,13 @@ __device__ __forceinline__ void pass_cell(const PassArgs& P, int b, unsigned cha
                             const int ng = S_pad >> 6;
                             for (int g0 = w; g0 < ng; g0 += kG * W) {
                                 const int nv = min(kG, (ng - g0 + W - 1) / W);   // uniform
+                                // every load issued unconditionally (groups past the end read
+                                // the last one again): no branch between them, so they are in
+                                // flight together
                                 Pos3 p[kG];
 #pragma unroll
                                 for (int j = 0; j < kG; ++j)
-                                    if (j < nv) p[j] = sq[(g0 + j * W) * 64 + lane];
+                                    p[j] = sq[min(g0 + j * W, ng - 1) * 64 + lane];
 #pragma unroll
                                 for (int j = 0; j < kG; ++j)
                                     if (j < nv) {

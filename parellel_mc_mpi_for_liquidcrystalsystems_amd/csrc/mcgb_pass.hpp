@@ -144,7 +144,9 @@ struct PassLds {
     __host__ __device__ static inline int list_stride(int cap) { return list_cap(cap) + 8; }
     // byte offsets: stencil tables 0, positions kHeader, orientations, lists, draws
     // (16-aligned); 12 * stage_cap is a multiple of 768, so every region keeps its alignment
-    static constexpr int kHeader = 1024;                 // s_start[128], s_off[128]
+    // s_start[126], s_off[126]: the positions begin below 1024 bytes, so a record's address
+    // offsets fit the 8-bit dword fields of the paired LDS loads (no address add per load)
+    static constexpr int kHeader = 1008;
     __host__ __device__ static inline int off_su(int cap) { return kHeader + 12 * stage_cap(cap); }
     __host__ __device__ static inline int off_list(int cap) {
         return off_su(cap) + cap * 3 * (int)sizeof(Real);
@@ -466,8 +468,8 @@ __device__ __forceinline__ void pass_cell(const PassArgs& P, int b, unsigned cha
     using L = PassLds<Real, W>;
     const int lcap = L::list_cap(cap);
 
-    int* s_start = (int*)smem;                                 // 128 (<= 125 used)
-    int* s_off = s_start + 128;                                // 128 (<= 126 used)
+    int* s_start = (int*)smem;                                 // 126 (<= 125 used)
+    int* s_off = s_start + 126;                                // 126 (<= 126 used)
     using R3 = Vec3<Real>;
     Pos3* sq = (Pos3*)(smem + L::kHeader);                     // stage_cap positions
     R3* su = (R3*)(smem + L::off_su(cap));                     // cap orientations

@@ -916,6 +916,9 @@ __device__ __forceinline__ void pass_cell(const PassArgs& P, int b, unsigned cha
                         // interleave; each lane adds its terms in item order, as one item per
                         // step would.  A missing second item repeats the first and is dropped.
                         const int items = 2 * base;
+                        // a lane's items all hold one state (item parity = lane parity): its
+                        // capped items are counted, and signed once after the loop
+                        int ncap = 0;
                         for (int it = lane; it < items; it += 128) {
                             const bool hb = it + 64 < items;
                             const int ea = slist[it >> 1], eb = slist[(hb ? it + 64 : it) >> 1];
@@ -938,11 +941,12 @@ __device__ __forceinline__ void pass_cell(const PassArgs& P, int b, unsigned cha
                             const Real ta = fa ? (nw ? v2[0] : -v2[0]) : (Real)0;
                             const Real tb = fb ? (nw ? v2[1] : -v2[1]) : (Real)0;
                             my_pairs += (int)ia + (int)ib;
-                            cpl += ((int)(ia & c2[0]) + (int)(ib & c2[1])) * (nw ? 1 : -1);
+                            ncap += (int)(ia & c2[0]) + (int)(ib & c2[1]);
                             acc.add(ta);
                             acc.add(tb);
                             if (TR) scl += (ta < 0 ? -ta : ta) + (tb < 0 ? -tb : tb);
                         }
+                        cpl += (lane & 1) ? ncap : -ncap;
                     } else if (!staged) {
                         // HBM path (stencil beyond the staging capacity): both states per
                         // candidate, no compaction

@@ -559,8 +559,10 @@ def main():
         tune_s = args.schedule
         if not args.waves:
             best, sched, ms = ctx.autotune(4)
-            tune = {"waves_per_cell": best, "schedule": SCHEDULES[sched], "ms_per_pass": ms}
+            tune = {"waves_per_cell": best, "schedule": SCHEDULES[sched], "ms_per_pass": ms,
+                    "pace_pct": ctx.pace()}
             tune_w, tune_s = best, sched
+        pace_pct = ctx.pace()
         if args.warmup:
             ctx.sweep(args.warmup)
         ctx.set_kernel_timing(False)   # the timed sweeps: graph replays only
@@ -617,6 +619,7 @@ def main():
         "energy_total": st["energy"],
         "relax_s": t_relax,
         "autotune": tune,
+        "pace_pct": pace_pct,
     }
     pairs_plain = None
     if not args.no_ess or not args.no_da:

@@ -237,13 +237,24 @@ int mcgb_series_clear(mcgb_ctx *ctx);
 /* Autotuner: times `sweeps` sweeps with each pass schedule (0 sequential, 1 speculative,
  * 2 sequential built for four wavefronts per SIMD, 3 two moves in flight) and each supported
  * number of wavefronts per active cell (1, 2, 3, 4, 8; schedule 3: 2 only; the LDS staging
- * capacity follows) and keeps the fastest.  The timed sweeps are ordinary sweeps of the
+ * capacity follows) and keeps the fastest; with schedule 0 kept it then re-checks the pacing
+ * (mcgb_set_pace, one warm-up and `sweeps` timed sweeps per setting).  The timed sweeps are ordinary sweeps of the
  * simulation (they advance it).  No counterpart in the reference.  ms_per_pass (may be NULL)
  * receives the measured pass times, index 5*schedule + {0,1,2,3,4} for 1, 2, 3, 4, 8
  * wavefronts (0 for a combination it does not run: the speculative schedule with
  * sublattice_colors 3, schedule 2 in FP64, schedule 3 with other than 2 wavefronts). */
 int mcgb_autotune(mcgb_ctx *ctx, int sweeps, int *best_waves, int *best_schedule,
                   double ms_per_pass[20]);
+
+/* Pacing of the sequential pass schedule (schedule 0): the cells of a pass run side by side and
+ * the pass lasts as long as its slowest cell, so a cell whose move loop falls behind `percent`
+ * of the last sweep's mean time per move raises its wavefronts' issue priority over the cells
+ * sharing their SIMDs (in steps of half a move).  Scheduling only: the chain and every result
+ * are the same at any setting.  0 turns it off; the default is 106, and mcgb_autotune, when it
+ * keeps schedule 0, re-checks off, 106 and 112 and keeps the fastest.  No counterpart in the
+ * reference (its cells run one after another on a CPU core). */
+int mcgb_set_pace(mcgb_ctx *ctx, int percent);
+int mcgb_get_pace(mcgb_ctx *ctx, int *percent);
 
 /* Average device time per launch of the sweep-pass kernel, the cell-list rebuild and the whole
  * sweep, measured with HIP events on the context's stream over the last mcgb_sweep() call. */

@@ -29,6 +29,11 @@ namespace {
 
 constexpr double kPi = 3.14159265358979323846;
 constexpr int kSeriesCapDefault = 1 << 16;
+// Sequential-pass pacing (k_pass): a cell whose move loop falls behind kPaceDefault percent of
+// the last sweep's mean time per move raises its wavefronts' issue priority.  At N = 256000
+// (1331 two-wavefront cells per pass, up to three wavefronts per SIMD) 100% and below slowed
+// the pass, 103-112% sped it up by 3-5%; with at most two wavefronts per SIMD it changes nothing.
+constexpr int kPaceDefault = 106;
 constexpr int kEvSet = 27 + 3;     // events of one timed sweep: start, rebuild, passes, end
 
 struct Events {
@@ -67,6 +72,7 @@ struct mcgb_ctx {
     bool o4 = false;           // FP32 sequential pass built for four wavefronts per SIMD
     bool duo = false;          // two moves in flight per cell (two wavefronts), traced or not
     int resum_interval = 100;
+    int pace_pct = kPaceDefault;   // sequential-pass pacing, percent of the mean move time (0: off)
     int series_cap = kSeriesCapDefault;
     bool any_npt = false;
     hipStream_t st = nullptr;
@@ -432,7 +438,7 @@ int enqueue_one(mcgb_ctx* c, mcgb_move_trace* trace, const hipEvent_t* ev) {
     HIPCHK(hipGetLastError());
     k_end<Real><<<dim3(kEndParts, c->nrep), 256, 0, c->st>>>(
         c->dC, c->dD, c->acc, (const Real*)c->u[b][0], (const Real*)c->u[b][1],
-        (const Real*)c->u[b][2], c->endp, c->end_ticket);
+        (const Real*)c->u[b][2], c->endp, c->end_ticket, c->pace_pct);
     // the re-summation (when flagged), the NPT volume move, the sweep's record: the record goes
     // with the energy reduction when no volume move follows
     if ((rc = enqueue_energy<Real>(c, 1, !c->any_npt))) return rc;
@@ -1460,6 +1466,14 @@ int mcgb_series_clear(mcgb_ctx* c) {
     return write_dyn(c, D);
 }
 
+// The pacing percentage is a kernel argument of k_end: a captured sweep graph holds it.
+static void set_pace_pct(mcgb_ctx* c, int pct) {
+    if (pct == c->pace_pct) return;
+    for (int k = 0; k < 2; ++k)
+        if (c->graph[k]) { (void)hipGraphExecDestroy(c->graph[k]); c->graph[k] = nullptr; }
+    c->pace_pct = pct;
+}
+
 static int set_waves(mcgb_ctx* c, int waves) {
     for (int k = 0; k < 2; ++k)
         if (c->graph[k]) { (void)hipGraphExecDestroy(c->graph[k]); c->graph[k] = nullptr; }
@@ -1511,9 +1525,41 @@ int mcgb_autotune(mcgb_ctx* c, int sweeps, int* best_waves, int* best_schedule,
     c->o4 = bs == 2;
     c->duo = bs == 3;
     set_waves(c, bw);
+    // the sequential schedule's pacing: off, the default and one step above it, each after a
+    // sweep that sets the pace from the cells' times under that setting
+    if (bs == 0) {
+        const int pcts[3] = {0, kPaceDefault, kPaceDefault + 6};
+        double pbest = 1e300;
+        int pb = c->pace_pct;
+        for (int i = 0; i < 3; ++i) {
+            set_pace_pct(c, pcts[i]);
+            if ((rc = sweep_impl(c, 1, nullptr)) || (rc = sweep_impl(c, sweeps, nullptr))) {
+                c->timing = tm;
+                return rc;
+            }
+            if (c->ms_pass < pbest) { pbest = c->ms_pass; pb = pcts[i]; }
+        }
+        set_pace_pct(c, pb);
+    }
     c->timing = tm;
     if (best_waves) *best_waves = bw;
     if (best_schedule) *best_schedule = bs;
+    return MCGB_OK;
+}
+
+int mcgb_set_pace(mcgb_ctx* c, int percent) {
+    if (!c) { mcgb_set_error("null context"); return MCGB_ERR_ARG; }
+    if (percent < 0 || percent > 1000) {
+        mcgb_set_error("mcgb_set_pace: percent must be in [0, 1000] (got %d)", percent);
+        return MCGB_ERR_ARG;
+    }
+    set_pace_pct(c, percent);
+    return MCGB_OK;
+}
+
+int mcgb_get_pace(mcgb_ctx* c, int* percent) {
+    if (!c || !percent) { mcgb_set_error("null argument"); return MCGB_ERR_ARG; }
+    *percent = c->pace_pct;
     return MCGB_OK;
 }
 

@@ -84,6 +84,9 @@ struct RepDyn {
     unsigned long long resums_forced;   // re-summations forced by a sweep's moved energy
     double qsum[6];              // sum over particles of u_a u_b (xx, yy, zz, xy, xz, yz) after
                                  // the passes of the last sweep (order-parameter record)
+    int pace;                    // sequential pass: the pace of its cells' moves, 10 ns ticks
+                                 // per move (from the last sweep; 0: no pacing)
+    int pad2;
 };
 
 // Draws of one trial move (draw_move): particle selector, fixed-point step, rotation vector,
@@ -117,6 +120,7 @@ struct CellAcc {
     long long cand;
     double dfin, moved_abs;
     int s1rej, s2rej;          // delayed acceptance: rejections by the surrogate / exact stage
+    int ticks, tmoves;         // sequential pass: the move loop's 10 ns ticks and moves (pacing)
 };
 
 // ------------------------------------------------------------------ Philox4x32-10

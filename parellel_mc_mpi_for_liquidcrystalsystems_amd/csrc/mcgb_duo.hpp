@@ -627,6 +627,7 @@ __device__ __forceinline__ void pass_cell_duo(const PassArgs& P, int b, unsigned
         ca.dfin = sred[0].d + sred[1].d;
         ca.moved_abs = sred[0].scale + sred[1].scale;
         ca.s1rej = 0; ca.s2rej = 0;
+        ca.ticks = 0; ca.tmoves = 0;
         P.acc[C.cell_off + cell] = ca;
     }
 }

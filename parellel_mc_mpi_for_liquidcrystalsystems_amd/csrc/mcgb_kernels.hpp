@@ -448,13 +448,15 @@ __global__ __launch_bounds__(256) void k_count_draws(const RepConst* __restrict_
 // of cell chunk b and the orientation products u_a u_b of particles b*256 + tid + j*kEndParts*256
 // into a partial; the last block of the replica to finish (a ticket) folds the kEndParts
 // partials into the running energy and counters, runs the step-size controller, flags an FP64
-// re-summation when needed and keeps the orientation sums for the sweep's record.  Every sum
-// runs in a fixed order (four particles' loads issued together, lane butterflies, then the
-// wavefronts in order): deterministic.
+// re-summation when needed, keeps the orientation sums for the sweep's record and sets the
+// pace of the next sweep's sequential passes (pace_pct percent of this sweep's mean move-loop
+// time per move; 0: no pacing).  Every sum runs in a fixed order (four particles' loads issued
+// together, lane butterflies, then the wavefronts in order): deterministic.
 constexpr int kEndParts = 256;
-constexpr int kEndVals = 19;       // 11 counters, finite dE, moved |dE|, 6 orientation sums
+constexpr int kEndCnt = 13;        // 11 move counters, move-loop ticks and moves
+constexpr int kEndVals = kEndCnt + 8;   // and finite dE, moved |dE|, 6 orientation sums
 struct EndPartial {
-    long long v[11];
+    long long v[kEndCnt];
     double d[8];                   // finite dE, moved |dE|, orientation sums
 };
 
@@ -464,7 +466,7 @@ __global__ __launch_bounds__(256) void k_end(const RepConst* __restrict__ RC,
                                              const CellAcc* __restrict__ acc,
                                              const Real* __restrict__ ux, const Real* __restrict__ uy,
                                              const Real* __restrict__ uz, EndPartial* part,
-                                             int* __restrict__ ticket) {
+                                             int* __restrict__ ticket, int pace_pct) {
     // the 19 sums travel as doubles: the counters are integers far below 2^53, exact in FP64,
     // so one fixed-order DPP reduction (result in lane 63) serves them all
     __shared__ double sv[4][kEndVals];
@@ -476,13 +478,14 @@ __global__ __launch_bounds__(256) void k_end(const RepConst* __restrict__ RC,
     const int chunk = (ncell + kEndParts - 1) / kEndParts;
     const int c0 = blockIdx.x * chunk, c1 = min(ncell, c0 + chunk);
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    long long v[11] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    long long v[kEndCnt] = {};
     double d[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     for (int c = c0 + tid; c < c1; c += 256) {
         const CellAcc& A = acc[C.cell_off + c];
         v[0] += A.att; v[1] += A.acc; v[2] += A.left; v[3] += A.hard;
         v[4] += A.cand; v[5] += A.pairs; v[6] += A.coll; v[7] += A.dcap; v[8] += A.dccap;
         v[9] += A.s1rej; v[10] += A.s2rej;
+        v[11] += A.ticks; v[12] += A.tmoves;
         d[0] += A.dfin;
         d[1] += A.moved_abs;
     }
@@ -505,17 +508,18 @@ __global__ __launch_bounds__(256) void k_end(const RepConst* __restrict__ RC,
     }
     double e[kEndVals];
 #pragma unroll
-    for (int k = 0; k < 11; ++k) e[k] = wave_sum_d63((double)v[k]);
+    for (int k = 0; k < kEndCnt; ++k) e[k] = wave_sum_d63((double)v[k]);
 #pragma unroll
-    for (int k = 0; k < 8; ++k) e[11 + k] = wave_sum_d63(d[k]);
+    for (int k = 0; k < 8; ++k) e[kEndCnt + k] = wave_sum_d63(d[k]);
     if (lane == 63)
         for (int k = 0; k < kEndVals; ++k) sv[w][k] = e[k];
     __syncthreads();
     if (tid == 0) {
         EndPartial& P = part[(size_t)r * kEndParts + blockIdx.x];
-        for (int k = 0; k < 11; ++k)
+        for (int k = 0; k < kEndCnt; ++k)
             P.v[k] = (long long)sv[0][k] + (long long)sv[1][k] + (long long)sv[2][k] + (long long)sv[3][k];
-        for (int k = 0; k < 8; ++k) P.d[k] = ((sv[0][11 + k] + sv[1][11 + k]) + sv[2][11 + k]) + sv[3][11 + k];
+        for (int k = 0; k < 8; ++k)
+            P.d[k] = ((sv[0][kEndCnt + k] + sv[1][kEndCnt + k]) + sv[2][kEndCnt + k]) + sv[3][kEndCnt + k];
         __threadfence();                                 // the partial before the ticket
         s_last = atomicAdd(&ticket[r], 1) == kEndParts - 1;
     }
@@ -524,19 +528,19 @@ __global__ __launch_bounds__(256) void k_end(const RepConst* __restrict__ RC,
     __threadfence();                                     // every partial after the ticket
     // partials lane, lane + 64, ... in order per lane, then one DPP reduction per sum
     static_assert(kEndParts % 64 == 0, "whole partials per lane");
-    for (int k = 0; k < 11; ++k) v[k] = 0;
+    for (int k = 0; k < kEndCnt; ++k) v[k] = 0;
     for (int k = 0; k < 8; ++k) d[k] = 0.0;
     for (int j = lane; j < kEndParts; j += 64) {
         const EndPartial& P = part[(size_t)r * kEndParts + j];
-        for (int k = 0; k < 11; ++k) v[k] += P.v[k];
+        for (int k = 0; k < kEndCnt; ++k) v[k] += P.v[k];
         for (int k = 0; k < 8; ++k) d[k] += P.d[k];
     }
 #pragma unroll
-    for (int k = 0; k < 11; ++k) e[k] = wave_sum_d63((double)v[k]);
+    for (int k = 0; k < kEndCnt; ++k) e[k] = wave_sum_d63((double)v[k]);
 #pragma unroll
-    for (int k = 0; k < 8; ++k) e[11 + k] = wave_sum_d63(d[k]);
+    for (int k = 0; k < 8; ++k) e[kEndCnt + k] = wave_sum_d63(d[k]);
     if (lane != 63) return;
-    for (int k = 0; k < 11; ++k) v[k] = (long long)e[k];
+    for (int k = 0; k < kEndCnt; ++k) v[k] = (long long)e[k];
     ticket[r] = 0;                                       // for the next sweep
     D.attempted += v[0]; D.accepted += v[1]; D.left += v[2]; D.hard += v[3];
     D.cand += v[4]; D.pairs += v[5]; D.coll += v[6];
@@ -544,9 +548,10 @@ __global__ __launch_bounds__(256) void k_end(const RepConst* __restrict__ RC,
     D.cap_coll += v[8];
     D.da_s1 += v[9];
     D.da_s2 += v[10];
-    D.e_fin += e[11];
-    D.moved_abs = e[12];
-    if (e[12] > 1.0e10) D.resum = 1;
+    D.e_fin += e[kEndCnt];
+    D.moved_abs = e[kEndCnt + 1];
+    if (e[kEndCnt + 1] > 1.0e10) D.resum = 1;
+    D.pace = (pace_pct > 0 && v[12] > 0) ? (int)(v[11] * pace_pct / (100 * v[12])) : 0;
     if (C.adapt_moves > 0) {
         D.ad_att += (unsigned long long)v[0];
         D.ad_acc += (unsigned long long)v[1];
@@ -557,7 +562,7 @@ __global__ __launch_bounds__(256) void k_end(const RepConst* __restrict__ RC,
             D.adapts += 1;
         }
     }
-    for (int k = 0; k < 6; ++k) D.qsum[k] = e[13 + k];
+    for (int k = 0; k < 6; ++k) D.qsum[k] = e[kEndCnt + 2 + k];
 }
 
 // ------------------------------------------------------------------ FP64 full energy

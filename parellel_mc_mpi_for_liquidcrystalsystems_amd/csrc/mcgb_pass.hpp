@@ -656,6 +656,10 @@ __device__ __forceinline__ void pass_cell(const PassArgs& P, int b, unsigned cha
             scc[j] = cj ? 1 : 0;
         }
 
+    // pacing of the sequential pass: its move loop timed on the 100 MHz clock, the pace set by
+    // k_end from the last sweep's cells
+    const int pace = (!GLOBAL && !TR && !DA && !O4) ? uni(D.pace) : 0;
+    const unsigned long long pace0 = (!GLOBAL && !TR && !DA && !O4) ? __builtin_amdgcn_s_memrealtime() : 0ull;
     if (nc == 0) {
         if (tracing)
             for (int m = tid; m < k; m += 64 * W) {
@@ -722,6 +726,18 @@ __device__ __forceinline__ void pass_cell(const PassArgs& P, int b, unsigned cha
             const int mend = min(k, m0 + KD);
             for (int m = m0; m < mend; ++m) {
                 const int ml = m - m0;
+                if (pace > 0) {
+                    // a cell behind the pace (m moves done by m * pace ticks after the first)
+                    // takes issue priority over the cells sharing its SIMDs, in steps of half
+                    // a move: the slowest cells set the pass time, the others have slack.
+                    // Scheduling only: the chain is the same at every priority.
+                    const long long el = (long long)(__builtin_amdgcn_s_memrealtime() - pace0);
+                    const long long bh = el - (long long)m * pace;
+                    if (bh > pace / 2) __builtin_amdgcn_s_setprio(3);
+                    else if (bh > 0) __builtin_amdgcn_s_setprio(2);
+                    else if (bh > -pace / 2) __builtin_amdgcn_s_setprio(1);
+                    else __builtin_amdgcn_s_setprio(0);
+                }
                 // the whole record in one batch of LDS loads (reading fields where they are
                 // used puts LDS latency on the move's critical path)
                 const MovePrep<Real> pm = sprep[ml];
@@ -1065,6 +1081,9 @@ __device__ __forceinline__ void pass_cell(const PassArgs& P, int b, unsigned cha
         ca.dcap = dcap_sum; ca.dccap = dccap_sum; ca.pairs = np; ca.coll = n_coll;
         ca.cand = n_cand; ca.dfin = dfin_sum; ca.moved_abs = moved;
         ca.s1rej = n_s1; ca.s2rej = n_s2;
+        ca.ticks = (!GLOBAL && !TR && !DA && !O4 && nc > 0)
+                       ? (int)(__builtin_amdgcn_s_memrealtime() - pace0) : 0;
+        ca.tmoves = (!GLOBAL && !TR && !DA && !O4 && nc > 0) ? k : 0;
         P.acc[C.cell_off + cell] = ca;
     }
 }

@@ -423,6 +423,7 @@ __global__ __launch_bounds__(64 * W) void k_pass_spec(PassArgs P) {
         ca.dcap = dcap_sum; ca.dccap = dccap_sum; ca.pairs = np; ca.coll = n_coll;
         ca.cand = n_cand; ca.dfin = dfin_sum; ca.moved_abs = moved;
         ca.s1rej = 0; ca.s2rej = 0;
+        ca.ticks = 0; ca.tmoves = 0;
         P.acc[C.cell_off + cell] = ca;
     }
 }

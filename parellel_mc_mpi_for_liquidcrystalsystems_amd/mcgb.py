@@ -132,6 +132,8 @@ def lib():
         "mcgb_series_clear": (C.c_int, [ctx]),
         "mcgb_kernel_times": (C.c_int, [ctx, dp, dp, dp]),
         "mcgb_autotune": (C.c_int, [ctx, C.c_int, ip, ip, dp]),
+        "mcgb_set_pace": (C.c_int, [ctx, C.c_int]),
+        "mcgb_get_pace": (C.c_int, [ctx, ip]),
         "mcgb_cell_list": (C.c_int, [ctx, C.c_int, C.c_uint32, ip, C.c_int, ip, C.c_int, up, up, ip]),
         "mcgb_pair_energy_batch": (C.c_int, [C.POINTER(Params), C.c_int, dp, dp, dp, C.c_int, dp, ip]),
         "mcgb_colloid_energy_batch": (C.c_int, [C.POINTER(Params), C.c_int, dp, dp, C.c_int, dp, ip]),
@@ -508,6 +510,15 @@ class Context:
         check(lib().mcgb_autotune(self.h, sweeps, C.byref(best), C.byref(sched), _d(ms)))
         keys = [f"{s}/{w}" for s in ("seq", "spec", "seq4", "duo") for w in (1, 2, 3, 4, 8)]
         return best.value, sched.value, dict(zip(keys, ms.tolist()))
+
+    def set_pace(self, percent):
+        """Sequential-pass pacing in percent of the last sweep's mean move time (0: off)."""
+        check(lib().mcgb_set_pace(self.h, int(percent)))
+
+    def pace(self):
+        p = C.c_int()
+        check(lib().mcgb_get_pace(self.h, C.byref(p)))
+        return p.value
 
     def set_kernel_timing(self, on=True):
         check(lib().mcgb_set_kernel_timing(self.h, 1 if on else 0))

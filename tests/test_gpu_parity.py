@@ -420,7 +420,9 @@ def test_autotune_keeps_results_and_picks_a_supported_width():
         run = [k for k in ms if not k.startswith("seq4/") and (not k.startswith("duo/") or k == "duo/2")]
         assert all(ms[k] > 0 for k in run)
         assert all(ms[k] == 0 for k in ms if k not in run)    # seq4: FP32 only; duo: W = 2 only
-        for _ in range((2 * 5 + 1) * 2):             # 2 schedules x 5 widths, duo at W = 2; 2 sweeps
+        # 2 schedules x 5 widths, duo at W = 2; 2 sweeps each; with the sequential schedule kept,
+        # three pacing settings of one warm-up and 2 timed sweeps each
+        for _ in range((2 * 5 + 1) * 2 + (3 * (1 + 2) if sched == 0 else 0)):
             s.sweep()
         q, u, _ = ctx.get_state_fixed()
         for d in range(3):

@@ -724,6 +724,13 @@ __device__ __forceinline__ void pass_cell(const PassArgs& P, int b, unsigned cha
             __syncthreads();
             uint32_t acc_mask = 0;            // moves of this chunk accepted so far
             const int mend = min(k, m0 + KD);
+            // FP32 sequential build: the next move's record read before the barrier that ends
+            // a move (the records do not change during the chunk), its LDS latency overlapping
+            // the barrier (18 more VGPRs, still three wavefronts per SIMD; the other builds
+            // have no room for them)
+            constexpr bool kPre = std::is_same<Real, float>::value && !O4 && !DA;
+            MovePrep<Real> pm_next;
+            if (kPre) pm_next = sprep[0];
             for (int m = m0; m < mend; ++m) {
                 const int ml = m - m0;
                 if (pace > 0) {
@@ -740,7 +747,7 @@ __device__ __forceinline__ void pass_cell(const PassArgs& P, int b, unsigned cha
                 }
                 // the whole record in one batch of LDS loads (reading fields where they are
                 // used puts LDS latency on the move's critical path)
-                const MovePrep<Real> pm = sprep[ml];
+                const MovePrep<Real> pm = kPre ? pm_next : sprep[ml];
                 const int jsel = uni(pm.jsel);
                 // the chunk's preparation of this move holds unless an earlier accepted move
                 // of the chunk moved the same particle
@@ -1064,6 +1071,7 @@ __device__ __forceinline__ void pass_cell(const PassArgs& P, int b, unsigned cha
                     rec.scale = scale;
                     P.trace[tbase + m] = rec;
                 }
+                if (kPre && m + 1 < mend) pm_next = sprep[ml + 1];
                 __syncthreads();
             }
         }

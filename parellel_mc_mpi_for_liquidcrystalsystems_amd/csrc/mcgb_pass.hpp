@@ -733,18 +733,6 @@ __device__ __forceinline__ void pass_cell(const PassArgs& P, int b, unsigned cha
             if (kPre) pm_next = sprep[0];
             for (int m = m0; m < mend; ++m) {
                 const int ml = m - m0;
-                if (pace > 0) {
-                    // a cell behind the pace (m moves done by m * pace ticks after the first)
-                    // takes issue priority over the cells sharing its SIMDs, in steps of half
-                    // a move: the slowest cells set the pass time, the others have slack.
-                    // Scheduling only: the chain is the same at every priority.
-                    const long long el = (long long)(__builtin_amdgcn_s_memrealtime() - pace0);
-                    const long long bh = el - (long long)m * pace;
-                    if (bh > pace / 2) __builtin_amdgcn_s_setprio(3);
-                    else if (bh > 0) __builtin_amdgcn_s_setprio(2);
-                    else if (bh > -pace / 2) __builtin_amdgcn_s_setprio(1);
-                    else __builtin_amdgcn_s_setprio(0);
-                }
                 // the whole record in one batch of LDS loads (reading fields where they are
                 // used puts LDS latency on the move's critical path)
                 const MovePrep<Real> pm = kPre ? pm_next : sprep[ml];
@@ -1072,6 +1060,20 @@ __device__ __forceinline__ void pass_cell(const PassArgs& P, int b, unsigned cha
                     P.trace[tbase + m] = rec;
                 }
                 if (kPre && m + 1 < mend) pm_next = sprep[ml + 1];
+                if (pace > 0) {
+                    // pacing: a cell behind the pace (m + 1 moves done by (m + 1) * pace ticks
+                    // after the first) takes issue priority for its next move over the cells
+                    // sharing its SIMDs, in steps of half a move: the slowest cells set the pass
+                    // time, the others have slack.  The clock is read before the barrier, so
+                    // its latency overlaps the barrier.  Scheduling only: the chain is the same
+                    // at every priority.
+                    const long long el = (long long)(__builtin_amdgcn_s_memrealtime() - pace0);
+                    const long long bh = el - (long long)(m + 1) * pace;
+                    if (bh > pace / 2) __builtin_amdgcn_s_setprio(3);
+                    else if (bh > 0) __builtin_amdgcn_s_setprio(2);
+                    else if (bh > -pace / 2) __builtin_amdgcn_s_setprio(1);
+                    else __builtin_amdgcn_s_setprio(0);
+                }
                 __syncthreads();
             }
         }

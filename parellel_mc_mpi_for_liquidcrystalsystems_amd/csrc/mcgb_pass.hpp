@@ -726,11 +726,17 @@ __device__ __forceinline__ void pass_cell(const PassArgs& P, int b, unsigned cha
             const int mend = min(k, m0 + KD);
             // FP32 sequential build: the next move's record read before the barrier that ends
             // a move (the records do not change during the chunk), its LDS latency overlapping
-            // the barrier (18 more VGPRs, still three wavefronts per SIMD; the other builds
-            // have no room for them)
-            constexpr bool kPre = std::is_same<Real, float>::value && !O4 && !DA;
+            // the barrier (with the group below, 168 VGPRs: still three wavefronts per SIMD; the
+            // other builds, and eight wavefronts per cell, have no room for them)
+            constexpr bool kPre = std::is_same<Real, float>::value && !O4 && !DA && W <= 4;
             MovePrep<Real> pm_next;
             if (kPre) pm_next = sprep[0];
+            // and, read at the same point, the first position group of the next move's scan
+            // (group w: it does not depend on the move); each wavefront writes an accepted
+            // move's new position itself before reading it
+            const int ngs = S_pad >> 6;
+            Pos3 pre0;
+            if (kPre && staged) pre0 = sq[min(w, ngs - 1) * 64 + lane];
             for (int m = m0; m < mend; ++m) {
                 const int ml = m - m0;
                 // the whole record in one batch of LDS loads (reading fields where they are
@@ -812,9 +818,16 @@ __device__ __forceinline__ void pass_cell(const PassArgs& P, int b, unsigned cha
                                 // the last one again): no branch between them, so they are in
                                 // flight together
                                 Pos3 p[kG];
+                                if (kPre && g0 == w) {          // uniform
+                                    p[0] = pre0;
 #pragma unroll
-                                for (int j = 0; j < kG; ++j)
-                                    p[j] = sq[min(g0 + j * W, ng - 1) * 64 + lane];
+                                    for (int j = 1; j < kG; ++j)
+                                        p[j] = sq[min(g0 + j * W, ng - 1) * 64 + lane];
+                                } else {
+#pragma unroll
+                                    for (int j = 0; j < kG; ++j)
+                                        p[j] = sq[min(g0 + j * W, ng - 1) * 64 + lane];
+                                }
 #pragma unroll
                                 for (int j = 0; j < kG; ++j)
                                     if (j < nv) {
@@ -1033,11 +1046,13 @@ __device__ __forceinline__ void pass_cell(const PassArgs& P, int b, unsigned cha
                         dfin_sum += dfin;
                         moved += fabs(dfin);
                         acc_mask = uni(acc_mask | (1u << ml));
+                        // the staged state: by every wavefront when it reads the next move's
+                        // first scan groups ahead of the barrier (the same bytes)
+                        if (staged && (kPre ? lane == 0 : tid == 0)) {
+                            sq[li] = Pos3{qnx, qny, qnz};
+                            su[li] = R3{unx, uny, unz};
+                        }
                         if (tid == 0) {
-                            if (staged) {
-                                sq[li] = Pos3{qnx, qny, qnz};
-                                su[li] = R3{unx, uny, unz};
-                            }
                             if (ccache) {
                                 scache[jsel] = cen;
                                 scc[jsel] = ccn ? 1 : 0;
@@ -1060,6 +1075,7 @@ __device__ __forceinline__ void pass_cell(const PassArgs& P, int b, unsigned cha
                     P.trace[tbase + m] = rec;
                 }
                 if (kPre && m + 1 < mend) pm_next = sprep[ml + 1];
+                if (kPre && staged) pre0 = sq[min(w, ngs - 1) * 64 + lane];
                 if (pace > 0) {
                     // pacing: a cell behind the pace (m + 1 moves done by (m + 1) * pace ticks
                     // after the first) takes issue priority for its next move over the cells
@@ -1098,11 +1114,12 @@ __device__ __forceinline__ void pass_cell(const PassArgs& P, int b, unsigned cha
     }
 }
 
-// At most three wavefronts per SIMD: with more allowed, the 1331 workgroups of a pass at
-// N = 256k (W = 2) run 10% slower (measured); schedule 2 (k_pass_o4) is the four-wavefront
-// build for workloads that gain from it.
+// Three wavefronts per SIMD, no more and no fewer (at most 168 VGPRs): with more allowed, the
+// 1331 workgroups of a pass at N = 256k (W = 2) run 10% slower (measured), and the FP32 build
+// with its prefetched record and scan group would otherwise take 170 VGPRs (two per SIMD);
+// schedule 2 (k_pass_o4) is the four-wavefront build for workloads that gain from it.
 template <typename Real, int W, bool GLOBAL, bool TR, bool DA = false>
-__global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(1, 3))) void k_pass(PassArgs P) {
+__global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(3, 3))) void k_pass(PassArgs P) {
     extern __shared__ __align__(16) unsigned char smem[];
     if (GLOBAL) {
         const int nov = P.ovf_count[P.pass];

@@ -503,6 +503,9 @@ def main():
     ap.add_argument("--relax", type=int, default=200,
                     help="untimed sweeps at 0.1/0.1 after the overlap-removal sweeps")
     ap.add_argument("--waves", type=int, default=0, help="wavefronts per cell (0: autotune)")
+    ap.add_argument("--pace", type=int, default=-1,
+                    help="sequential-pass pacing in percent of the mean move time (0: off; "
+                         "default: the library's, re-checked by the autotuner)")
     ap.add_argument("--schedule", type=int, default=1,
                     help="pass schedule with --waves: 0 sequential, 1 speculative, "
                          "2 sequential built for four wavefronts per SIMD, 3 two moves in "
@@ -543,6 +546,8 @@ def main():
         p.waves_per_cell = args.waves
         p.pass_schedule = args.schedule
     with m.Context(p) as ctx:
+        if args.pace >= 0:
+            ctx.set_pace(args.pace)
         ctx.set_state(pos, u, box)
         t_relax = time.perf_counter()
         ctx.set_kernel_timing(False)

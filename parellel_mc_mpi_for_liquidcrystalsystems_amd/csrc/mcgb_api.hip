@@ -31,8 +31,9 @@ constexpr double kPi = 3.14159265358979323846;
 constexpr int kSeriesCapDefault = 1 << 16;
 // Sequential-pass pacing (k_pass): a cell whose move loop falls behind kPaceDefault percent of
 // the last sweep's mean time per move raises its wavefronts' issue priority.  At N = 256000
-// (1331 two-wavefront cells per pass, up to three wavefronts per SIMD) 100% and below slowed
-// the pass, 103-112% sped it up by 3-5%; with at most two wavefronts per SIMD it changes nothing.
+// (1331 two-wavefront cells per pass, up to three wavefronts per SIMD) 95% and below slowed the
+// pass, 100% gave +3%, 106-109% +7-8%, 115% +5%; with at most two wavefronts per SIMD it
+// changes nothing.
 constexpr int kPaceDefault = 106;
 constexpr int kEvSet = 27 + 3;     // events of one timed sweep: start, rebuild, passes, end
 
@@ -1528,7 +1529,7 @@ int mcgb_autotune(mcgb_ctx* c, int sweeps, int* best_waves, int* best_schedule,
     // the sequential schedule's pacing: off, the default and one step above it, each after a
     // sweep that sets the pace from the cells' times under that setting
     if (bs == 0) {
-        const int pcts[3] = {0, kPaceDefault, kPaceDefault + 6};
+        const int pcts[3] = {0, kPaceDefault, kPaceDefault + 3};
         double pbest = 1e300;
         int pb = c->pace_pct;
         for (int i = 0; i < 3; ++i) {

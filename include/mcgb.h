@@ -247,11 +247,12 @@ int mcgb_autotune(mcgb_ctx *ctx, int sweeps, int *best_waves, int *best_schedule
                   double ms_per_pass[20]);
 
 /* Pacing of the sequential pass schedule (schedule 0): the cells of a pass run side by side and
- * the pass lasts as long as its slowest cell, so a cell whose move loop falls behind `percent`
- * of the last sweep's mean time per move raises its wavefronts' issue priority over the cells
- * sharing their SIMDs (in steps of half a move).  Scheduling only: the chain and every result
- * are the same at any setting.  0 turns it off; the default is 106, and mcgb_autotune, when it
- * keeps schedule 0, re-checks off, 106 and 109 and keeps the fastest.  No counterpart in the
+ * the pass lasts as long as its slowest cell, so a cell that falls behind `percent` of the last
+ * sweep's mean time per move (timed from the start of its workgroup) raises its wavefronts'
+ * issue priority over the cells sharing their SIMDs (in steps of half a move).  Scheduling
+ * only: the chain and every result are the same at any setting.  0 turns it off; the default is
+ * 109, and mcgb_autotune, when it keeps schedule 0, re-checks off, 109 and 112 and keeps the
+ * fastest.  No counterpart in the
  * reference (its cells run one after another on a CPU core). */
 int mcgb_set_pace(mcgb_ctx *ctx, int percent);
 int mcgb_get_pace(mcgb_ctx *ctx, int *percent);

@@ -29,12 +29,12 @@ namespace {
 
 constexpr double kPi = 3.14159265358979323846;
 constexpr int kSeriesCapDefault = 1 << 16;
-// Sequential-pass pacing (k_pass): a cell whose move loop falls behind kPaceDefault percent of
-// the last sweep's mean time per move raises its wavefronts' issue priority.  At N = 256000
-// (1331 two-wavefront cells per pass, up to three wavefronts per SIMD) 95% and below slowed the
-// pass, 100% gave +3%, 106-109% +7-8%, 115% +5%; with at most two wavefronts per SIMD it
-// changes nothing.
-constexpr int kPaceDefault = 106;
+// Sequential-pass pacing (k_pass): a cell that falls behind kPaceDefault percent of the last
+// sweep's mean time per move (its time counted from the start of its workgroup) raises its
+// wavefronts' issue priority.  At N = 256000 (1331 two-wavefront cells per pass, up to three
+// wavefronts per SIMD): 103% +6%, 109% +9%, 115% +8% over no pacing; with at most two
+// wavefronts per SIMD it changes nothing.
+constexpr int kPaceDefault = 109;
 constexpr int kEvSet = 27 + 3;     // events of one timed sweep: start, rebuild, passes, end
 
 struct Events {

@@ -120,7 +120,7 @@ struct CellAcc {
     long long cand;
     double dfin, moved_abs;
     int s1rej, s2rej;          // delayed acceptance: rejections by the surrogate / exact stage
-    int ticks, tmoves;         // sequential pass: the move loop's 10 ns ticks and moves (pacing)
+    int ticks, tmoves;         // sequential pass: the cell's 10 ns ticks and moves (pacing)
 };
 
 // ------------------------------------------------------------------ Philox4x32-10

@@ -449,8 +449,8 @@ __global__ __launch_bounds__(256) void k_count_draws(const RepConst* __restrict_
 // into a partial; the last block of the replica to finish (a ticket) folds the kEndParts
 // partials into the running energy and counters, runs the step-size controller, flags an FP64
 // re-summation when needed, keeps the orientation sums for the sweep's record and sets the
-// pace of the next sweep's sequential passes (pace_pct percent of this sweep's mean move-loop
-// time per move; 0: no pacing).  Every sum runs in a fixed order (four particles' loads issued
+// pace of the next sweep's sequential passes (pace_pct percent of this sweep's mean cell time
+// per move; 0: no pacing).  Every sum runs in a fixed order (four particles' loads issued
 // together, lane butterflies, then the wavefronts in order): deterministic.
 constexpr int kEndParts = 256;
 constexpr int kEndCnt = 13;        // 11 move counters, move-loop ticks and moves

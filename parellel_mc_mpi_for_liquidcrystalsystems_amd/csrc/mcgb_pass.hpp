@@ -528,6 +528,9 @@ __device__ __forceinline__ void pass_cell(const PassArgs& P, int b, unsigned cha
     uint2 dpre = make_uint2(0u, 0u);
     if (w == 0 && lane < min(k, KD) * kW2) dpre = drw2[lane];
 
+    // pacing of the sequential pass: the cell timed on the 100 MHz clock from the start of its
+    // workgroup (a slow staging counts against it), the pace set by k_end from the last sweep
+    const unsigned long long pace0 = (!GLOBAL && !TR && !DA && !O4) ? __builtin_amdgcn_s_memrealtime() : 0ull;
     // stencil table: NS = (2R+1)^3 neighbour cells in (dz, dy, dx) order, the active cell at
     // NS/2; two cells per lane, one inclusive scan per half
     const int RS = col - 1, S1 = 2 * RS + 1, NS = S1 * S1 * S1;
@@ -656,10 +659,7 @@ __device__ __forceinline__ void pass_cell(const PassArgs& P, int b, unsigned cha
             scc[j] = cj ? 1 : 0;
         }
 
-    // pacing of the sequential pass: its move loop timed on the 100 MHz clock, the pace set by
-    // k_end from the last sweep's cells
     const int pace = (!GLOBAL && !TR && !DA && !O4) ? uni(D.pace) : 0;
-    const unsigned long long pace0 = (!GLOBAL && !TR && !DA && !O4) ? __builtin_amdgcn_s_memrealtime() : 0ull;
     if (nc == 0) {
         if (tracing)
             for (int m = tid; m < k; m += 64 * W) {

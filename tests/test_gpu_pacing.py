@@ -36,10 +36,10 @@ def run(pace, precision, N, sweeps=6):
 
 @pytest.mark.parametrize("precision,N", [(32, 256000), (64, 64000)])
 def test_pacing_leaves_the_chain_unchanged(precision, N):
-    print(f"pacing: FP{precision} N {N}, pace 0 / 106 / 40", flush=True)
+    print(f"pacing: FP{precision} N {N}, pace 0 / 109 / 40", flush=True)
     q0, u0, s0 = run(0, precision, N)
     assert s0["attempted"] > 3 * N
-    for pace in (106, 40):
+    for pace in (109, 40):
         q, u, s = run(pace, precision, N)
         for d in range(3):
             assert np.array_equal(q[d], q0[d]), f"pace {pace}"
@@ -51,9 +51,9 @@ def test_pacing_leaves_the_chain_unchanged(precision, N):
 def test_pace_setting_is_checked():
     p = m.gb_params(4000, 0.32, 1.0, precision=32, seed=SEED)
     with m.Context(p) as ctx:
-        assert ctx.pace() == 106
+        assert ctx.pace() == 109
         with pytest.raises(Exception):
             ctx.set_pace(-1)
         with pytest.raises(Exception):
             ctx.set_pace(1001)
-        assert ctx.pace() == 106
+        assert ctx.pace() == 109

@@ -470,7 +470,6 @@ __global__ __launch_bounds__(256) void k_end(const RepConst* __restrict__ RC,
     // the 19 sums travel as doubles: the counters are integers far below 2^53, exact in FP64,
     // so one fixed-order DPP reduction (result in lane 63) serves them all
     __shared__ double sv[4][kEndVals];
-    __shared__ int s_last;
     const int r = blockIdx.y;
     const RepConst& C = RC[r];
     RepDyn& D = RD[r];
@@ -514,17 +513,20 @@ __global__ __launch_bounds__(256) void k_end(const RepConst* __restrict__ RC,
     if (lane == 63)
         for (int k = 0; k < kEndVals; ++k) sv[w][k] = e[k];
     __syncthreads();
-    if (tid == 0) {
+    if (w != 0) return;                                  // wavefront 0 writes the partial
+    {
+        // one sum per lane, the wavefronts in order
         EndPartial& P = part[(size_t)r * kEndParts + blockIdx.x];
-        for (int k = 0; k < kEndCnt; ++k)
-            P.v[k] = (long long)sv[0][k] + (long long)sv[1][k] + (long long)sv[2][k] + (long long)sv[3][k];
-        for (int k = 0; k < 8; ++k)
-            P.d[k] = ((sv[0][kEndCnt + k] + sv[1][kEndCnt + k]) + sv[2][kEndCnt + k]) + sv[3][kEndCnt + k];
+        if (lane < kEndCnt)
+            P.v[lane] = (long long)sv[0][lane] + (long long)sv[1][lane] + (long long)sv[2][lane] +
+                        (long long)sv[3][lane];
+        else if (lane < kEndVals)
+            P.d[lane - kEndCnt] = ((sv[0][lane] + sv[1][lane]) + sv[2][lane]) + sv[3][lane];
         __threadfence();                                 // the partial before the ticket
-        s_last = atomicAdd(&ticket[r], 1) == kEndParts - 1;
+        int last = 0;
+        if (lane == 0) last = atomicAdd(&ticket[r], 1) == kEndParts - 1;
+        if (!__builtin_amdgcn_readfirstlane(last)) return;   // uniform: the wavefront
     }
-    __syncthreads();
-    if (!s_last || w != 0) return;                       // uniform per wavefront
     __threadfence();                                     // every partial after the ticket
     // partials lane, lane + 64, ... in order per lane, then one DPP reduction per sum
     static_assert(kEndParts % 64 == 0, "whole partials per lane");

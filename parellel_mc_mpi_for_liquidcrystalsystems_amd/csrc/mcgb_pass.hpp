@@ -466,7 +466,6 @@ __device__ __forceinline__ void pass_cell(const PassArgs& P, int b, unsigned cha
     Real* __restrict__ Uy = (Real*)P.uy;
     Real* __restrict__ Uz = (Real*)P.uz;
     using L = PassLds<Real, W>;
-    const int lcap = L::list_cap(cap);
 
     int* s_start = (int*)smem;                                 // 126 (<= 125 used)
     int* s_off = s_start + 126;                                // 126 (<= 126 used)

@@ -90,7 +90,6 @@ struct mcgb_ctx {
     unsigned long long* scan_st = nullptr;   // [nrep][scan_tiles] look-back status of the scan
     int* slot = nullptr;                 // each particle's place within its cell (rebuild)
     int* tmp_pid = nullptr;              // particle ids scattered beside tmp (rebuild)
-    int* end_ticket = nullptr;           // [nrep] blocks of k_end done this sweep
     EndPartial* endp = nullptr;          // [nrep][kEndParts] partial sweep totals
     int scan_tiles = 1;
     CellAcc* acc = nullptr;
@@ -282,7 +281,7 @@ int enqueue_rebuild(mcgb_ctx* c, bool draws = false) {
 
 // FP64 energy of the current cell list; mode as in k_energy_reduce.
 template <typename Real>
-int enqueue_energy(mcgb_ctx* c, int mode, bool record = false) {
+int enqueue_energy(mcgb_ctx* c, int mode, bool record = false, bool fold = false) {
     const int b = c->cur;
     const dim3 ge((c->Nmax + 127) / 128, c->nrep);
     const int* cs = mode == 2 ? c->cell_start2 : c->cell_start;
@@ -291,7 +290,8 @@ int enqueue_energy(mcgb_ctx* c, int mode, bool record = false) {
                                           (const Real*)c->u[b][2], cs, c->list2, c->efin, c->ecap,
                                           c->eccap, c->ehard);
     k_energy_reduce<<<c->nrep, 256, 0, c->st>>>(c->dC, c->dD, mode, c->efin, c->ecap, c->eccap, c->ehard,
-                                                record ? c->series : nullptr, c->series_cap);
+                                                record ? c->series : nullptr, c->series_cap,
+                                                fold ? c->endp : nullptr, c->pace_pct);
     HIPCHK(hipGetLastError());
     return MCGB_OK;
 }
@@ -439,10 +439,10 @@ int enqueue_one(mcgb_ctx* c, mcgb_move_trace* trace, const hipEvent_t* ev) {
     HIPCHK(hipGetLastError());
     k_end<Real><<<dim3(kEndParts, c->nrep), 256, 0, c->st>>>(
         c->dC, c->dD, c->acc, (const Real*)c->u[b][0], (const Real*)c->u[b][1],
-        (const Real*)c->u[b][2], c->endp, c->end_ticket, c->pace_pct);
+        (const Real*)c->u[b][2], c->endp);
     // the re-summation (when flagged), the NPT volume move, the sweep's record: the record goes
     // with the energy reduction when no volume move follows
-    if ((rc = enqueue_energy<Real>(c, 1, !c->any_npt))) return rc;
+    if ((rc = enqueue_energy<Real>(c, 1, !c->any_npt, true))) return rc;
     if (c->any_npt) {
         if ((rc = enqueue_npt<Real>(c))) return rc;
         k_record<<<(c->nrep + 63) / 64, 64, 0, c->st>>>(c->dC, c->dD, c->nrep, c->series,
@@ -739,7 +739,6 @@ int mcgb_create(const mcgb_params* p, int n_replicas, mcgb_ctx** out) {
     if (!rc) rc = dalloc_t(c, &c->scan_st, (size_t)c->nrep * c->scan_tiles);
     if (!rc) rc = dalloc_t(c, &c->slot, N);
     if (!rc) rc = dalloc_t(c, &c->tmp_pid, N);
-    if (!rc) rc = dalloc_t(c, &c->end_ticket, c->nrep);
     if (!rc) rc = dalloc_t(c, &c->endp, (size_t)c->nrep * kEndParts);
     if (!rc) rc = dalloc_t(c, &c->acc, c->ncell_tot);
     if (!rc) rc = dalloc_t(c, &c->draws, (size_t)c->Ntot + c->ncell_tot);
@@ -756,11 +755,9 @@ int mcgb_create(const mcgb_params* p, int n_replicas, mcgb_ctx** out) {
     if (!rc) rc = dalloc_t(c, &c->dBlk, c->nblk);
     if (rc) { mcgb_destroy(c); return rc; }
     // the cell counts are zero between rebuilds (each rebuild's k_sort_gather resets them)
-    // so are the scan's status words (cleared by k_scatter) and the k_end tickets (reset by the
-    // last block)
+    // and so are the scan's status words (cleared by k_scatter)
     if (hipMemset(c->count, 0, sizeof(int) * (size_t)c->ncell_tot) != hipSuccess ||
-        hipMemset(c->scan_st, 0, sizeof(unsigned long long) * (size_t)c->nrep * c->scan_tiles) != hipSuccess ||
-        hipMemset(c->end_ticket, 0, sizeof(int) * (size_t)c->nrep) != hipSuccess) {
+        hipMemset(c->scan_st, 0, sizeof(unsigned long long) * (size_t)c->nrep * c->scan_tiles) != hipSuccess) {
         mcgb_set_error("hipMemset of the cell counts failed");
         mcgb_destroy(c);
         return MCGB_ERR_HIP;

@@ -64,6 +64,8 @@ struct RepDyn {
     double e_fin;                // running energy = cap_pairs*1e9 + cap_coll*1e8 + e_fin
     unsigned long long attempted, accepted, left, hard, cand, pairs, coll, vol_att, vol_acc;
     double moved_abs;            // sum |finite dE| of accepted moves this sweep
+    unsigned long long moved_q;  // the same per k_end block, rounded and clamped (1e15), summed
+                                 // as integers (the FP64 re-summation test)
     double max_drift;
     int resum;                   // re-sum requested for this sweep
     int overflow;                // a stencil exceeded the LDS staging capacity

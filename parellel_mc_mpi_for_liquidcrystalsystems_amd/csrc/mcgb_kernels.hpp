@@ -132,6 +132,7 @@ __global__ __launch_bounds__(64) void k_begin(const RepConst* __restrict__ RC,
     D.nact = nc / P;
     D.k = (C.N + nc - 1) / nc;
     D.moved_abs = 0.0;
+    D.moved_q = 0ull;
     D.resum = (resum_interval > 0 && ((t + 1) % (uint32_t)resum_interval) == 0) ? 1 : 0;
 }
 
@@ -446,28 +447,28 @@ __global__ __launch_bounds__(256) void k_count_draws(const RepConst* __restrict_
 
 // End of the passes, one launch (k_end): block b of replica r folds the per-cell accumulators
 // of cell chunk b and the orientation products u_a u_b of particles b*256 + tid + j*kEndParts*256
-// into a partial; the last block of the replica to finish (a ticket) folds the kEndParts
-// partials into the running energy and counters, runs the step-size controller, flags an FP64
-// re-summation when needed, keeps the orientation sums for the sweep's record and sets the
-// pace of the next sweep's sequential passes (pace_pct percent of this sweep's mean cell time
-// per move; 0: no pacing).  Every sum runs in a fixed order (four particles' loads issued
-// together, lane butterflies, then the wavefronts in order): deterministic.
+// into a partial, and adds its moved |dE|, rounded to whole energy units, to the replica's
+// integer total (the re-summation test of k_energy: an integer sum is the same in any order).
+// The next launches fold the partials (end_fold in k_energy_reduce): no fence or ticket here,
+// the launch boundary orders them.  Every sum runs in a fixed order (four particles' loads
+// issued together, lane butterflies, then the wavefronts in order): deterministic.
 constexpr int kEndParts = 256;
-constexpr int kEndCnt = 13;        // 11 move counters, move-loop ticks and moves
+constexpr int kEndCnt = 13;        // 11 move counters, the cells' ticks and moves
 constexpr int kEndVals = kEndCnt + 8;   // and finite dE, moved |dE|, 6 orientation sums
 struct EndPartial {
     long long v[kEndCnt];
     double d[8];                   // finite dE, moved |dE|, orientation sums
 };
+// a sweep whose accepted moves moved more than this much finite energy is re-summed in FP64
+constexpr unsigned long long kForcedMoved = 10000000000ull;     // 1e10
 
 template <typename Real>
 __global__ __launch_bounds__(256) void k_end(const RepConst* __restrict__ RC,
                                              RepDyn* __restrict__ RD,
                                              const CellAcc* __restrict__ acc,
                                              const Real* __restrict__ ux, const Real* __restrict__ uy,
-                                             const Real* __restrict__ uz, EndPartial* part,
-                                             int* __restrict__ ticket, int pace_pct) {
-    // the 19 sums travel as doubles: the counters are integers far below 2^53, exact in FP64,
+                                             const Real* __restrict__ uz, EndPartial* part) {
+    // the 21 sums travel as doubles: the counters are integers far below 2^53, exact in FP64,
     // so one fixed-order DPP reduction (result in lane 63) serves them all
     __shared__ double sv[4][kEndVals];
     const int r = blockIdx.y;
@@ -514,36 +515,46 @@ __global__ __launch_bounds__(256) void k_end(const RepConst* __restrict__ RC,
         for (int k = 0; k < kEndVals; ++k) sv[w][k] = e[k];
     __syncthreads();
     if (w != 0) return;                                  // wavefront 0 writes the partial
-    {
-        // one sum per lane, the wavefronts in order
-        EndPartial& P = part[(size_t)r * kEndParts + blockIdx.x];
-        if (lane < kEndCnt)
-            P.v[lane] = (long long)sv[0][lane] + (long long)sv[1][lane] + (long long)sv[2][lane] +
-                        (long long)sv[3][lane];
-        else if (lane < kEndVals)
-            P.d[lane - kEndCnt] = ((sv[0][lane] + sv[1][lane]) + sv[2][lane]) + sv[3][lane];
-        __threadfence();                                 // the partial before the ticket
-        int last = 0;
-        if (lane == 0) last = atomicAdd(&ticket[r], 1) == kEndParts - 1;
-        if (!__builtin_amdgcn_readfirstlane(last)) return;   // uniform: the wavefront
+    // one sum per lane, the wavefronts in order
+    EndPartial& P = part[(size_t)r * kEndParts + blockIdx.x];
+    if (lane < kEndCnt)
+        P.v[lane] = (long long)sv[0][lane] + (long long)sv[1][lane] + (long long)sv[2][lane] +
+                    (long long)sv[3][lane];
+    else if (lane < kEndVals)
+        P.d[lane - kEndCnt] = ((sv[0][lane] + sv[1][lane]) + sv[2][lane]) + sv[3][lane];
+    if (lane == 0) {
+        const int k = kEndCnt + 1;
+        const double mv = ((sv[0][k] + sv[1][k]) + sv[2][k]) + sv[3][k];
+        // clamped (near-cap terms reach 1e36; anything above the test's 1e10 is enough)
+        const unsigned long long q = (unsigned long long)llrint(fmin(mv, 1.0e15));
+        if (q) atomicAdd(&D.moved_q, q);
     }
-    __threadfence();                                     // every partial after the ticket
-    // partials lane, lane + 64, ... in order per lane, then one DPP reduction per sum
+}
+
+// The fold of k_end's partials (wavefront 0 of k_energy_reduce's block for replica r): partials
+// lane, lane + 64, ... in order per lane, then one DPP reduction per sum; lane 63 updates the
+// running energy and counters, runs the step-size controller, flags an FP64 re-summation when
+// the sweep moved more than kForcedMoved, keeps the orientation sums for the sweep's record and
+// sets the pace of the next sweep's sequential passes (pace_pct percent of this sweep's mean
+// cell time per move; 0: no pacing).
+__device__ inline void end_fold(const RepConst& C, RepDyn& D, const EndPartial* __restrict__ part,
+                                int pace_pct) {
     static_assert(kEndParts % 64 == 0, "whole partials per lane");
-    for (int k = 0; k < kEndCnt; ++k) v[k] = 0;
-    for (int k = 0; k < 8; ++k) d[k] = 0.0;
+    const int lane = threadIdx.x & 63;
+    long long v[kEndCnt] = {};
+    double d[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     for (int j = lane; j < kEndParts; j += 64) {
-        const EndPartial& P = part[(size_t)r * kEndParts + j];
+        const EndPartial& P = part[j];
         for (int k = 0; k < kEndCnt; ++k) v[k] += P.v[k];
         for (int k = 0; k < 8; ++k) d[k] += P.d[k];
     }
+    double e[kEndVals];
 #pragma unroll
     for (int k = 0; k < kEndCnt; ++k) e[k] = wave_sum_d63((double)v[k]);
 #pragma unroll
     for (int k = 0; k < 8; ++k) e[kEndCnt + k] = wave_sum_d63(d[k]);
     if (lane != 63) return;
     for (int k = 0; k < kEndCnt; ++k) v[k] = (long long)e[k];
-    ticket[r] = 0;                                       // for the next sweep
     D.attempted += v[0]; D.accepted += v[1]; D.left += v[2]; D.hard += v[3];
     D.cand += v[4]; D.pairs += v[5]; D.coll += v[6];
     D.cap_pairs += v[7];
@@ -552,7 +563,7 @@ __global__ __launch_bounds__(256) void k_end(const RepConst* __restrict__ RC,
     D.da_s2 += v[10];
     D.e_fin += e[kEndCnt];
     D.moved_abs = e[kEndCnt + 1];
-    if (e[kEndCnt + 1] > 1.0e10) D.resum = 1;
+    if (D.moved_q > kForcedMoved) D.resum = 1;
     D.pace = (pace_pct > 0 && v[12] > 0) ? (int)(v[11] * pace_pct / (100 * v[12])) : 0;
     if (C.adapt_moves > 0) {
         D.ad_att += (unsigned long long)v[0];
@@ -594,7 +605,7 @@ __global__ __launch_bounds__(128) void k_energy(
     const int r = blockIdx.y;
     const RepConst& C = RC[r];
     const RepDyn& D = RD[r];
-    if (mode == 1 && !D.resum) return;
+    if (mode == 1 && !D.resum && D.moved_q <= kForcedMoved) return;   // k_end's moved energy
     if (mode == 2 && !D.trial_ok) return;
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int nw = (C.N + 63) >> 6;
@@ -761,7 +772,7 @@ __device__ inline void energy_reduce(const RepConst& C, RepDyn& D, int mode,
             const double run = (double)D.cap_pairs * 1e9 + (double)D.cap_coll * 1e8 + D.e_fin;
             const double now = sc[0][0] * 1e9 + sc[0][1] * 1e8 + sf[0];
             const double dr = fabs(run - now);
-            if (D.moved_abs <= 1.0e10) {
+            if (D.moved_q <= kForcedMoved) {
                 if (dr > D.max_drift) D.max_drift = dr;
             } else {
                 D.resums_forced += 1;
@@ -792,7 +803,8 @@ __device__ inline void energy_reduce(const RepConst& C, RepDyn& D, int mode,
 __device__ inline void series_record(const RepConst& C, RepDyn& D, int r, double* __restrict__ series,
                                      int series_cap, int advance);
 
-// One block per replica; series != null: then the sweep's record (series_record, advancing the
+// One block per replica; endp != null (the sweep's mode-1 call): first the fold of k_end's
+// partials (end_fold); series != null: then the sweep's record (series_record, advancing the
 // sweep index), after the energy.
 __global__ __launch_bounds__(256) void k_energy_reduce(const RepConst* __restrict__ RC,
                                                        RepDyn* __restrict__ RD, int mode,
@@ -801,10 +813,16 @@ __global__ __launch_bounds__(256) void k_energy_reduce(const RepConst* __restric
                                                        const int* __restrict__ eccap,
                                                        const int* __restrict__ ehard,
                                                        double* __restrict__ series = nullptr,
-                                                       int series_cap = 0) {
+                                                       int series_cap = 0,
+                                                       const EndPartial* __restrict__ endp = nullptr,
+                                                       int pace_pct = 0) {
     const int r = blockIdx.x;
     const RepConst& C = RC[r];
     RepDyn& D = RD[r];
+    if (endp) {
+        if (threadIdx.x < 64) end_fold(C, D, endp + (size_t)r * kEndParts, pace_pct);
+        __syncthreads();                                // the fold's updates of D before the energy
+    }
     const bool run = !(mode == 1 && !D.resum) && !(mode == 2 && !D.trial_ok);   // uniform
     if (run) energy_reduce(C, D, mode, efin, ecap, eccap, ehard);
     if (series && threadIdx.x == 0) series_record(C, D, r, series, series_cap, 1);

@@ -170,21 +170,27 @@ __global__ void k_grid_g0(const RepConst* __restrict__ RC, RepDyn* __restrict__ 
 // (the slot order within a cell is arbitrary; k_sort_gather orders every cell by id).
 __device__ inline int wave_cell_add(int* __restrict__ ctr, int c, bool valid) {
     const int lane = threadIdx.x & 63;
-    int slot = 0;
+    // the distinct cells first (no memory traffic): each lane learns its cell's leader (lowest
+    // lane) and its rank among the cell's lanes, each leader the cell's lane count
+    int leader = lane, rank = 0, n = 0;
     unsigned long long pending = __builtin_amdgcn_ballot_w64(valid);
     while (pending) {                                  // uniform: a ballot
-        const int leader = __builtin_ctzll(pending);
-        const int lc = __builtin_amdgcn_readlane(c, leader);
+        const int ld = __builtin_ctzll(pending);
+        const int lc = __builtin_amdgcn_readlane(c, ld);
         const unsigned long long m = __builtin_amdgcn_ballot_w64(valid && c == lc);
-        int old = 0;
-        if (lane == leader) old = atomicAdd(&ctr[lc], __popcll(m));
-        old = __builtin_amdgcn_readlane(old, leader);
-        if ((m >> lane) & 1ull)
-            slot = old + (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
-                                                         __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+        if ((m >> lane) & 1ull) {
+            leader = ld;
+            rank = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                                  __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+        }
+        if (lane == ld) n = __popcll(m);
         pending &= ~m;
     }
-    return slot;
+    // then one atomic per distinct cell, all in flight together (the leaders' lanes)
+    int old = 0;
+    if (valid && leader == lane) old = atomicAdd(&ctr[c], n);
+    old = __shfl(old, leader);
+    return valid ? old + rank : 0;
 }
 
 // Cell and slot of particle i of replica r (every lane of the wavefront calls it: the counting

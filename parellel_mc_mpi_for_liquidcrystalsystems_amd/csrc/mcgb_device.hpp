@@ -64,8 +64,9 @@ struct RepDyn {
     double e_fin;                // running energy = cap_pairs*1e9 + cap_coll*1e8 + e_fin
     unsigned long long attempted, accepted, left, hard, cand, pairs, coll, vol_att, vol_acc;
     double moved_abs;            // sum |finite dE| of accepted moves this sweep
-    unsigned long long moved_q;  // the same per k_end block, rounded and clamped (1e15), summed
-                                 // as integers (the FP64 re-summation test)
+    unsigned long long moved_q[8];   // the same per k_end block, rounded and clamped (1e15),
+                                     // summed as integers into 8 words (blocks b mod 8, so that
+                                     // the atomics spread) for the FP64 re-summation test
     double max_drift;
     int resum;                   // re-sum requested for this sweep
     int overflow;                // a stencil exceeded the LDS staging capacity
@@ -94,6 +95,13 @@ struct RepDyn {
 // Draws of one trial move (draw_move): particle selector, fixed-point step, rotation vector,
 // Metropolis threshold -log(U) T, and the threshold of the surrogate stage of delayed
 // acceptance -log(U') T (U' an independent uniform).
+// The moved-energy total of a sweep (k_end's integer sums).
+__host__ __device__ inline unsigned long long moved_total(const RepDyn& D) {
+    unsigned long long t = 0;
+    for (int k = 0; k < 8; ++k) t += D.moved_q[k];
+    return t;
+}
+
 struct MoveDraw {
     uint32_t j;
     int32_t dq[3];

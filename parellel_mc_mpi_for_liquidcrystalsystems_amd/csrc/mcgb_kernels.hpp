@@ -132,7 +132,7 @@ __global__ __launch_bounds__(64) void k_begin(const RepConst* __restrict__ RC,
     D.nact = nc / P;
     D.k = (C.N + nc - 1) / nc;
     D.moved_abs = 0.0;
-    D.moved_q = 0ull;
+    for (int k = 0; k < 8; ++k) D.moved_q[k] = 0ull;
     D.resum = (resum_interval > 0 && ((t + 1) % (uint32_t)resum_interval) == 0) ? 1 : 0;
 }
 
@@ -533,7 +533,7 @@ __global__ __launch_bounds__(256) void k_end(const RepConst* __restrict__ RC,
         const double mv = ((sv[0][k] + sv[1][k]) + sv[2][k]) + sv[3][k];
         // clamped (near-cap terms reach 1e36; anything above the test's 1e10 is enough)
         const unsigned long long q = (unsigned long long)llrint(fmin(mv, 1.0e15));
-        if (q) atomicAdd(&D.moved_q, q);
+        if (q) atomicAdd(&D.moved_q[blockIdx.x & 7], q);
     }
 }
 
@@ -569,7 +569,7 @@ __device__ inline void end_fold(const RepConst& C, RepDyn& D, const EndPartial* 
     D.da_s2 += v[10];
     D.e_fin += e[kEndCnt];
     D.moved_abs = e[kEndCnt + 1];
-    if (D.moved_q > kForcedMoved) D.resum = 1;
+    if (moved_total(D) > kForcedMoved) D.resum = 1;
     D.pace = (pace_pct > 0 && v[12] > 0) ? (int)(v[11] * pace_pct / (100 * v[12])) : 0;
     if (C.adapt_moves > 0) {
         D.ad_att += (unsigned long long)v[0];
@@ -611,7 +611,7 @@ __global__ __launch_bounds__(128) void k_energy(
     const int r = blockIdx.y;
     const RepConst& C = RC[r];
     const RepDyn& D = RD[r];
-    if (mode == 1 && !D.resum && D.moved_q <= kForcedMoved) return;   // k_end's moved energy
+    if (mode == 1 && !D.resum && moved_total(D) <= kForcedMoved) return;   // k_end's moved energy
     if (mode == 2 && !D.trial_ok) return;
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int nw = (C.N + 63) >> 6;
@@ -778,7 +778,7 @@ __device__ inline void energy_reduce(const RepConst& C, RepDyn& D, int mode,
             const double run = (double)D.cap_pairs * 1e9 + (double)D.cap_coll * 1e8 + D.e_fin;
             const double now = sc[0][0] * 1e9 + sc[0][1] * 1e8 + sf[0];
             const double dr = fabs(run - now);
-            if (D.moved_q <= kForcedMoved) {
+            if (moved_total(D) <= kForcedMoved) {
                 if (dr > D.max_drift) D.max_drift = dr;
             } else {
                 D.resums_forced += 1;

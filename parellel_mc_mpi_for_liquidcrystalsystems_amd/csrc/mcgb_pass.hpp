@@ -4,15 +4,18 @@
 //
 // Per activation: the stencil ((2R+1)^3 cell segments of the device cell list, R = 1 for 8
 // sublattices, 2 for 27) is staged in LDS.  The moves run in chunks of 9: the chunk's draws
-// (written by k_draws) arrive from HBM one chunk ahead, and 9 lanes prepare one record per
+// (written by k_count_draws) arrive from HBM one chunk ahead, and 9 lanes prepare one record per
 // move (draws, the particle's state, trial orientation, trial colloid term, the leave-cell and
 // colloid-core tests, the pre-filter radius).  Per move: each
 // wavefront tests its share of the stencil against the old and trial positions (FP32
 // midpoint pre-filter), compacts the hits into a wavefront-private LDS list, evaluates both GB
 // energies of every listed pair (cutoff decided on the FP64 r^2) two items per lane at a time,
 // and reduces (DPP) the energy difference; the wavefront partials meet in LDS, every thread
-// takes the same Metropolis decision, thread 0 commits an accepted move to LDS and HBM.
-// Two workgroup barriers per move.  Stencils beyond the staging capacity read HBM directly.
+// takes the same Metropolis decision, and an accepted move is committed to LDS (in the FP32
+// sequential build by every wavefront, which then reads the next move's record and first scan
+// group ahead of the closing barrier) and by thread 0 to HBM.  Two workgroup barriers per
+// move.  Cells behind the pace set from the last sweep raise their issue priority (pacing).
+// Stencils beyond the staging capacity read HBM directly.
 #pragma once
 
 #include <type_traits>
@@ -515,7 +518,7 @@ __device__ __forceinline__ void pass_cell(const PassArgs& P, int b, unsigned cha
     const int* cs = P.cell_start + C.cs_off;
     const uint32_t gx = D.g[0], gy = D.g[1], gz = D.g[2];
 
-    // the draws of the cell's moves (k_draws), KD moves (KD * 56 contiguous bytes) at a time,
+    // the draws of the cell's moves (k_count_draws), KD moves (KD * 56 contiguous bytes) at a time,
     // spread over wavefront 0 as 8-byte words: the first chunk's loads are issued here, ahead of
     // the staging, and every later chunk's while the chunk before it runs
     const int k = D.k;

@@ -286,7 +286,7 @@ __global__ __launch_bounds__(64 * W) void k_pass_spec(PassArgs P) {
     if (nc > 0) {
         for (int m0 = 0; m0 < k; m0 += KD) {
             __syncthreads();
-            if (tid < KD && m0 + tid < k) {   // the draws of up to KD moves (k_draws)
+            if (tid < KD && m0 + tid < k) {   // the draws of up to KD moves (k_count_draws)
                 const MoveDraw& d = drw[m0 + tid];
                 sj[tid] = d.j;
                 sdqx[tid] = d.dq[0];

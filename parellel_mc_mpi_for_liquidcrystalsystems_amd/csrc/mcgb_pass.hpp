@@ -815,7 +815,12 @@ __device__ __forceinline__ void pass_cell(const PassArgs& P, int b, unsigned cha
                             constexpr int kG = 4;
                             const int ng = S_pad >> 6;
                             for (int g0 = w; g0 < ng; g0 += kG * W) {
-                                const int nv = min(kG, (ng - g0 + W - 1) / W);   // uniform
+                                // uniform; taken through an opaque scalar move so that the
+                                // compiler does not hoist the group tests out of the move loop
+                                // as lane masks (they overflow the scalar registers and come
+                                // back by v_readlane, one per group and move)
+                                int nv;
+                                asm volatile("s_mov_b32 %0, %1" : "=s"(nv) : "s"(min(kG, (ng - g0 + W - 1) / W)));
                                 // every load issued unconditionally (groups past the end read
                                 // the last one again): no branch between them, so they are in
                                 // flight together

@@ -1464,7 +1464,7 @@ int mcgb_series_clear(mcgb_ctx* c) {
     return write_dyn(c, D);
 }
 
-// The pacing percentage is a kernel argument of k_end: a captured sweep graph holds it.
+// The pacing percentage is a kernel argument of k_energy_reduce: a captured sweep graph holds it.
 static void set_pace_pct(mcgb_ctx* c, int pct) {
     if (pct == c->pace_pct) return;
     for (int k = 0; k < 2; ++k)

@@ -531,7 +531,7 @@ __device__ __forceinline__ void pass_cell(const PassArgs& P, int b, unsigned cha
     if (w == 0 && lane < min(k, KD) * kW2) dpre = drw2[lane];
 
     // pacing of the sequential pass: the cell timed on the 100 MHz clock from the start of its
-    // workgroup (a slow staging counts against it), the pace set by k_end from the last sweep
+    // workgroup (a slow staging counts against it), the pace set by the last sweep's end fold
     const unsigned long long pace0 = (!GLOBAL && !TR && !DA && !O4) ? __builtin_amdgcn_s_memrealtime() : 0ull;
     // stencil table: NS = (2R+1)^3 neighbour cells in (dz, dy, dx) order, the active cell at
     // NS/2; two cells per lane, one inclusive scan per half

@@ -397,14 +397,4 @@ __host__ __device__ inline GBd make_gbd(const RepConst& C) {
     return g;
 }
 
-// ------------------------------------------------------------------ compensated sums
-
-__device__ inline void two_sum(float& hi, float& lo, float v) {
-    float s = hi + v;
-    float bp = s - hi;
-    float err = (hi - (s - bp)) + (v - bp);
-    hi = s;
-    lo += err;
-}
-
 }  // namespace mcgb

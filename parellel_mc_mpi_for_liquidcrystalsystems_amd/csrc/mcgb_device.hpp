@@ -23,9 +23,10 @@ struct RepConst {
     int part_off;          // first particle slot in the particle arrays
     int cell_off;          // first cell slot (counts/accumulators), capacity ncell_cap
     int cs_off;            // first cell_start slot (ncell_cap + 1 entries)
-    int ncap[3];           // allocated cells per axis (even)
+    int ncap[3];           // grid laid out (cubic box, NPT allowance): its product is the
+                           // cell capacity; a box's grid takes any shape within it (fit_grid)
     int act_off;           // first block of this replica in the pass grid
-    int nact_cap;          // blocks reserved for this replica (= ncap product / 8)
+    int nact_cap;          // blocks reserved for this replica (= ncap product / col^3)
     uint32_t key0, key1;   // Philox key
     int colloid_on;        // colloid_wall_option == 1
     int npt;               // NPT ln V moves
@@ -92,9 +93,6 @@ struct RepDyn {
     int pad2;
 };
 
-// Draws of one trial move (draw_move): particle selector, fixed-point step, rotation vector,
-// Metropolis threshold -log(U) T, and the threshold of the surrogate stage of delayed
-// acceptance -log(U') T (U' an independent uniform).
 // The moved-energy total of a sweep (k_end's integer sums).
 __host__ __device__ inline unsigned long long moved_total(const RepDyn& D) {
     unsigned long long t = 0;
@@ -102,6 +100,9 @@ __host__ __device__ inline unsigned long long moved_total(const RepDyn& D) {
     return t;
 }
 
+// Draws of one trial move (draw_move): particle selector, fixed-point step, rotation vector,
+// Metropolis threshold -log(U) T, and the threshold of the surrogate stage of delayed
+// acceptance -log(U') T (U' an independent uniform).
 struct MoveDraw {
     uint32_t j;
     int32_t dq[3];
@@ -178,6 +179,20 @@ __host__ __device__ inline int grid_dim(double L, double rc, int col) {
     if (col != 3) return cell_dim(L, rc);
     int n = 3 * (int)floor(L / (1.5 * rc * (1.0 + 1e-6)));
     return n >= 6 ? n : 0;
+}
+
+// The cell arrays hold `cap` cells in all (the grid of the cubic box the context was laid out
+// for, or of its NPT growth allowance).  A box whose own grid needs more (an edge longer than
+// the layout planned for) coarsens its longest axis first, col cells at a time, down to at most
+// cap cells: wider cells keep the checkerboard exact, and the box keeps its own grid wherever
+// that fits.  Every axis stays at >= 2 col cells (2 col per axis fit: each allocated axis
+// holds at least that many).
+__host__ __device__ inline void fit_grid(int n[3], int cap, int col) {
+    while (n[0] * n[1] * n[2] > cap) {
+        int a = n[1] > n[0] ? 1 : 0;
+        if (n[2] > n[a]) a = 2;
+        n[a] -= col;
+    }
 }
 
 // ------------------------------------------------------------------ GB pair, FP32

@@ -119,12 +119,13 @@ __global__ __launch_bounds__(64) void k_begin(const RepConst* __restrict__ RC,
         uint32_t tmp = ord[i]; ord[i] = ord[j]; ord[j] = tmp;
     }
     for (int i = 0; i < P; ++i) D.order[i] = ord[i];
+    int nn[3];
+    for (int d = 0; d < 3; ++d) nn[d] = grid_dim(D.L[d], C.rc, C.col);
+    fit_grid(nn, C.ncap[0] * C.ncap[1] * C.ncap[2], C.col);
     int nc = 1;
     for (int d = 0; d < 3; ++d) {
-        int n = grid_dim(D.L[d], C.rc, C.col);
-        if (n > C.ncap[d]) n = C.ncap[d];
-        D.n[d] = n;
-        nc *= n;
+        D.n[d] = nn[d];
+        nc *= nn[d];
         D.s[d] = D.L[d] / kTwo32;
         D.K[d] = step_amplitude(D.dmax, D.L[d]);
     }
@@ -143,12 +144,13 @@ __global__ void k_grid_g0(const RepConst* __restrict__ RC, RepDyn* __restrict__ 
     if (r >= nrep) return;
     const RepConst& C = RC[r];
     RepDyn& D = RD[r];
+    int nn[3];
+    for (int d = 0; d < 3; ++d) nn[d] = grid_dim(D.L[d], C.rc, C.col);
+    fit_grid(nn, C.ncap[0] * C.ncap[1] * C.ncap[2], C.col);
     int nc = 1;
     for (int d = 0; d < 3; ++d) {
-        int n = grid_dim(D.L[d], C.rc, C.col);
-        if (n > C.ncap[d]) n = C.ncap[d];
-        D.n[d] = n;
-        nc *= n;
+        D.n[d] = nn[d];
+        nc *= nn[d];
         D.s[d] = D.L[d] / kTwo32;
         D.K[d] = step_amplitude(D.dmax, D.L[d]);
         D.g[d] = 0u;
@@ -846,12 +848,14 @@ __global__ void k_npt_trial(const RepConst* __restrict__ RC, RepDyn* __restrict_
     const double dlnv = C.npt_max_dlnv * (2.0 * u01(D.vol[0]) - 1.0);
     const double f = exp(dlnv / 3.0);
     int ok = 1;
+    int nn[3];
     for (int d = 0; d < 3; ++d) {
         D.Lt[d] = D.L[d] * f;
-        int n = cell_dim(D.Lt[d], C.rc);      // wider cells than needed are still exact
-        if (n == 0) ok = 0;
-        D.nt[d] = n > C.ncap[d] ? C.ncap[d] : n;
+        nn[d] = cell_dim(D.Lt[d], C.rc);
+        if (nn[d] == 0) ok = 0;
     }
+    if (ok) fit_grid(nn, C.ncap[0] * C.ncap[1] * C.ncap[2], 2);
+    for (int d = 0; d < 3; ++d) D.nt[d] = nn[d];
     D.trial_ok = ok;
 }
 

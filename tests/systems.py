@@ -67,6 +67,43 @@ def layered_system(N, rho, colloid_radius=0.0, seed=0, tilt=0.1, layer=3.25):
     return q, [np.ascontiguousarray(u[:, d]) for d in range(3)], Lv
 
 
+def box_system(N, rho, aspect, colloid_radius=0.0, seed=0, tilt=0.1, layer=3.25):
+    """The layered start in a rectangular box: volume N/rho + 4/3 pi R^3, edge lengths in the
+    ratios `aspect` (so every axis has its own cell count and fixed-point scale); square-ish
+    layers spaced at least `layer` apart in z, sites within R + 2 of the centre removed.
+    Returns (q (3,N) uint32, u (3,N) float64, L (3,))."""
+    rng = np.random.default_rng(seed)
+    V = N / rho + (4.0 / 3.0) * np.pi * colloid_radius ** 3
+    a = np.asarray(aspect, dtype=np.float64)
+    Lv = a * (V / np.prod(a)) ** (1.0 / 3.0)
+    nz = max(1, int(Lv[2] // layer))
+    excl = colloid_radius + 2.0 if colloid_radius > 0 else 0.0
+    per_layer = N / nz
+    nx = int(np.ceil(np.sqrt(per_layer * Lv[0] / Lv[1])))
+    ny = int(np.ceil(np.sqrt(per_layer * Lv[1] / Lv[0])))
+    while True:
+        gx = (np.arange(nx) + 0.5) * (Lv[0] / nx)
+        gy = (np.arange(ny) + 0.5) * (Lv[1] / ny)
+        gz = (np.arange(nz) + 0.5) * (Lv[2] / nz)
+        X, Y, Zg = np.meshgrid(gx, gy, gz, indexing="ij")
+        sites = np.stack([X.ravel(), Y.ravel(), Zg.ravel()], axis=1)
+        if colloid_radius > 0:
+            sites = sites[np.linalg.norm(sites - 0.5 * Lv, axis=1) >= excl]
+        if sites.shape[0] >= N:
+            break
+        nx += 1
+        ny += 1
+    sites = sites[np.sort(rng.permutation(sites.shape[0])[:N])]
+    sites[:, :2] += rng.uniform(-0.03, 0.03, size=(N, 2))
+    u = np.zeros((N, 3))
+    u[:, 2] = 1.0
+    u[:, :2] += rng.normal(scale=tilt, size=(N, 2))
+    u /= np.linalg.norm(u, axis=1, keepdims=True)
+    u *= np.where(rng.uniform(size=(N, 1)) < 0.5, -1.0, 1.0)
+    q = [oc.to_fixed(sites[:, d], Lv[d]) for d in range(3)]
+    return q, [np.ascontiguousarray(u[:, d]) for d in range(3)], Lv
+
+
 def cond_tol(base, scale):
     """Tolerance of a summed energy difference whose largest term may sit near the overlap cap:
     a pair at r - sigma + sigma0 = sigma0/rho carries a relative condition number of about

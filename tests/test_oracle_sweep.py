@@ -131,3 +131,35 @@ def test_27_colour_sweeps_sample_the_same_energy():
         means.append((E.mean(), E.std() * np.sqrt(tau / len(E))))
     (m2, e2), (m3, e3) = means
     assert abs(m2 - m3) < 4.0 * np.hypot(e2, e3), means
+
+
+@pytest.mark.parametrize("colors", [2, 3])
+def test_rectangular_box_replay_and_energy(colors):
+    """A box with three different edge lengths (its own cell count and fixed-point scale per
+    axis): every move of a sweep replays against brute force, and the running energy equals
+    the brute-force total (per-axis minimum image) afterwards."""
+    import ctypes as C
+    from tests.systems import box_system
+    c = oc.consts(temperature=1.0, colloid_radius=3.0, colloid_on=1, colors=colors)
+    q, u, L = box_system(6000, 0.3, (1.45, 1.0, 0.69), colloid_radius=3.0, seed=3)
+    n = [oc.grid_dim(L[d], c.cutoff, colors) for d in range(3)]
+    assert len(set(n)) == 3
+    s = oc.System(c, q, u, L, seed=4321, nthreads=4)
+    s.sweep()
+    q0 = [a.copy() for a in s.q]
+    u0 = [a.copy() for a in s.u]
+    t = s.t
+    tr, info, cnt = s.sweep(trace=True)
+    qf, uf, st = replay_sweep(c, q0, u0, s.L, s.seed, 0, t, tr, info)
+    assert st["checked"] > 3000 and st["accepted"] > 800
+    for d in range(3):
+        assert np.array_equal(qf[d], s.q[d])
+        np.testing.assert_allclose(uf[d], s.u[d], rtol=0, atol=1e-13)
+    e = s.total_energy()
+    assert s.energy[0] == e[0] and s.energy[2] == pytest.approx(e[2], rel=1e-10, abs=1e-9)
+    x = s.positions()
+    eb = np.empty(4)
+    oc.lib().orc_total_energy_brute(C.byref(c), s.N, *(oc._d(np.ascontiguousarray(a)) for a in x),
+                                    *(oc._d(a) for a in s.u), oc._d(s.L), oc._d(eb))
+    assert eb[0] == e[0] and eb[1] == e[1]
+    assert eb[2] == pytest.approx(e[2], rel=1e-10, abs=1e-9)

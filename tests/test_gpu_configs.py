@@ -5,7 +5,8 @@
   - the same geometry at N = 20000: FP64 sweeps move by move against the oracle, and an FP32
     sweep replayed move by move in FP64 with the distribution of its |dE|-relative errors;
   - at full size: the cell list bit-exact against the oracle, and after six sweeps the capped
-    pair and colloid counts exact and the finite running energy against the FP64 recompute.
+    pair and colloid counts exact and the finite running energy against the FP64 recompute;
+  - at eight times the full size (N = 2048000, layered start), FP32 and FP64: the same checks.
 * cfg4 (delayed acceptance with the colloid at T* = 0.8), at reduced N: a zero network gives
   the plain chain bit for bit with the colloid term in stage 1; a poor network samples the same
   <E>/N and P2 as the plain sweep (2 sigma, autocorrelation-corrected).
@@ -165,6 +166,39 @@ def test_cfg1_full_size_cell_list_caps_and_energy():
         cs, ids, shift, order, dims = ctx.cell_list(6)
         g, o, _ = oc.sweep_setup(seed, 0, 6)
         cs_o, so_o = oc.cell_list(q1, np.array([n] * 3, dtype=np.int32), g)
+        assert np.array_equal(cs, cs_o) and np.array_equal(ids, so_o)
+
+
+@pytest.mark.parametrize("precision", [32, 64])
+def test_eight_times_the_full_size(precision):
+    """N = 2048000 at the cfg1 density with the R = 5 colloid (a 46^3 grid, 12167 active cells
+    per pass, 8x the benchmark's state), from a layered start: the cell lists bit-exact against
+    the oracle, and after three sweeps of the two-wavefront sequential kernel the cap counts
+    exact and the running energy equal to the FP64 recompute."""
+    N = 2048000
+    print(f"N = {N}: FP{precision}", flush=True)
+    q, u, L = layered_system(N, RHO, colloid_radius=R, seed=77)
+    p = m.gb_params(N, RHO, 1.0, colloid_radius=R, precision=precision, seed=4077,
+                    energy_check_interval=0, pass_schedule=0, waves_per_cell=2)
+    with m.Context(p) as ctx:
+        ctx.set_state_fixed(q, u, L)
+        n = oc.cell_dim(L[0], 4.0)
+        assert n == 46
+        cs, ids, shift, order, dims = ctx.cell_list(0)
+        g, _, _ = oc.sweep_setup(4077, 0, 0)
+        cs_o, so_o = oc.cell_list(q, np.array([n] * 3, dtype=np.int32), g)
+        assert list(dims) == [n] * 3
+        assert np.array_equal(cs, cs_o) and np.array_equal(ids, so_o)
+        ctx.sweep(3)
+        st = ctx.stats()
+        assert st["attempted"] > 2 * N and 0 < st["accepted"] < st["attempted"]
+        tot = ctx.total_energy()
+        assert st["cap_pairs"] == tot[1] and st["cap_colloid"] == tot[2]
+        tol = 1e-9 if precision == 64 else 1e-6
+        assert abs(st["energy_finite"] - tot[3]) <= tol * (abs(tot[3]) + N)
+        q1, _, _ = ctx.get_state_fixed()
+        cs, ids, shift, _, _ = ctx.cell_list(3)
+        cs_o, so_o = oc.cell_list(q1, np.array([n] * 3, dtype=np.int32), shift)
         assert np.array_equal(cs, cs_o) and np.array_equal(ids, so_o)
 
 

@@ -172,7 +172,12 @@ int mcgb_device_count(void);
 /* Sets replica r's configuration: n particles (must equal its num_particles), positions in
  * [0, box) (wrapped otherwise), orientations (normalised here), box lengths.  Positions are
  * stored as 32-bit fixed-point fractions of the box (x = q * L / 2^32).  Recomputes the FP64
- * total energy.  Replaces the scatter of main() (MCnptColloid.c:528-609). */
+ * total energy.  Replaces the scatter of main() (MCnptColloid.c:528-609).
+ * The box may be rectangular; every edge must hold 4 cells of the cutoff.  The cell arrays are
+ * laid out at mcgb_create for the cubic box of num_particles / system_density (NPT: 1.3x its
+ * edge); a box whose own cell grid needs more cells than that coarsens its longest axis (wider
+ * cells, still exact, but not the reference's grid).  Give system_density the density of the
+ * box you will set to keep its own grid. */
 int mcgb_set_state(mcgb_ctx *ctx, int replica, int n,
                    const double *x, const double *y, const double *z,
                    const double *ux, const double *uy, const double *uz,

@@ -17,7 +17,8 @@ emitted from GPU trajectories at both temperatures; plain and delayed acceptance
 the same equilibrated state, with the plain kernel as fallback (mcgb_da_guard).
 
 Also: the NPT workload (density), the 32-replica batch, the 8000-particle CPU configuration
-(`cfg0`: the FP64 oracle on the host cores beside the GPU in FP32 and FP64), and `cpu_baseline`,
+(`n8000`: N = 8000 without a colloid, the FP64 oracle on the host cores beside the GPU in FP32
+and FP64), and `cpu_baseline`,
 the FP64 oracle timed on a bounded sample of the plain workload.  One JSON line.
 """
 import argparse
@@ -419,10 +420,10 @@ def replicas_leg(args):
             "schedule": SCHEDULES[sched], "autotune_ms_per_pass": ms}
 
 
-def cfg0_leg(args):
+def n8000_leg(args):
     """The CPU configuration: N = 8000, rho* = 0.30, T* = 1.2, no colloid, 20^3 simple-cubic
     lattice (spacing 1.4938), orientations uniform; 300 untimed sweeps at 0.6 / 0.6 remove the
-    lattice's capped overlaps, then `cfg0_sweeps` sweeps at 0.1 / 0.1.  The FP64 oracle on the
+    lattice's capped overlaps, then `n8000_sweeps` sweeps at 0.1 / 0.1.  The FP64 oracle on the
     host cores (OpenMP over the independent cells of a pass) against the GPU in FP64 (the same
     chain: its E/N series equals the oracle's) and FP32 (a different chain of the same
     distribution: the means compared within 2 combined errors)."""
@@ -439,7 +440,7 @@ def cfg0_leg(args):
     uu /= np.linalg.norm(uu, axis=1, keepdims=True)
     u = [uu[:, d].copy() for d in range(3)]
     box = np.full(3, L)
-    S = args.cfg0_sweeps
+    S = args.n8000_sweeps
     out = {"workload": "N=8000 rho*=0.30 T*=1.2, no colloid, 20^3 lattice start",
            "relax_sweeps": 300, "sweeps": S}
     # the GPU, both precisions, from the same start
@@ -456,7 +457,7 @@ def cfg0_leg(args):
                 qstart = ctx.get_state_fixed()
                 t_start = int(ctx.stats()["sweeps"])
             ctx.sweep(2)
-            dt, st, e, p2, _ = timed(ctx, S, label=f"cfg0 gpu fp{prec}")
+            dt, st, e, p2, _ = timed(ctx, S, label=f"n8000 gpu fp{prec}")
         out[f"gpu_fp{prec}"] = {"moves_per_s": st["attempted"] / dt, "seconds": dt,
                                 "E/N": series_report(e, dt), "P2": series_report(p2, dt)}
         if prec == 64:
@@ -495,7 +496,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-da", action="store_true", help="skip the delayed-acceptance leg")
     ap.add_argument("--no-ess", action="store_true", help="skip the plain ESS leg")
-    ap.add_argument("--no-cfg0", action="store_true", help="skip the N=8000 CPU-configuration leg")
+    ap.add_argument("--no-n8000", action="store_true", help="skip the N=8000 CPU-configuration leg")
     ap.add_argument("--cpu-sweeps", type=int, default=10)
     ap.add_argument("--n", type=int, default=256000)
     ap.add_argument("--relax-big", type=int, default=300,
@@ -522,7 +523,7 @@ def main():
     ap.add_argument("--no-npt", action="store_true", help="skip the NPT leg")
     ap.add_argument("--npt-sweeps", type=int, default=8000)
     ap.add_argument("--no-replicas", action="store_true", help="skip the 32-replica leg")
-    ap.add_argument("--cfg0-sweeps", type=int, default=2000)
+    ap.add_argument("--n8000-sweeps", type=int, default=2000)
     ap.add_argument("--colors", type=int, default=2, choices=(2, 3),
                     help="sublattice colours per axis of the plain leg "
                          "(2: 8 sublattices of cells >= r_c; 3: 27 of cells >= r_c/2)")
@@ -637,8 +638,8 @@ def main():
         out["npt"] = npt_leg(args, tune_w or 2, tune_s)
     if not args.no_replicas:
         out["replicas"] = replicas_leg(args)
-    if not args.no_cfg0:
-        out["cfg0"] = cfg0_leg(args)
+    if not args.no_n8000:
+        out["n8000"] = n8000_leg(args)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(pos, u, box, N, rho, T, R, seed, sweeps=args.cpu_sweeps)
     print(json.dumps(out))

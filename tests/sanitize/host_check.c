@@ -42,7 +42,7 @@ int main(int argc, char **argv)
     double dv[11];
 
     /* loader: good files, unknown key (a warning), missing keys, malformed value, no file */
-    const char *good[] = {"params_cfg0.txt", "params_cfg1.txt", "params_unknown.txt"};
+    const char *good[] = {"params_n8000.txt", "params_n256000_colloid.txt", "params_unknown.txt"};
     for (int k = 0; k < 3; ++k) {
         char *f = join(fix, good[k]);
         EXPECT(mcgb_params_load(f, &p) == MCGB_OK);

@@ -1,13 +1,13 @@
 """The benchmark configurations with a colloid, run through the C ABI against the FP64 oracle.
 
-* cfg1 (plain, N = 256000, rho* = 0.32, T* = 1.0, colloid R = 5, the library's jittered
-  lattice start with every site within 5.5 of the colloid removed):
+* the benchmark's colloid workload (plain, N = 256000, rho* = 0.32, T* = 1.0, colloid R = 5, the
+  library's jittered lattice start with every site within 5.5 of the colloid removed):
   - the same geometry at N = 20000: FP64 sweeps move by move against the oracle, and an FP32
     sweep replayed move by move in FP64 with the distribution of its |dE|-relative errors;
   - at full size: the cell list bit-exact against the oracle, and after six sweeps the capped
     pair and colloid counts exact and the finite running energy against the FP64 recompute;
   - at eight times the full size (N = 2048000, layered start), FP32 and FP64: the same checks.
-* cfg4 (delayed acceptance with the colloid at T* = 0.8), at reduced N: a zero network gives
+* delayed acceptance with the R = 5 colloid at T* = 0.8, at reduced N: a zero network gives
   the plain chain bit for bit with the colloid term in stage 1; a poor network samples the same
   <E>/N and P2 as the plain sweep (2 sigma, autocorrelation-corrected).
 """
@@ -31,8 +31,8 @@ def need_gpu():
         pytest.fail("no GPU visible to HIP: the gpu tests need an MI355X")
 
 
-def cfg1(N, precision, T=1.0, seed=20261203, **kw):
-    """Context holding the cfg1 start (library generator, as the benchmark), its fixed-point
+def colloid_lattice(N, precision, T=1.0, seed=20261203, **kw):
+    """Context holding the colloid lattice start (library generator, as the benchmark), its fixed-point
     state, and the oracle constants."""
     p = m.gb_params(N, RHO, T, colloid_radius=R, precision=precision, seed=seed,
                     lattice_jitter=JITTER, colloid_exclusion_radius=EXCL,
@@ -45,8 +45,8 @@ def cfg1(N, precision, T=1.0, seed=20261203, **kw):
     return ctx, c, seed
 
 
-def test_cfg1_reduced_fp64_sweeps_match_oracle_move_by_move():
-    ctx, c, seed = cfg1(20000, 64)
+def test_colloid_lattice_n20000_fp64_sweeps_match_oracle_move_by_move():
+    ctx, c, seed = colloid_lattice(20000, 64)
     with ctx:
         q, u, L = ctx.get_state_fixed()
         s = oc.System(c, q, u, L, seed=seed, nthreads=8)
@@ -54,7 +54,7 @@ def test_cfg1_reduced_fp64_sweeps_match_oracle_move_by_move():
         assert st["cap_pairs"] == s.energy[0] and st["cap_colloid"] == s.energy[1]
         assert s.energy[0] > 0                # the lattice start holds capped overlaps
         for sweep in range(2):
-            print(f"cfg1 N=20000 FP64: traced sweep {sweep}", flush=True)
+            print(f"colloid lattice N=20000 FP64: traced sweep {sweep}", flush=True)
             tr_g, info_g = ctx.sweep_traced()
             tr_o, info_o, _ = s.sweep(trace=True)
             assert np.array_equal(info_g, info_o)
@@ -75,7 +75,7 @@ def test_cfg1_reduced_fp64_sweeps_match_oracle_move_by_move():
         assert st["cap_pairs"] == s.energy[0] and st["cap_colloid"] == s.energy[1]
 
 
-def test_cfg1_reduced_fp32_replay_and_relative_error_distribution():
+def test_colloid_lattice_n20000_fp32_replay_and_relative_error_distribution():
     """Every move of an FP32 sweep replayed in FP64 from the same state: decisions follow, cap
     counts are exact, and the distribution of the |dE|-relative error is reported against the
     move's cancellation factor kappa = sum|terms| / |dE|.
@@ -86,14 +86,14 @@ def test_cfg1_reduced_fp32_replay_and_relative_error_distribution():
     most 1e-5 kappa relative to |dE| itself: 1e-5 relative wherever old and trial energies do
     not cancel (kappa <= 1); moves with kappa > 1 are the cancellation cases, excluded from the
     plain 1e-5 |dE|-relative bound and held to 1e-5 kappa."""
-    ctx, c, seed = cfg1(20000, 32)
+    ctx, c, seed = colloid_lattice(20000, 32)
     with ctx:
         ctx.sweep(40)                          # relax the lattice start a little
         q0, u0, L = ctx.get_state_fixed()
         t = ctx.stats()["sweeps"]
         tr, info = ctx.sweep_traced()
         got = []
-        print("cfg1 N=20000 FP32: replaying one sweep", flush=True)
+        print("colloid lattice N=20000 FP32: replaying one sweep", flush=True)
         qf, uf, stt = replay_sweep(c, q0, u0, L, seed, 0, t, tr, info, check_tol=1e-5,
                                    fp32=True, collect=got, max_scale=NEAR_CAP)
         q1, u1, _ = ctx.get_state_fixed()
@@ -106,7 +106,7 @@ def test_cfg1_reduced_fp32_replay_and_relative_error_distribution():
     # condition-aware bound with the FP32 rounding of sigma, ~1e-7
     far = a[:, 2] > NEAR_CAP
     assert np.all(np.abs(a[far, 0] - a[far, 1]) <= cond_tol(1e-7, a[far, 2]))
-    print(f"cfg1 FP32: {far.sum()} near-cap moves within the condition-aware bound", flush=True)
+    print(f"colloid lattice FP32: {far.sum()} near-cap moves within the condition-aware bound", flush=True)
     a = a[~far]
     dg, dr, scale = a[:, 0], a[:, 1], a[:, 2]
     err = np.abs(dg - dr)
@@ -114,7 +114,7 @@ def test_cfg1_reduced_fp32_replay_and_relative_error_distribution():
     good = kappa <= 100.0
     rel = err[good] / np.abs(dr[good])
     pct = np.percentile(rel, [50, 90, 99, 100])
-    print(f"cfg1 FP32 |dE|-relative error over {good.sum()} of {len(a)} moves (kappa <= 100): "
+    print(f"colloid lattice FP32 |dE|-relative error over {good.sum()} of {len(a)} moves (kappa <= 100): "
           f"median {pct[0]:.2e}, 90% {pct[1]:.2e}, 99% {pct[2]:.2e}, max {pct[3]:.2e}; "
           f"excluded (kappa > 100): {(~good).sum()}", flush=True)
     for lo, hi in ((0, 10), (10, 100), (100, 1e3), (1e3, 1e4), (1e4, np.inf)):
@@ -131,9 +131,9 @@ def test_cfg1_reduced_fp32_replay_and_relative_error_distribution():
     assert pct[0] <= 1e-5                                     # typical move (kappa <= 100)
 
 
-def test_cfg1_full_size_cell_list_caps_and_energy():
+def test_colloid_lattice_n256000_cell_list_caps_and_energy():
     N = 256000
-    ctx, c, seed = cfg1(N, 32)
+    ctx, c, seed = colloid_lattice(N, 32)
     with ctx:
         q, u, L = ctx.get_state_fixed()
         n = oc.cell_dim(L[0], 4.0)
@@ -171,7 +171,7 @@ def test_cfg1_full_size_cell_list_caps_and_energy():
 
 @pytest.mark.parametrize("precision", [32, 64])
 def test_eight_times_the_full_size(precision):
-    """N = 2048000 at the cfg1 density with the R = 5 colloid (a 46^3 grid, 12167 active cells
+    """N = 2048000 at the benchmark's density with the R = 5 colloid (a 46^3 grid, 12167 active cells
     per pass, 8x the benchmark's state), from a layered start: the cell lists bit-exact against
     the oracle, and after three sweeps of the two-wavefront sequential kernel the cap counts
     exact and the running energy equal to the FP64 recompute."""
@@ -202,9 +202,9 @@ def test_eight_times_the_full_size(precision):
         assert np.array_equal(cs, cs_o) and np.array_equal(ids, so_o)
 
 
-# ------------------------------------------------------------------ cfg4: delayed acceptance
+# ------------------------------------------------------------------ delayed acceptance, colloid, T* = 0.8
 
-def cfg4_pair(N, precision, seed=5, pseed=91):
+def da_colloid_pair(N, precision, seed=5, pseed=91):
     T = 0.8
     q, u, L = layered_system(N, RHO, colloid_radius=R, seed=seed)
     out = []
@@ -218,8 +218,8 @@ def cfg4_pair(N, precision, seed=5, pseed=91):
 
 
 @pytest.mark.parametrize("precision", [64, 32])
-def test_cfg4_zero_network_with_the_colloid_is_the_plain_chain(precision):
-    a, b = cfg4_pair(6000, precision)
+def test_da_colloid_zero_network_is_the_plain_chain(precision):
+    a, b = da_colloid_pair(6000, precision)
     with a, b:
         b.surrogate_set(np.zeros(m.SUR_NPARAM, dtype=np.float32))
         b.da_enable(True)
@@ -248,11 +248,11 @@ def series_stats(ctx, sweeps):
     return out
 
 
-def test_cfg4_poor_network_samples_the_plain_distribution():
+def test_da_colloid_poor_network_samples_the_plain_distribution():
     """An untrained network (random weights) as stage 1 changes only the efficiency: <E>/N and
     P2 of the delayed-acceptance chain agree with the plain chain's within 2 combined standard
     errors (tau_int-corrected), with the colloid at T* = 0.8."""
-    a, b = cfg4_pair(4000, 32)
+    a, b = da_colloid_pair(4000, 32)
     with a, b:
         b.surrogate_init(21)
         b.da_enable(True)
@@ -262,7 +262,7 @@ def test_cfg4_poor_network_samples_the_plain_distribution():
         (eb, seb, tb), (pb, spb, tpb) = series_stats(b, 6000)
         zE = (ea - eb) / np.hypot(sea, seb)
         zP = (pa - pb) / np.hypot(spa, spb)
-        print(f"cfg4: plain E/N {ea:.4f}+-{sea:.4f} (tau {ta:.0f}) P2 {pa:.4f}+-{spa:.4f} "
+        print(f"delayed acceptance, colloid: plain E/N {ea:.4f}+-{sea:.4f} (tau {ta:.0f}) P2 {pa:.4f}+-{spa:.4f} "
               f"(tau {tpa:.0f}); DA E/N {eb:.4f}+-{seb:.4f} (tau {tb:.0f}) P2 {pb:.4f}+-{spb:.4f} "
               f"(tau {tpb:.0f}); z {zE:+.2f} {zP:+.2f}", flush=True)
         assert abs(zE) < 2.0 and abs(zP) < 2.0

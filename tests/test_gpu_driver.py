@@ -18,7 +18,7 @@ pytestmark = pytest.mark.gpu
 
 
 def _params(tmp_path, extra="", **over):
-    txt = open(os.path.join(GOLD, "params_cfg0.txt")).read()
+    txt = open(os.path.join(GOLD, "params_n8000.txt")).read()
     for k, v in over.items():
         txt = re.sub(r"(?m)^%s\s+\S+" % k, "%s %s" % (k, v), txt)
     txt += extra

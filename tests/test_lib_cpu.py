@@ -25,7 +25,7 @@ def test_library_exports_every_declared_symbol():
         assert hasattr(L, n), n
 
 
-@pytest.mark.parametrize("fname", ["params_cfg0.txt", "params_cfg1.txt", "params_unknown.txt"])
+@pytest.mark.parametrize("fname", ["params_n8000.txt", "params_n256000_colloid.txt", "params_unknown.txt"])
 def test_loader_matches_reference_loader(fname, capfd):
     ref = LOADER["loader"][fname]["values"]
     p = m.load_params(os.path.join(GOLD, fname))
@@ -197,7 +197,7 @@ def test_driver_reports_missing_parameter_file(tmp_path):
 
 
 def _driver_params(tmp_path, **over):
-    txt = open(os.path.join(GOLD, "params_cfg0.txt")).read()
+    txt = open(os.path.join(GOLD, "params_n8000.txt")).read()
     for k, v in over.items():
         txt = re.sub(r"(?m)^%s\s+\S+" % k, "%s %s" % (k, v), txt)
     path = tmp_path / "parametersFile.txt"

@@ -136,7 +136,7 @@ def test_gb_pair_matches_live_reference_random():
             assert u[m] == pytest.approx(ref, rel=1e-13, abs=1e-13)
 
 
-@pytest.mark.parametrize("fname", ["params_cfg0.txt", "params_cfg1.txt", "params_unknown.txt"])
+@pytest.mark.parametrize("fname", ["params_n8000.txt", "params_n256000_colloid.txt", "params_unknown.txt"])
 def test_reference_loader_parses_fixture_files(fname):
     # what the reference's own loader read from our fixture files (same key format)
     rec = LOADER["loader"][fname]

@@ -64,7 +64,7 @@ def test_driver_under_sanitizers(tmp_path):
                        cwd=tmp_path, timeout=120)
     ok, text = clean(r)
     assert r.returncode == 1 and ok and "Error opening the parameters file" in r.stderr, text
-    txt = open(os.path.join(GOLD, "params_cfg0.txt")).read()
+    txt = open(os.path.join(GOLD, "params_n8000.txt")).read()
     for k, v in (("num_particles", 1000), ("system_density", 0.02),
                  ("initial_configuration_type", 2), ("total_mc_cycles", 0)):
         txt = re.sub(r"(?m)^%s\s+\S+" % k, "%s %s" % (k, v), txt)

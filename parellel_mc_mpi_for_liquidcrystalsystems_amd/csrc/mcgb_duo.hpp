@@ -124,9 +124,10 @@ __device__ __forceinline__ void pass_cell_duo(const PassArgs& P, int b, unsigned
     const MoveDraw* __restrict__ drw = P.draws + C.rec_off + tbase;
     const uint2* __restrict__ drw2 = (const uint2*)drw;
     constexpr int kW2 = (int)(sizeof(MoveDraw) / 8);
-    static_assert(sizeof(MoveDraw) % 8 == 0 && KD * kW2 <= 64, "a chunk's draws fit one wavefront");
-    uint2 dpre = make_uint2(0u, 0u);
+    static_assert(sizeof(MoveDraw) % 8 == 0 && KD * kW2 <= 128, "a chunk's draws fit two words per lane");
+    uint2 dpre = make_uint2(0u, 0u), dpre_b = make_uint2(0u, 0u);   // words lane, lane + 64
     if (w == 0 && lane < min(k, KD) * kW2) dpre = drw2[lane];
+    if (w == 0 && lane + 64 < min(k, KD) * kW2) dpre_b = drw2[lane + 64];
 
     // stencil table (as k_pass)
     const int RS = col - 1, S1 = 2 * RS + 1, NS = S1 * S1 * S1;
@@ -250,7 +251,10 @@ __device__ __forceinline__ void pass_cell_duo(const PassArgs& P, int b, unsigned
             MovePrep<Real> mp = {};
             if (w == 0) {
                 if (lane < min(k - m0, KD) * kW2) sdraw2[lane] = dpre;
+                if (lane + 64 < min(k - m0, KD) * kW2) sdraw2[lane + 64] = dpre_b;
                 if (lane < min(k - m0 - KD, KD) * kW2) dpre = drw2[(size_t)(m0 + KD) * kW2 + lane];
+                if (lane + 64 < min(k - m0 - KD, KD) * kW2)
+                    dpre_b = drw2[(size_t)(m0 + KD) * kW2 + lane + 64];
                 __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
                 __builtin_amdgcn_wave_barrier();
                 __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");

@@ -3,8 +3,8 @@
 // attempt_monte_carlo_move, :2445-2604).
 //
 // Per activation: the stencil ((2R+1)^3 cell segments of the device cell list, R = 1 for 8
-// sublattices, 2 for 27) is staged in LDS.  The moves run in chunks of 9: the chunk's draws
-// (written by k_count_draws) arrive from HBM one chunk ahead, and 9 lanes prepare one record per
+// sublattices, 2 for 27) is staged in LDS.  The moves run in chunks of 16: the chunk's draws
+// (written by k_count_draws) arrive from HBM one chunk ahead, and 16 lanes prepare one record per
 // move (draws, the particle's state, trial orientation, trial colloid term, the leave-cell and
 // colloid-core tests, the pre-filter radius).  Per move: each
 // wavefront tests its share of the stencil against the old and trial positions (FP32
@@ -136,7 +136,7 @@ constexpr uint32_t kPrepLeft = 1u << 17, kPrepHard = 1u << 18;
 
 template <typename Real, int W>
 struct PassLds {
-    static constexpr int kDraws = 9;                 // moves whose draws are staged at a time (9 x 7 8-byte words fill one wavefront)
+    static constexpr int kDraws = 16;                // moves whose draws are staged at a time (16 x 7 8-byte words: two per lane of one wavefront)
     __host__ __device__ static inline int list_cap(int cap) {
         return ((cap + 64 * W - 1) / (64 * W)) * 64;
     }
@@ -526,9 +526,10 @@ __device__ __forceinline__ void pass_cell(const PassArgs& P, int b, unsigned cha
     const MoveDraw* __restrict__ drw = P.draws + C.rec_off + tbase;
     const uint2* __restrict__ drw2 = (const uint2*)drw;
     constexpr int kW2 = (int)(sizeof(MoveDraw) / 8);          // 8-byte words per draw
-    static_assert(sizeof(MoveDraw) % 8 == 0 && KD * kW2 <= 64, "a chunk's draws fit one wavefront");
-    uint2 dpre = make_uint2(0u, 0u);
+    static_assert(sizeof(MoveDraw) % 8 == 0 && KD * kW2 <= 128, "a chunk's draws fit two words per lane");
+    uint2 dpre = make_uint2(0u, 0u), dpre_b = make_uint2(0u, 0u);   // words lane, lane + 64
     if (w == 0 && lane < min(k, KD) * kW2) dpre = drw2[lane];
+    if (w == 0 && lane + 64 < min(k, KD) * kW2) dpre_b = drw2[lane + 64];
 
     // pacing of the sequential pass: the cell timed on the 100 MHz clock from the start of its
     // workgroup (a slow staging counts against it), the pace set by the last sweep's end fold
@@ -677,7 +678,10 @@ __device__ __forceinline__ void pass_cell(const PassArgs& P, int b, unsigned cha
                 // this chunk's draws into LDS (read back below by the same wavefront), the
                 // next chunk's loads issued
                 if (lane < min(k - m0, KD) * kW2) sdraw2[lane] = dpre;
+                if (lane + 64 < min(k - m0, KD) * kW2) sdraw2[lane + 64] = dpre_b;
                 if (lane < min(k - m0 - KD, KD) * kW2) dpre = drw2[(size_t)(m0 + KD) * kW2 + lane];
+                if (lane + 64 < min(k - m0 - KD, KD) * kW2)
+                    dpre_b = drw2[(size_t)(m0 + KD) * kW2 + lane + 64];
                 __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
                 __builtin_amdgcn_wave_barrier();
                 __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");

@@ -41,7 +41,7 @@ F_CAND, F_PAIR, F_COLL = 17.0, 68.0, 45.0
 # HBM bytes per launch of the W=2 pass kernel on this workload: rocprofv3 FETCH_SIZE x 2.000 +
 # WRITE_SIZE x 1.000 (the factors a plain 256 MiB float32 copy reads), from the two counter
 # passes summarised in profiles/sweep_fp32_N256000_colloid_hbm_traffic.md
-TRAFFIC_PASS_BYTES = 28.35e6
+TRAFFIC_PASS_BYTES = 29.74e6
 # pass schedules (mcgb_params.pass_schedule)
 SCHEDULES = ("sequential", "speculative", "sequential_4waves", "two_moves_in_flight")
 # step sizes that move the lattice start's capped overlaps out of the cap (untimed)
